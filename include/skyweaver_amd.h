@@ -1,0 +1,171 @@
+/*
+ * skyweaver_amd.h -- C ABI of the MI355X (gfx950) implementation of the
+ * skyweaver BeamformerPipeline hot path.
+ *
+ * Plain C: opaque handles, POD descriptors, raw device pointers and sizes.
+ * Every entry point returns SW_OK (0) or a negative SW_ERR_* code; the
+ * message of the last failure on the calling thread is available from
+ * sw_last_error().  No exception crosses this boundary.
+ *
+ * Data layouts are the reference's (slowest -> fastest dimension), complex
+ * int8 stored as interleaved (re, im) bytes:
+ *   TAFTP  [T/256][A_in][F][256][P]     input voltages
+ *   FTPA   [F][T][P][A]                 transposed / dedispersed voltages
+ *   TFB    [T/ts][F][B]                 coherent beams (x4 bytes for IQUV)
+ *   BTF    [S][T/ts][F]                 incoherent beam per beamset
+ *   FPA    [F][P][A]                    statistics {mean, std, skew, kurt}
+ *   TDB    [T'][D][B]                   incoherently dedispersed beams
+ * Streams are hipStream_t passed as void* (NULL = default stream).
+ *
+ * Reference paths below are relative to the skyweaver repository root
+ * (sw/ = cpp/skyweaver/).
+ */
+#ifndef SKYWEAVER_AMD_H
+#define SKYWEAVER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SW_OK 0
+#define SW_ERR_INVALID -1     /* bad argument / shape (std::runtime_error in the reference) */
+#define SW_ERR_HIP -2         /* HIP runtime error (CUDA_ERROR_CHECK in the reference) */
+#define SW_ERR_ALLOC -3       /* device allocation failed */
+#define SW_ERR_IO -4          /* file open / read / write failure */
+#define SW_ERR_DELAY_EPOCH -5 /* no valid delay model for the epoch (InvalidDelayEpoch) */
+#define SW_ERR_EOF -6         /* end of delay-model file */
+#define SW_ERR_UNSUPPORTED -7 /* valid request this build does not implement */
+
+enum sw_stokes {
+    SW_STOKES_I = 0,
+    SW_STOKES_Q = 1,
+    SW_STOKES_U = 2,
+    SW_STOKES_V = 3,
+    SW_STOKES_IQUV = 4
+};
+
+/* Coherent-beamformer kernel forms. */
+enum sw_cb_form {
+    SW_CB_PLAIN = 0, /* VALU v_dot4 int8 dot products (the reference's dp4a form) */
+    SW_CB_MFMA  = 1  /* v_mfma_i32_32x32x32_i8 complex GEMM with fused epilogue */
+};
+
+/* Copies at most len-1 bytes of the last error message of this thread. */
+int sw_last_error(char* buf, size_t len);
+const char* sw_version(void);
+
+/* ------------------------------------------------ device memory and streams
+ * Thin wrappers over the HIP runtime this library is linked against, so a
+ * caller (a test harness, a host language without HIP bindings) needs no
+ * second GPU runtime in its process.  No reference counterpart: the
+ * reference allocates through thrust containers (sw/DescribedVector.hpp). */
+int sw_device_count(int* count);
+int sw_malloc(void** d_ptr, size_t bytes);
+int sw_free(void* d_ptr);
+int sw_host_alloc(void** h_ptr, size_t bytes); /* pinned host memory */
+int sw_host_free(void* h_ptr);
+int sw_memcpy_htod(void* d_dst, const void* h_src, size_t bytes, void* stream);
+int sw_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes, void* stream);
+int sw_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes, void* stream);
+int sw_memset(void* d_ptr, int value, size_t bytes, void* stream);
+int sw_stream_create(void** stream); /* default priority */
+int sw_stream_destroy(void* stream);
+int sw_stream_synchronize(void* stream);
+int sw_event_create(void** event);
+int sw_event_destroy(void* event);
+int sw_event_record(void* event, void* stream);
+int sw_event_elapsed_ms(float* ms, void* start, void* stop);
+
+/* ---------------------------------------------------------------- Transposer
+ * Transposer::transpose(TAFTPVoltagesD const&, FTPAVoltagesD&, size_t
+ * input_nantennas, cudaStream_t), sw/Transposer.cuh:66-69,
+ * sw/src/Transposer.cu:117-184.  Antennas >= nant_in are written as zeros. */
+int sw_transpose(const void* d_taftp, void* d_ftpa, int nheap_groups,
+                 int nant_in, int nant_out, int nchans, void* stream);
+
+/* ------------------------------------------------------ StatisticsCalculator
+ * StatisticsCalculator::calculate_statistics(FTPAVoltagesD const&),
+ * sw/StatisticsCalculator.cuh:65, sw/src/StatisticsCalculator.cu:54-130.
+ * d_stats: FPA of struct {float mean, std, skew, kurtosis}.
+ * d_workspace: sw_statistics_workspace_size() bytes of device scratch, or
+ * NULL to use a library-owned buffer. */
+size_t sw_statistics_workspace_size(int nchans, int npol, int nantennas);
+int sw_statistics(const void* d_ftpa, float* d_stats, int nchans, int nsamples,
+                  int npol, int nantennas, void* d_workspace, void* stream);
+
+/* StatisticsCalculator::update_scalings(beamset_weights, nbeamsets),
+ * sw/src/StatisticsCalculator.cu:132-239 (host arithmetic).  All arrays are
+ * host memory; outputs are [nbeamsets][nchans], indexed beamset*F + f. */
+int sw_update_scalings(const float* h_stats, const float* h_beamset_weights,
+                       int nbeamsets, int nchans, int npol, int nantennas,
+                       int cb_tscrunch, int ib_tscrunch, float output_level,
+                       float* h_cb_offsets, float* h_cb_scaling,
+                       float* h_ib_offsets, float* h_ib_scaling);
+
+/* ------------------------------------------------------ beamformer dimensions */
+typedef struct sw_bf_dims {
+    int nchans;      /* F */
+    int nsamples;    /* T (input samples per channel) */
+    int nantennas;   /* A, padded antenna count of the FTPA buffer, multiple of 4 */
+    int nbeams;      /* B */
+    int tscrunch;    /* CB and IB time scrunch (one value, as the reference requires) */
+    int stokes;      /* enum sw_stokes */
+    int nbeamsets;   /* S >= 1 */
+    int ib_subtract; /* SKYWEAVER_IB_SUBTRACTION: 1 = subtract the IB from the CB */
+} sw_bf_dims;
+
+/* ------------------------------------------------------ IncoherentBeamformer
+ * IncoherentBeamformer::beamform(input, output_raw, output, output_scale,
+ * output_offset, antenna_weights, nbeamsets, stream),
+ * sw/IncoherentBeamformer.cuh:84-91, sw/src/IncoherentBeamformer.cu:16-183.
+ * Per-beamset antenna weights [S][A] are honoured for every beamset.
+ * d_raw: BTF float (float4 for IQUV); d_out: BTF int8 (char4 for IQUV);
+ * d_scale / d_offset: [S][F]; d_ant_weights: [S][A]. */
+int sw_ib_beamform(const void* d_ftpa, float* d_raw, void* d_out,
+                   const float* d_scale, const float* d_offset,
+                   const float* d_ant_weights, const sw_bf_dims* dims,
+                   void* stream);
+
+/* -------------------------------------------------------- CoherentBeamformer
+ * CoherentBeamformer::beamform(input, weights, output_scale, output_offset,
+ * beamset_mapping, ib_powers, output, nbeamsets, stream),
+ * sw/CoherentBeamformer.cuh:84-92, sw/src/CoherentBeamformer.cu:13-269.
+ * d_weights: [F][B][A] complex int8; d_scale / d_offset: [S][F];
+ * d_beamset_map: [B] int; d_ib_raw: BTF from sw_ib_beamform of the same
+ * input; d_out: TFB int8 (char4 for IQUV).
+ * form = enum sw_cb_form.  The MFMA form repacks the weights into
+ * d_workspace (sw_cb_workspace_size() bytes; NULL = library-owned). */
+size_t sw_cb_workspace_size(const sw_bf_dims* dims);
+int sw_cb_beamform(const void* d_ftpa, const void* d_weights,
+                   const float* d_scale, const float* d_offset,
+                   const int* d_beamset_map, const float* d_ib_raw,
+                   void* d_out, const sw_bf_dims* dims, int form,
+                   void* d_workspace, void* stream);
+
+/* The MFMA form in two steps: repack [F][B][A] weights once per weight
+ * update (WeightsManager::weights, sw/src/WeightsManager.cu:110-139 is where
+ * the reference produces them), then beamform from the packed buffer. */
+int sw_cb_pack_weights(const void* d_weights, void* d_packed, int nchans,
+                       int nbeams, int nantennas, void* stream);
+int sw_cb_beamform_packed(const void* d_ftpa, const void* d_packed,
+                          const float* d_scale, const float* d_offset,
+                          const int* d_beamset_map, const float* d_ib_raw,
+                          void* d_out, const sw_bf_dims* dims, void* stream);
+
+/* -------------------------------------------------- synthetic input (bench)
+ * Seeded Gaussian complex int8: xoshiro256** seeded through splitmix64,
+ * Box-Muller in FP64, value = clip(round_half_away(sigma*z), -127, 127).
+ * Stream k of the generator starts at k xoshiro256** jumps (2^128 steps)
+ * from the seeded state; chunk c of 2^20 complex values uses stream c. */
+int sw_synth_gaussian_c8(int8_t* h_out, size_t ncomplex, uint64_t seed,
+                         double sigma, int nthreads);
+/* Random-phase weights (lround(127 cos th), lround(127 sin th)), th ~ U[0, 2pi). */
+int sw_synth_phase_weights(int8_t* h_out, size_t ncomplex, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SKYWEAVER_AMD_H */

@@ -1,0 +1,302 @@
+"""skyweaver_amd -- MI355X (gfx950) implementation of the skyweaver
+BeamformerPipeline hot path.
+
+The product is the C-ABI library ``_build/libskyweaver_amd.so`` (hand-written
+HIP kernels, declared in ``include/skyweaver_amd.h``).  This module is a thin
+ctypes binding used by the tests and the benchmark.  Device memory, streams
+and events also come from the C ABI (one HIP runtime per process), every
+compute call goes through it, and a missing library is an error -- there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libskyweaver_amd.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "skyweaver_amd.h")
+
+SW_OK = 0
+STOKES = {"I": 0, "Q": 1, "U": 2, "V": 3, "IQUV": 4}
+FORMS = {"plain": 0, "mfma": 1}
+
+_lib = None
+
+
+class SkyweaverError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"skyweaver_amd error {code}: {msg}")
+        self.code = code
+
+
+def build(jobs: int = 8) -> str:
+    """Compile the HIP library for gfx950 (hipcc, in-tree)."""
+    subprocess.run(["make", "-s", "-C", _HERE, f"-j{jobs}"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    """Load the C-ABI library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class BfDims(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "nchans", "nsamples", "nantennas", "nbeams", "tscrunch", "stokes", "nbeamsets", "ib_subtract")]
+
+
+_vp = ctypes.c_void_p
+_c = ctypes.c_int
+_sz = ctypes.c_size_t
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def _declare(L):
+    L.sw_last_error.argtypes = [ctypes.c_char_p, _sz]
+    L.sw_last_error.restype = _c
+    L.sw_version.restype = ctypes.c_char_p
+    L.sw_transpose.argtypes = [_vp, _vp, _c, _c, _c, _c, _vp]
+    L.sw_statistics_workspace_size.argtypes = [_c, _c, _c]
+    L.sw_statistics_workspace_size.restype = _sz
+    L.sw_statistics.argtypes = [_vp, _vp, _c, _c, _c, _c, _vp, _vp]
+    L.sw_update_scalings.argtypes = [_vp, _vp, _c, _c, _c, _c, _c, _c, ctypes.c_float, _vp, _vp, _vp, _vp]
+    L.sw_ib_beamform.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(BfDims), _vp]
+    L.sw_cb_workspace_size.argtypes = [ctypes.POINTER(BfDims)]
+    L.sw_cb_workspace_size.restype = _sz
+    L.sw_cb_beamform.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(BfDims), _c, _vp, _vp]
+    L.sw_cb_pack_weights.argtypes = [_vp, _vp, _c, _c, _c, _vp]
+    L.sw_cb_beamform_packed.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(BfDims), _vp]
+    L.sw_synth_gaussian_c8.argtypes = [_vp, _sz, ctypes.c_uint64, ctypes.c_double, _c]
+    L.sw_synth_phase_weights.argtypes = [_vp, _sz, ctypes.c_uint64]
+    _pvp = ctypes.POINTER(ctypes.c_void_p)
+    L.sw_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    L.sw_malloc.argtypes = [_pvp, _sz]
+    L.sw_free.argtypes = [_vp]
+    L.sw_host_alloc.argtypes = [_pvp, _sz]
+    L.sw_host_free.argtypes = [_vp]
+    L.sw_memcpy_htod.argtypes = [_vp, _vp, _sz, _vp]
+    L.sw_memcpy_dtoh.argtypes = [_vp, _vp, _sz, _vp]
+    L.sw_memcpy_dtod.argtypes = [_vp, _vp, _sz, _vp]
+    L.sw_memset.argtypes = [_vp, _c, _sz, _vp]
+    L.sw_stream_create.argtypes = [_pvp]
+    L.sw_stream_destroy.argtypes = [_vp]
+    L.sw_stream_synchronize.argtypes = [_vp]
+    L.sw_event_create.argtypes = [_pvp]
+    L.sw_event_destroy.argtypes = [_vp]
+    L.sw_event_record.argtypes = [_vp, _vp]
+    L.sw_event_elapsed_ms.argtypes = [ctypes.POINTER(ctypes.c_float), _vp, _vp]
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    lib().sw_last_error(buf, 1024)
+    return buf.value.decode()
+
+
+def check(rc: int) -> None:
+    if rc != SW_OK:
+        raise SkyweaverError(rc, last_error())
+
+
+def exported_symbols() -> list:
+    """Names of every function declared in include/skyweaver_amd.h."""
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char\*)\s+(sw_\w+)\s*\(", text, re.M)))
+
+
+# ------------------------------------------------------ memory and streams
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    check(lib().sw_device_count(ctypes.byref(n)))
+    return n.value
+
+
+class DeviceBuffer:
+    """Device allocation owned by this object (freed on close / GC)."""
+
+    def __init__(self, nbytes: int, shape=None, dtype=None):
+        import numpy as np
+        p = ctypes.c_void_p()
+        check(lib().sw_malloc(ctypes.byref(p), int(nbytes)))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+        self.shape = tuple(shape) if shape is not None else (int(nbytes),)
+        self.dtype = np.dtype(dtype) if dtype is not None else np.dtype(np.uint8)
+
+    @classmethod
+    def empty(cls, shape, dtype, fill=None):
+        import numpy as np
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        b = cls(n, shape, dtype)
+        if fill is not None:
+            b.fill_bytes(fill)
+        return b
+
+    @classmethod
+    def from_host(cls, arr, stream=None):
+        import numpy as np
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes, arr.shape, arr.dtype)
+        b.upload(arr, stream)
+        return b
+
+    def upload(self, arr, stream=None):
+        import numpy as np
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(lib().sw_memcpy_htod(self.ptr, arr.ctypes.data, arr.nbytes, _stream(stream)))
+        check(lib().sw_stream_synchronize(_stream(stream)))
+
+    def download(self, stream=None):
+        import numpy as np
+        out = np.empty(self.shape, self.dtype)
+        check(lib().sw_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes, _stream(stream)))
+        check(lib().sw_stream_synchronize(_stream(stream)))
+        return out
+
+    def fill_bytes(self, value: int, stream=None):
+        check(lib().sw_memset(self.ptr, int(value), self.nbytes, _stream(stream)))
+        check(lib().sw_stream_synchronize(_stream(stream)))
+
+    def close(self):
+        if self.ptr:
+            lib().sw_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(lib().sw_stream_create(ctypes.byref(p)))
+        self.handle = p.value
+
+    def synchronize(self):
+        check(lib().sw_stream_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().sw_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(lib().sw_event_create(ctypes.byref(p)))
+        self.handle = p.value
+
+    def record(self, stream=None):
+        check(lib().sw_event_record(self.handle, _stream(stream)))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = ctypes.c_float()
+        check(lib().sw_event_elapsed_ms(ctypes.byref(ms), self.handle, end.handle))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().sw_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def synchronize(stream=None):
+    check(lib().sw_stream_synchronize(_stream(stream)))
+
+
+def _ptr(t) -> int:
+    if t is None:
+        return 0
+    return int(t.ptr) if isinstance(t, DeviceBuffer) else int(t)
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        return 0
+    return int(stream.handle) if isinstance(stream, Stream) else int(stream)
+
+
+def dims(nchans, nsamples, nantennas, nbeams, tscrunch, stokes="I", nbeamsets=1, ib_subtract=True) -> BfDims:
+    return BfDims(nchans, nsamples, nantennas, nbeams, tscrunch, STOKES[stokes] if isinstance(stokes, str) else stokes,
+                  nbeamsets, 1 if ib_subtract else 0)
+
+
+def transpose(taftp, ftpa, nheaps, nant_in, nant_out, nchans, stream=None):
+    check(lib().sw_transpose(_ptr(taftp), _ptr(ftpa), nheaps, nant_in, nant_out, nchans, _stream(stream)))
+
+
+def statistics(ftpa, stats, nchans, nsamples, nantennas, workspace=None, stream=None):
+    check(lib().sw_statistics(_ptr(ftpa), _ptr(stats), nchans, nsamples, 2, nantennas, _ptr(workspace),
+                              _stream(stream)))
+
+
+def ib_beamform(ftpa, raw, out, scale, offset, ant_weights, d: BfDims, stream=None):
+    check(lib().sw_ib_beamform(_ptr(ftpa), _ptr(raw), _ptr(out), _ptr(scale), _ptr(offset), _ptr(ant_weights),
+                               ctypes.byref(d), _stream(stream)))
+
+
+def cb_beamform(ftpa, weights, scale, offset, bmap, ib_raw, out, d: BfDims, form="mfma", workspace=None,
+                stream=None):
+    check(lib().sw_cb_beamform(_ptr(ftpa), _ptr(weights), _ptr(scale), _ptr(offset), _ptr(bmap), _ptr(ib_raw),
+                               _ptr(out), ctypes.byref(d), FORMS[form], _ptr(workspace), _stream(stream)))
+
+
+def cb_workspace_size(d: BfDims) -> int:
+    return int(lib().sw_cb_workspace_size(ctypes.byref(d)))
+
+
+def cb_pack_weights(weights, packed, nchans, nbeams, nantennas, stream=None):
+    check(lib().sw_cb_pack_weights(_ptr(weights), _ptr(packed), nchans, nbeams, nantennas, _stream(stream)))
+
+
+def cb_beamform_packed(ftpa, packed, scale, offset, bmap, ib_raw, out, d: BfDims, stream=None):
+    check(lib().sw_cb_beamform_packed(_ptr(ftpa), _ptr(packed), _ptr(scale), _ptr(offset), _ptr(bmap),
+                                      _ptr(ib_raw), _ptr(out), ctypes.byref(d), _stream(stream)))
+
+
+def update_scalings(stats_np, beamset_weights_np, nbeamsets, cb_tscrunch, ib_tscrunch, output_level):
+    """Host scalings from an FPA statistics array [F][P][A][4] float32."""
+    import numpy as np
+    st = np.ascontiguousarray(stats_np, np.float32)
+    F, P, A = st.shape[:3]
+    bw = np.ascontiguousarray(beamset_weights_np, np.float32)
+    outs = [np.zeros(nbeamsets * F, np.float32) for _ in range(4)]
+    check(lib().sw_update_scalings(st.ctypes.data, bw.ctypes.data, nbeamsets, F, P, A, cb_tscrunch, ib_tscrunch,
+                                   float(output_level), *[o.ctypes.data for o in outs]))
+    return dict(zip(("cb_offsets", "cb_scaling", "ib_offsets", "ib_scaling"), outs))
+
+
+def synth_voltages(shape, seed, sigma=16.0, nthreads=0):
+    """Seeded Gaussian complex int8 host array of `shape` (last axis = re/im)."""
+    import numpy as np
+    out = np.empty(shape, np.int8)
+    assert out.shape[-1] == 2
+    check(lib().sw_synth_gaussian_c8(out.ctypes.data, out.size // 2, seed, float(sigma), nthreads))
+    return out
+
+
+def synth_weights(shape, seed):
+    """Random-phase complex int8 weights of amplitude 127 (last axis = re/im)."""
+    import numpy as np
+    out = np.empty(shape, np.int8)
+    assert out.shape[-1] == 2
+    check(lib().sw_synth_phase_weights(out.ctypes.data, out.size // 2, seed))
+    return out

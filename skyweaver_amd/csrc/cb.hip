@@ -1,0 +1,542 @@
+// cb.hip -- coherent beamformer: complex int8 beam sums over antennas, Stokes
+// detection, time scrunch, incoherent-beam subtraction and int8 requantisation.
+//
+// Reference: CoherentBeamformer::beamform / bf_ftpa_general_k,
+// sw/src/CoherentBeamformer.cu:13-269 (dp4a form); oracle
+// sw/test/src/CoherentBeamformerTester.cu:43-152.  Per (f, b, t, p):
+//   V = sum_a W[f][b][a] * v[f][t][p][a]           (exact integers)
+//   power(t') = sum_{t in window, ascending} Stokes(V_p0, V_p1)   (float)
+//   out[t'][f][b] = clamp(rint((power - ib[s][t'][f]*127^2) / scale[s][f]))
+//   (or rescale with the offset when IB subtraction is off)
+//
+// Two forms of the antenna sum share the float epilogue, whose operation
+// order is the oracle's, so both are bit-identical to it:
+//  * plain: one lane per beam, four v_dot4_i32_i8 accumulators
+//    (re*re, im*im, re*im, im*re) per 4 antennas -- the reference's dp4a
+//    arithmetic on VALU, one DM per launch;
+//  * mfma: v_mfma_i32_32x32x32_i8 with time samples as rows and beams as
+//    columns.  K is [re | im] of the antennas (K' = 2A):
+//        Re = sum_a vr*wr + vi*(~wi) + sum_a vi      (-x == ~x + 1, no overflow)
+//        Im = sum_a vr*wi + vi*wr
+//    The per-row correction sum_a vi is itself an MFMA against a ones
+//    operand and seeds the Re accumulator.  Weights are repacked once per
+//    weight update into [F][B][2][2A] rows ([wr | ~wi], [wi | wr]).
+#include "sw_common.hpp"
+
+namespace sw {
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint32_t re_bytes(uint32_t d0, uint32_t d1)
+{
+    return __builtin_amdgcn_perm(d1, d0, 0x06040200u); // d0.b0 d0.b2 d1.b0 d1.b2
+}
+__device__ __forceinline__ uint32_t im_bytes(uint32_t d0, uint32_t d1)
+{
+    return __builtin_amdgcn_perm(d1, d0, 0x07050301u); // d0.b1 d0.b3 d1.b1 d1.b3
+}
+
+// ------------------------------------------------------------ epilogue
+template <int S>
+struct Epi {
+    static constexpr int NV = nstokes(S);
+    // Stokes of one sample from the two polarisations' complex beam sums.
+    __device__ __forceinline__ static void detect(float r0, float i0, float r1, float i1,
+                                                  float* st)
+    {
+        if constexpr(NV == 1) {
+            st[0] = stokes<S>(r0, i0, r1, i1);
+        } else {
+            st[0] = stokes<SW_STOKES_I>(r0, i0, r1, i1);
+            st[1] = stokes<SW_STOKES_Q>(r0, i0, r1, i1);
+            st[2] = stokes<SW_STOKES_U>(r0, i0, r1, i1);
+            st[3] = stokes<SW_STOKES_V>(r0, i0, r1, i1);
+        }
+    }
+    // Requantise and store one output (t', f, b).
+    __device__ __forceinline__ static void store(const float* power, int8_t* out,
+                                                 const float* ib_raw, const float* scale,
+                                                 const float* offset, const int* bmap,
+                                                 int tout, int f, int b, int nchans,
+                                                 int nbeams, int nsamps_out, int ib_sub)
+    {
+        const int s     = bmap[b];
+        const int so    = s * nchans + f;
+        const float sc  = scale[so];
+        const size_t oi = ((size_t)tout * nchans + f) * nbeams + b;
+        const size_t ii = ((size_t)s * nsamps_out + tout) * nchans + f;
+        if constexpr(NV == 1) {
+            float r;
+            if(ib_sub)
+                r = ib_subtract(power[0], ib_raw[ii], sc);
+            else
+                r = rescale<S>(power[0], offset[so], sc);
+            out[oi] = clamp_i8(r);
+        } else {
+            const float4 ib = reinterpret_cast<const float4*>(ib_raw)[ii];
+            float r[4];
+            if(ib_sub) {
+                r[0] = ib_subtract(power[0], ib.x, sc);
+                r[1] = ib_subtract(power[1], ib.y, sc);
+                r[2] = ib_subtract(power[2], ib.z, sc);
+                r[3] = ib_subtract(power[3], ib.w, sc);
+            } else {
+                const float off = offset[so];
+                r[0] = rescale<SW_STOKES_I>(power[0], off, sc);
+                r[1] = rescale<SW_STOKES_Q>(power[1], off, sc);
+                r[2] = rescale<SW_STOKES_U>(power[2], off, sc);
+                r[3] = rescale<SW_STOKES_V>(power[3], off, sc);
+            }
+            char4 q;
+            q.x = clamp_i8(r[0]);
+            q.y = clamp_i8(r[1]);
+            q.z = clamp_i8(r[2]);
+            q.w = clamp_i8(r[3]);
+            reinterpret_cast<char4*>(out)[oi] = q;
+        }
+    }
+};
+
+// ======================================================== plain (v_dot4) form
+constexpr int kPlainThreads = 256;  // 4 waves x 64 beams
+constexpr int kPlainTB      = 64;   // samples staged per LDS pass
+
+template <int S, int A4>
+__global__ void __launch_bounds__(kPlainThreads)
+cb_plain_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ weights,
+                int8_t* __restrict__ out, const float* __restrict__ scale,
+                const float* __restrict__ offset, const int* __restrict__ bmap,
+                const float* __restrict__ ib_raw, int nchans, int nsamples, int nbeams,
+                int tscrunch, int tb_per_block, int ib_sub)
+{
+    constexpr int NV = nstokes(S);
+    constexpr int A  = A4 * 4;
+    // LDS: [kPlainTB][2][A4] of (re x4, im x4)
+    __shared__ uint2 vt[kPlainTB * 2 * A4];
+    const int f   = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int b   = blockIdx.z * kPlainThreads + tid;
+    const bool active = b < nbeams;
+    // beam weights in registers, transposed to (re x4, im x4) per 4 antennas
+    uint32_t wr[A4], wi[A4];
+    {
+        const uint2* wrow =
+            reinterpret_cast<const uint2*>(weights + ((size_t)f * nbeams + (active ? b : 0)) * A * 2);
+#pragma unroll
+        for(int g = 0; g < A4; ++g) {
+            const uint2 q = wrow[g];
+            wr[g] = active ? re_bytes(q.x, q.y) : 0u;
+            wi[g] = active ? im_bytes(q.x, q.y) : 0u;
+        }
+    }
+    const int nsamps_out = nsamples / tscrunch;
+    const int t_begin    = blockIdx.x * tb_per_block;
+    const int t_end      = min(nsamples, t_begin + tb_per_block);
+    float power[NV];
+#pragma unroll
+    for(int k = 0; k < NV; ++k) power[k] = 0.0f;
+    for(int tc = t_begin; tc < t_end; tc += kPlainTB) {
+        const int nt = min(kPlainTB, t_end - tc);
+        __syncthreads();
+        // stage nt samples x 2 pols x A4 groups: 8 bytes each
+        const uint2* src = reinterpret_cast<const uint2*>(ftpa + ((size_t)f * nsamples + tc) * 4 * A);
+        for(int i = tid; i < nt * 2 * A4; i += kPlainThreads) {
+            const uint2 q = src[i];
+            vt[i] = make_uint2(re_bytes(q.x, q.y), im_bytes(q.x, q.y));
+        }
+        __syncthreads();
+        for(int t = 0; t < nt; ++t) {
+            float re[2], im[2];
+#pragma unroll
+            for(int p = 0; p < 2; ++p) {
+                int xx = 0, yy = 0, xy = 0, yx = 0;
+                const uint2* v = vt + (t * 2 + p) * A4;
+#pragma unroll
+                for(int g = 0; g < A4; ++g) {
+                    const uint2 q = v[g];   // broadcast read
+                    xx = __builtin_amdgcn_sdot4((int)wr[g], (int)q.x, xx, false);
+                    yy = __builtin_amdgcn_sdot4((int)wi[g], (int)q.y, yy, false);
+                    xy = __builtin_amdgcn_sdot4((int)wr[g], (int)q.y, xy, false);
+                    yx = __builtin_amdgcn_sdot4((int)wi[g], (int)q.x, yx, false);
+                }
+                re[p] = (float)xx - (float)yy;
+                im[p] = (float)xy + (float)yx;
+            }
+            float st[NV];
+            Epi<S>::detect(re[0], im[0], re[1], im[1], st);
+#pragma unroll
+            for(int k = 0; k < NV; ++k) power[k] += st[k];
+            const int tg = tc + t;
+            if((tg + 1) % tscrunch == 0) {
+                if(active)
+                    Epi<S>::store(power, out, ib_raw, scale, offset, bmap, tg / tscrunch, f, b,
+                                  nchans, nbeams, nsamps_out, ib_sub);
+#pragma unroll
+                for(int k = 0; k < NV; ++k) power[k] = 0.0f;
+            }
+        }
+    }
+}
+
+// ========================================================== MFMA form
+constexpr int kMfmaWaves = 4;  // waves per workgroup, 32 time rows each
+
+// Packed weights: for each (f, b) two rows of 2A bytes:
+//   row 0 (real part):      [ wr[0..A) | ~wi[0..A) ]
+//   row 1 (imaginary part): [ wi[0..A) |  wr[0..A) ]
+__global__ void cb_pack_kernel(const uint8_t* __restrict__ w, uint8_t* __restrict__ packed,
+                               int nant, size_t nrows)
+{
+    // one thread per 4 antennas of one (f, b) row
+    const size_t i   = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int a4     = nant / 4;
+    if(i >= nrows * a4) return;
+    const size_t row = i / a4;
+    const int g      = (int)(i % a4);
+    const uint2 q    = reinterpret_cast<const uint2*>(w + row * nant * 2)[g];
+    const uint32_t r = re_bytes(q.x, q.y), im = im_bytes(q.x, q.y);
+    uint32_t* o = reinterpret_cast<uint32_t*>(packed + row * nant * 4);
+    const int qa = nant / 4;  // dwords per A bytes
+    o[g]          = r;
+    o[qa + g]     = ~im;
+    o[2 * qa + g] = im;
+    o[3 * qa + g] = r;
+}
+
+template <int S, int TS, int NSTEPS>
+__global__ void __launch_bounds__(kMfmaWaves * 64)
+cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ packed,
+               int8_t* __restrict__ out, const float* __restrict__ scale,
+               const float* __restrict__ offset, const int* __restrict__ bmap,
+               const float* __restrict__ ib_raw, int nchans, int nsamples, int nbeams,
+               int ntchunks, int ib_sub)
+{
+    constexpr int NV = nstokes(S);
+    constexpr int A  = NSTEPS * 16;  // antennas (K' = 2A = 32 * NSTEPS)
+    constexpr int KP = 2 * A;
+    // XCD-aware mapping: consecutive workgroups go round-robin over the 8
+    // XCDs; keep all time chunks of a channel on one XCD so its packed
+    // weights stay resident in that XCD's L2.
+    int f, tc;
+    {
+        const int bid = blockIdx.x;
+        if((nchans & 7) == 0) {
+            const int xcd = bid & 7, local = bid >> 3;
+            f  = xcd + 8 * (local / ntchunks);
+            tc = local % ntchunks;
+        } else {
+            f  = bid / ntchunks;
+            tc = bid % ntchunks;
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int t0   = (tc * kMfmaWaves + wave) * 32;  // first time row of this wave
+    if(t0 >= nsamples) return;
+    const int row = lane & 31;  // A-operand row / output column index
+    const int grp = lane >> 5;  // k group (16 bytes)
+
+    // ---- A operands: 32 time rows x K' for both polarisations
+    v4i va[2][NSTEPS];
+#pragma unroll
+    for(int p = 0; p < 2; ++p) {
+        const uint8_t* vrow = ftpa + (((size_t)f * nsamples + t0 + row) * 2 + p) * A * 2;
+#pragma unroll
+        for(int s = 0; s < NSTEPS; ++s) {
+            const int k  = 32 * s + 16 * grp;
+            const bool imag = k >= A;
+            const int a0 = imag ? k - A : k;
+            const uint4 lo = *reinterpret_cast<const uint4*>(vrow + a0 * 2);
+            const uint4 hi = *reinterpret_cast<const uint4*>(vrow + a0 * 2 + 16);
+            v4i x;
+            if(!imag) {
+                x[0] = re_bytes(lo.x, lo.y); x[1] = re_bytes(lo.z, lo.w);
+                x[2] = re_bytes(hi.x, hi.y); x[3] = re_bytes(hi.z, hi.w);
+            } else {
+                x[0] = im_bytes(lo.x, lo.y); x[1] = im_bytes(lo.z, lo.w);
+                x[2] = im_bytes(hi.x, hi.y); x[3] = im_bytes(hi.z, hi.w);
+            }
+            va[p][s] = x;
+        }
+    }
+    // ---- per-row correction sum_a vi (seed of the real accumulators)
+    v16i corr[2];
+#pragma unroll
+    for(int p = 0; p < 2; ++p) {
+        v16i c = {0};
+#pragma unroll
+        for(int s = 0; s < NSTEPS; ++s) {
+            const int k = 32 * s + 16 * grp;
+            const int one = (k >= A) ? 0x01010101 : 0;
+            const v4i ones = {one, one, one, one};
+            c = __builtin_amdgcn_mfma_i32_32x32x32_i8(va[p][s], ones, c, 0, 0, 0);
+        }
+        corr[p] = c;
+    }
+
+    const int nsamps_out = nsamples / TS;
+    const int nbt = (nbeams + 31) / 32;
+    for(int bt = 0; bt < nbt; ++bt) {
+        const int b = bt * 32 + row;
+        const bool bvalid = b < nbeams;
+        const uint8_t* wrow = packed + ((size_t)f * nbeams + (bvalid ? b : 0)) * 2 * KP + 16 * grp;
+        v16i acc_re[2] = {corr[0], corr[1]};
+        v16i acc_im[2] = {v16i{0}, v16i{0}};
+#pragma unroll
+        for(int s = 0; s < NSTEPS; ++s) {
+            v4i wre = *reinterpret_cast<const v4i*>(wrow + 32 * s);
+            v4i wim = *reinterpret_cast<const v4i*>(wrow + KP + 32 * s);
+            if(!bvalid) { wre = v4i{0, 0, 0, 0}; wim = v4i{0, 0, 0, 0}; }
+#pragma unroll
+            for(int p = 0; p < 2; ++p) {
+                acc_re[p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(va[p][s], wre, acc_re[p], 0, 0, 0);
+                acc_im[p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(va[p][s], wim, acc_im[p], 0, 0, 0);
+            }
+        }
+        // ---- detect: register r of this lane holds time row
+        //      8*(r/4) + 4*grp + (r%4) of column (beam) `row`
+        float st[16][NV];
+#pragma unroll
+        for(int r = 0; r < 16; ++r)
+            Epi<S>::detect((float)acc_re[0][r], (float)acc_im[0][r], (float)acc_re[1][r],
+                           (float)acc_im[1][r], st[r]);
+        // ---- exchange halves so each lane holds 16 consecutive rows:
+        //      lanes 0-31 rows 0-15, lanes 32-63 rows 16-31, in the order
+        //      r = 0-3, 8-11, 4-7, 12-15.
+#pragma unroll
+        for(int r = 0; r < 8; ++r) {
+#pragma unroll
+            for(int k = 0; k < NV; ++k) {
+                const float send = grp ? st[r][k] : st[8 + r][k];
+                const float recv = __shfl_xor(send, 32);
+                if(grp) st[r][k] = recv; else st[8 + r][k] = recv;
+            }
+        }
+        constexpr int seq[16] = {0, 1, 2, 3, 8, 9, 10, 11, 4, 5, 6, 7, 12, 13, 14, 15};
+        if constexpr(TS <= 16) {
+#pragma unroll
+            for(int o = 0; o < 16 / TS; ++o) {
+                float power[NV];
+#pragma unroll
+                for(int k = 0; k < NV; ++k) power[k] = 0.0f;
+#pragma unroll
+                for(int q = 0; q < TS; ++q)
+#pragma unroll
+                    for(int k = 0; k < NV; ++k) power[k] += st[seq[o * TS + q]][k];
+                if(bvalid)
+                    Epi<S>::store(power, out, ib_raw, scale, offset, bmap,
+                                  (t0 + 16 * grp + o * TS) / TS, f, b, nchans, nbeams,
+                                  nsamps_out, ib_sub);
+            }
+        } else {  // TS == 32: the upper half continues the lower half's sum
+            float power[NV];
+#pragma unroll
+            for(int k = 0; k < NV; ++k) power[k] = 0.0f;
+#pragma unroll
+            for(int q = 0; q < 16; ++q)
+#pragma unroll
+                for(int k = 0; k < NV; ++k) power[k] += st[seq[q]][k];
+            float carried[NV];
+#pragma unroll
+            for(int k = 0; k < NV; ++k) carried[k] = __shfl_xor(power[k], 32);
+            if(grp) {
+#pragma unroll
+                for(int k = 0; k < NV; ++k) power[k] = carried[k];
+#pragma unroll
+                for(int q = 0; q < 16; ++q)
+#pragma unroll
+                    for(int k = 0; k < NV; ++k) power[k] += st[seq[q]][k];
+                if(bvalid)
+                    Epi<S>::store(power, out, ib_raw, scale, offset, bmap, t0 / 32, f, b,
+                                  nchans, nbeams, nsamps_out, ib_sub);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ dispatch
+template <int S>
+int launch_plain(const uint8_t* ftpa, const uint8_t* w, int8_t* out, const float* sc,
+                 const float* off, const int* bmap, const float* ib, const sw_bf_dims& d,
+                 hipStream_t st)
+{
+    const int tb_per_block = max(1024, d.tscrunch * ((1024 + d.tscrunch - 1) / d.tscrunch));
+    dim3 grid((d.nsamples + tb_per_block - 1) / tb_per_block, d.nchans,
+              (d.nbeams + kPlainThreads - 1) / kPlainThreads);
+#define SW_PLAIN(A4V)                                                                      \
+    case A4V:                                                                              \
+        hipLaunchKernelGGL((cb_plain_kernel<S, A4V>), grid, dim3(kPlainThreads), 0, st,    \
+                           ftpa, w, out, sc, off, bmap, ib, d.nchans, d.nsamples,          \
+                           d.nbeams, d.tscrunch, tb_per_block, d.ib_subtract);             \
+        break;
+    switch(d.nantennas / 4) {
+        SW_PLAIN(1) SW_PLAIN(2) SW_PLAIN(3) SW_PLAIN(4) SW_PLAIN(5) SW_PLAIN(6) SW_PLAIN(7)
+        SW_PLAIN(8) SW_PLAIN(10) SW_PLAIN(12) SW_PLAIN(14) SW_PLAIN(16) SW_PLAIN(20)
+        SW_PLAIN(24) SW_PLAIN(28) SW_PLAIN(32)
+    default:
+        set_error("sw_cb_beamform: plain form supports A in {4..32, 40, 48, 56, 64, 80, 96, 112, 128}");
+        return SW_ERR_UNSUPPORTED;
+    }
+#undef SW_PLAIN
+    SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
+template <int S, int TS>
+int launch_mfma_ts(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, const float* sc,
+                   const float* off, const int* bmap, const float* ib, const sw_bf_dims& d,
+                   hipStream_t st)
+{
+    const int ntchunks = (d.nsamples + 32 * kMfmaWaves - 1) / (32 * kMfmaWaves);
+    dim3 grid(ntchunks * d.nchans);
+#define SW_MFMA(NS)                                                                        \
+    case NS:                                                                               \
+        hipLaunchKernelGGL((cb_mfma_kernel<S, TS, NS>), grid, dim3(kMfmaWaves * 64), 0,    \
+                           st, ftpa, packed, out, sc, off, bmap, ib, d.nchans,             \
+                           d.nsamples, d.nbeams, ntchunks, d.ib_subtract);                 \
+        break;
+    switch(d.nantennas / 16) {
+        SW_MFMA(1) SW_MFMA(2) SW_MFMA(3) SW_MFMA(4) SW_MFMA(5) SW_MFMA(6) SW_MFMA(7) SW_MFMA(8)
+    default:
+        set_error("sw_cb_beamform: MFMA form supports A in {16, 32, ..., 128}");
+        return SW_ERR_UNSUPPORTED;
+    }
+#undef SW_MFMA
+    SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
+template <int S>
+int launch_mfma(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, const float* sc,
+                const float* off, const int* bmap, const float* ib, const sw_bf_dims& d,
+                hipStream_t st)
+{
+    switch(d.tscrunch) {
+    case 1: return launch_mfma_ts<S, 1>(ftpa, packed, out, sc, off, bmap, ib, d, st);
+    case 2: return launch_mfma_ts<S, 2>(ftpa, packed, out, sc, off, bmap, ib, d, st);
+    case 4: return launch_mfma_ts<S, 4>(ftpa, packed, out, sc, off, bmap, ib, d, st);
+    case 8: return launch_mfma_ts<S, 8>(ftpa, packed, out, sc, off, bmap, ib, d, st);
+    case 16: return launch_mfma_ts<S, 16>(ftpa, packed, out, sc, off, bmap, ib, d, st);
+    case 32: return launch_mfma_ts<S, 32>(ftpa, packed, out, sc, off, bmap, ib, d, st);
+    default:
+        set_error("sw_cb_beamform: MFMA form supports tscrunch in {1, 2, 4, 8, 16, 32}");
+        return SW_ERR_UNSUPPORTED;
+    }
+}
+
+int check_dims(const sw_bf_dims& d)
+{
+    SW_CHECK_ARG(d.nchans > 0 && d.nbeams > 0 && d.nantennas > 0, "empty dimensions");
+    SW_CHECK_ARG(d.nantennas % 4 == 0, "Number of antennas must be a multiple of 4");
+    SW_CHECK_ARG(d.nbeamsets > 0, "Number of beamsets must be greater than zero");
+    SW_CHECK_ARG(d.tscrunch > 0 && d.nsamples > 0 && d.nsamples % d.tscrunch == 0,
+                 "nsamples must be a positive multiple of tscrunch");
+    SW_CHECK_ARG(d.stokes >= SW_STOKES_I && d.stokes <= SW_STOKES_IQUV, "bad stokes mode");
+    return SW_OK;
+}
+
+} // namespace
+
+// exposed to the pipeline
+int cb_mfma_supported(const sw_bf_dims& d)
+{
+    return d.nantennas % 16 == 0 && d.nantennas <= 128 && d.nsamples % 32 == 0 &&
+           (d.tscrunch == 1 || d.tscrunch == 2 || d.tscrunch == 4 || d.tscrunch == 8 ||
+            d.tscrunch == 16 || d.tscrunch == 32);
+}
+
+} // namespace sw
+
+extern "C" size_t sw_cb_workspace_size(const sw_bf_dims* dims)
+{
+    if(!dims) return 0;
+    return (size_t)dims->nchans * dims->nbeams * dims->nantennas * 4;
+}
+
+extern "C" int sw_cb_pack_weights(const void* d_weights, void* d_packed, int nchans,
+                                  int nbeams, int nantennas, void* stream)
+{
+    SW_CHECK_ARG(d_weights && d_packed, "null buffer");
+    SW_CHECK_ARG(nchans > 0 && nbeams > 0 && nantennas > 0 && nantennas % 4 == 0,
+                 "bad weight dimensions");
+    SW_CHECK_ARG((reinterpret_cast<uintptr_t>(d_weights) & 7) == 0 &&
+                     (reinterpret_cast<uintptr_t>(d_packed) & 15) == 0,
+                 "weights must be 8-byte and packed weights 16-byte aligned");
+    const size_t nrows = (size_t)nchans * nbeams;
+    const size_t n     = nrows * (nantennas / 4);
+    hipLaunchKernelGGL(sw::cb_pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       sw::as_stream(stream), static_cast<const uint8_t*>(d_weights),
+                       static_cast<uint8_t*>(d_packed), nantennas, nrows);
+    SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
+extern "C" int sw_cb_beamform_packed(const void* d_ftpa, const void* d_packed,
+                                     const float* d_scale, const float* d_offset,
+                                     const int* d_beamset_map, const float* d_ib_raw,
+                                     void* d_out, const sw_bf_dims* dims, void* stream)
+{
+    SW_CHECK_ARG(dims, "null dims");
+    int rc = sw::check_dims(*dims);
+    if(rc) return rc;
+    const sw_bf_dims& d = *dims;
+    SW_CHECK_ARG(d_ftpa && d_packed && d_scale && d_offset && d_beamset_map && d_ib_raw && d_out,
+                 "null buffer");
+    SW_CHECK_ARG((reinterpret_cast<uintptr_t>(d_ftpa) & 15) == 0 &&
+                     (reinterpret_cast<uintptr_t>(d_packed) & 15) == 0,
+                 "voltages and packed weights must be 16-byte aligned");
+    if(!sw::cb_mfma_supported(d)) {
+        sw::set_error("sw_cb_beamform_packed: shape not supported by the MFMA form "
+                      "(needs A % 16 == 0, A <= 128, T % 32 == 0, tscrunch in {1..32, pow2})");
+        return SW_ERR_UNSUPPORTED;
+    }
+    auto* in  = static_cast<const uint8_t*>(d_ftpa);
+    auto* pk  = static_cast<const uint8_t*>(d_packed);
+    auto* out = static_cast<int8_t*>(d_out);
+    hipStream_t st = sw::as_stream(stream);
+    switch(d.stokes) {
+    case SW_STOKES_I: return sw::launch_mfma<SW_STOKES_I>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    case SW_STOKES_Q: return sw::launch_mfma<SW_STOKES_Q>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    case SW_STOKES_U: return sw::launch_mfma<SW_STOKES_U>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    case SW_STOKES_V: return sw::launch_mfma<SW_STOKES_V>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    default: return sw::launch_mfma<SW_STOKES_IQUV>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    }
+}
+
+extern "C" int sw_cb_beamform(const void* d_ftpa, const void* d_weights, const float* d_scale,
+                              const float* d_offset, const int* d_beamset_map,
+                              const float* d_ib_raw, void* d_out, const sw_bf_dims* dims,
+                              int form, void* d_workspace, void* stream)
+{
+    SW_CHECK_ARG(dims, "null dims");
+    int rc = sw::check_dims(*dims);
+    if(rc) return rc;
+    const sw_bf_dims& d = *dims;
+    SW_CHECK_ARG(d_ftpa && d_weights && d_scale && d_offset && d_beamset_map && d_ib_raw && d_out,
+                 "null buffer");
+    if(form == SW_CB_MFMA) {
+        void* ws = d_workspace ? d_workspace : sw::scratch(sw_cb_workspace_size(dims), 1);
+        if(!ws) return SW_ERR_ALLOC;
+        rc = sw_cb_pack_weights(d_weights, ws, d.nchans, d.nbeams, d.nantennas, stream);
+        if(rc) return rc;
+        return sw_cb_beamform_packed(d_ftpa, ws, d_scale, d_offset, d_beamset_map, d_ib_raw,
+                                     d_out, dims, stream);
+    }
+    SW_CHECK_ARG(form == SW_CB_PLAIN, "unknown beamformer form");
+    SW_CHECK_ARG((reinterpret_cast<uintptr_t>(d_ftpa) & 7) == 0 &&
+                     (reinterpret_cast<uintptr_t>(d_weights) & 7) == 0,
+                 "voltages and weights must be 8-byte aligned");
+    auto* in  = static_cast<const uint8_t*>(d_ftpa);
+    auto* w   = static_cast<const uint8_t*>(d_weights);
+    auto* out = static_cast<int8_t*>(d_out);
+    hipStream_t st = sw::as_stream(stream);
+    switch(d.stokes) {
+    case SW_STOKES_I: return sw::launch_plain<SW_STOKES_I>(in, w, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    case SW_STOKES_Q: return sw::launch_plain<SW_STOKES_Q>(in, w, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    case SW_STOKES_U: return sw::launch_plain<SW_STOKES_U>(in, w, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    case SW_STOKES_V: return sw::launch_plain<SW_STOKES_V>(in, w, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    default: return sw::launch_plain<SW_STOKES_IQUV>(in, w, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    }
+}

@@ -155,6 +155,47 @@ int sw_cb_beamform_packed(const void* d_ftpa, const void* d_packed,
                           const int* d_beamset_map, const float* d_ib_raw,
                           void* d_out, const sw_bf_dims* dims, void* stream);
 
+/* ------------------------------------------------------ CoherentDedisperser
+ * create_coherent_dedisperser_config + CoherentDedisperser::dedisperse,
+ * sw/CoherentDedisperser.cuh:55-92, sw/src/CoherentDedisperser.cu:19-353.
+ * A plan holds, per coherent DM, the FFT length N, the overlap O (even),
+ * the FP64-generated chirp tables and the overlap-save history (the previous
+ * gulp's last O samples of every channel; zeros before the first gulp).
+ * Output sample j of a gulp is the dedispersed input sample j - O/2 (the
+ * reference's filter delay tsamp*O/2).  fft_len / overlap entries <= 0 (or
+ * NULL arrays) select O = even-ceil of the intra-channel smear at the
+ * bottom of the band and N = the smallest power of two >= max(1024, 16 O). */
+typedef struct sw_codedisp_config {
+    int nchans;        /* F */
+    int nantennas;     /* A (FTPA padded antenna count) */
+    int nsamples;      /* T per gulp */
+    double low_freq;   /* Hz, bottom edge of the band (centre - bw/2) */
+    double chan_bw;    /* Hz, channel bandwidth */
+    double tsamp;      /* s, channel sampling interval */
+    int ndms;
+    const float* dms;  /* [ndms] pc cm^-3 */
+    const int* fft_len;/* [ndms] or NULL */
+    const int* overlap;/* [ndms] or NULL */
+} sw_codedisp_config;
+typedef struct sw_codedisp sw_codedisp;
+
+int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, sw_codedisp** plan);
+int sw_codedisp_destroy(sw_codedisp* plan);
+int sw_codedisp_info(const sw_codedisp* plan, int dm_idx, int* fft_len, int* overlap);
+/* Dedisperse the gulp d_ftpa (FTPA [F][T][2][A]) for the DMs
+ * [dm_begin, dm_begin + dm_count): d_out[d] receives FTPA int8 for DM d;
+ * d_out_f32 (NULL or per-DM NULL allowed) receives the unquantised y/N as
+ * float (re, im) pairs.  batched = 1 shares each forward FFT between up to 8
+ * DMs of equal (N, O) in one launch; batched = 0 launches one DM at a time.
+ * Both give identical results.  The history is not advanced. */
+int sw_codedisp_run(sw_codedisp* plan, const void* d_ftpa, void* const* d_out,
+                    void* const* d_out_f32, int dm_begin, int dm_count, int batched,
+                    void* stream);
+/* Advance the overlap-save history to the end of the gulp d_ftpa. */
+int sw_codedisp_push_history(sw_codedisp* plan, const void* d_ftpa, void* stream);
+/* Replace the history ([F][Omax][2][A] complex int8, Omax = largest overlap). */
+int sw_codedisp_set_history(sw_codedisp* plan, const void* d_hist, void* stream);
+
 /* -------------------------------------------------- synthetic input (bench)
  * Seeded Gaussian complex int8: xoshiro256** seeded through splitmix64,
  * Box-Muller in FP64, value = clip(round_half_away(sigma*z), -127, 127).

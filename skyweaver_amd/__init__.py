@@ -53,6 +53,13 @@ class BfDims(ctypes.Structure):
         "nchans", "nsamples", "nantennas", "nbeams", "tscrunch", "stokes", "nbeamsets", "ib_subtract")]
 
 
+class CodedispConfig(ctypes.Structure):
+    _fields_ = [("nchans", ctypes.c_int), ("nantennas", ctypes.c_int), ("nsamples", ctypes.c_int),
+                ("low_freq", ctypes.c_double), ("chan_bw", ctypes.c_double), ("tsamp", ctypes.c_double),
+                ("ndms", ctypes.c_int), ("dms", ctypes.POINTER(ctypes.c_float)),
+                ("fft_len", ctypes.POINTER(ctypes.c_int)), ("overlap", ctypes.POINTER(ctypes.c_int))]
+
+
 _vp = ctypes.c_void_p
 _c = ctypes.c_int
 _sz = ctypes.c_size_t
@@ -77,6 +84,12 @@ def _declare(L):
     L.sw_synth_gaussian_c8.argtypes = [_vp, _sz, ctypes.c_uint64, ctypes.c_double, _c]
     L.sw_synth_phase_weights.argtypes = [_vp, _sz, ctypes.c_uint64]
     _pvp = ctypes.POINTER(ctypes.c_void_p)
+    L.sw_codedisp_create.argtypes = [ctypes.POINTER(CodedispConfig), _vp, _pvp]
+    L.sw_codedisp_destroy.argtypes = [_vp]
+    L.sw_codedisp_info.argtypes = [_vp, _c, ctypes.POINTER(_c), ctypes.POINTER(_c)]
+    L.sw_codedisp_run.argtypes = [_vp, _vp, _pvp, _pvp, _c, _c, _c, _vp]
+    L.sw_codedisp_push_history.argtypes = [_vp, _vp, _vp]
+    L.sw_codedisp_set_history.argtypes = [_vp, _vp, _vp]
     L.sw_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
     L.sw_malloc.argtypes = [_pvp, _sz]
     L.sw_free.argtypes = [_vp]
@@ -300,3 +313,55 @@ def synth_weights(shape, seed):
     assert out.shape[-1] == 2
     check(lib().sw_synth_phase_weights(out.ctypes.data, out.size // 2, seed))
     return out
+
+
+class CoherentDedisperser:
+    """Overlap-save coherent dedispersion plan (sw_codedisp_* in the C ABI)."""
+
+    def __init__(self, nchans, nantennas, nsamples, low_freq, chan_bw, tsamp, dms, fft_len=None,
+                 overlap=None, stream=None):
+        import numpy as np
+        self._dms = np.ascontiguousarray(dms, np.float32)
+        n = len(self._dms)
+        self._fft = np.ascontiguousarray(fft_len if fft_len is not None else [0] * n, np.int32)
+        self._ov = np.ascontiguousarray(overlap if overlap is not None else [-1] * n, np.int32)
+        cfg = CodedispConfig(nchans, nantennas, nsamples, low_freq, chan_bw, tsamp, n,
+                             self._dms.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                             self._fft.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                             self._ov.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        h = ctypes.c_void_p()
+        check(lib().sw_codedisp_create(ctypes.byref(cfg), _stream(stream), ctypes.byref(h)))
+        self.handle = h.value
+        self.ndms = n
+
+    def info(self, d):
+        N, O = ctypes.c_int(), ctypes.c_int()
+        check(lib().sw_codedisp_info(self.handle, d, ctypes.byref(N), ctypes.byref(O)))
+        return N.value, O.value
+
+    def run(self, ftpa, outs, outs_f32=None, dm_begin=0, dm_count=None, batched=True, stream=None):
+        n = self.ndms
+        dm_count = n - dm_begin if dm_count is None else dm_count
+        arr = (ctypes.c_void_p * n)(*[_ptr(o) if o is not None else None for o in outs])
+        farr = None
+        if outs_f32 is not None:
+            farr = (ctypes.c_void_p * n)(*[_ptr(o) if o is not None else None for o in outs_f32])
+        check(lib().sw_codedisp_run(self.handle, _ptr(ftpa), arr, farr, dm_begin, dm_count, 1 if batched else 0,
+                                    _stream(stream)))
+
+    def push_history(self, ftpa, stream=None):
+        check(lib().sw_codedisp_push_history(self.handle, _ptr(ftpa), _stream(stream)))
+
+    def set_history(self, hist, stream=None):
+        check(lib().sw_codedisp_set_history(self.handle, _ptr(hist), _stream(stream)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sw_codedisp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,204 @@
+// fft.hpp -- FP32 complex FFT building blocks for gfx950: in-register DFTs of
+// radix 2/4/8/16 and a Stockham autosort pass over a column tile in LDS.
+//
+// Conventions: forward transforms use exp(-2 pi i n k / N), inverse
+// transforms exp(+2 pi i n k / N), both unnormalised.  Twiddles come from a
+// table generated in FP64 and rounded once to FP32.  Complex products use
+// explicit fmaf (the library is otherwise compiled without contraction).
+#pragma once
+
+#include "sw_common.hpp"
+
+namespace sw {
+namespace fft {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 rot90(float2 a)
+{
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+// multiply by exp(-+ 2 pi i m / 8) for m = 1, 3 (m = 2 is rot90)
+template <bool INV, int M>
+__device__ __forceinline__ float2 rot45(float2 a)
+{
+    constexpr float h = 0.70710678118654752440f;
+    // exp(-i pi/4) = h(1 - i); exp(-3 i pi/4) = h(-1 - i); inverse: conjugates
+    if constexpr(M == 1) {
+        return INV ? make_float2(h * (a.x - a.y), h * (a.x + a.y))
+                   : make_float2(h * (a.x + a.y), h * (a.y - a.x));
+    } else {
+        return INV ? make_float2(-h * (a.x + a.y), h * (a.x - a.y))
+                   : make_float2(h * (a.y - a.x), -h * (a.x + a.y));
+    }
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft2(float2& a, float2& b)
+{
+    const float2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+// natural order in and out
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2& v3)
+{
+    const float2 t0 = cadd(v0, v2), t1 = csub(v0, v2);
+    const float2 t2 = cadd(v1, v3), t3 = rot90<INV>(csub(v1, v3));
+    v0 = cadd(t0, t2);
+    v2 = csub(t0, t2);
+    v1 = cadd(t1, t3);
+    v3 = csub(t1, t3);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft8(float2* v)
+{
+    // n = 2 n1 + n2: DFT4 over n1, twiddle W8^(n2 k1), DFT2 over n2
+    dft4<INV>(v[0], v[2], v[4], v[6]);
+    dft4<INV>(v[1], v[3], v[5], v[7]);
+    v[3] = rot45<INV, 1>(v[3]);
+    v[5] = rot90<INV>(v[5]);
+    v[7] = rot45<INV, 3>(v[7]);
+    float2 o[8];
+#pragma unroll
+    for(int k1 = 0; k1 < 4; ++k1) {
+        o[k1]     = cadd(v[2 * k1], v[2 * k1 + 1]);
+        o[k1 + 4] = csub(v[2 * k1], v[2 * k1 + 1]);
+    }
+#pragma unroll
+    for(int k = 0; k < 8; ++k) v[k] = o[k];
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft16(float2* v)
+{
+    // n = 4 n1 + n2: DFT4 over n1 for each n2, twiddle W16^(n2 k1), DFT4 over n2
+#pragma unroll
+    for(int n2 = 0; n2 < 4; ++n2) dft4<INV>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+    // a[k1][n2] now sits in v[4 k1 + n2]
+    constexpr float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+    constexpr float h  = 0.70710678118654752440f;
+    const float sg = INV ? 1.0f : -1.0f;
+    // W16^m = cos(2 pi m/16) -+ i sin(2 pi m/16)
+    const float2 w1 = make_float2(c1, sg * s1), w2 = make_float2(h, sg * h),
+                 w3 = make_float2(s1, sg * c1), w6 = make_float2(-h, sg * h),
+                 w9 = make_float2(-c1, -sg * s1);
+    v[5]  = cmul(v[5], w1);   // k1=1 n2=1
+    v[6]  = cmul(v[6], w2);   // k1=1 n2=2
+    v[7]  = cmul(v[7], w3);   // k1=1 n2=3
+    v[9]  = cmul(v[9], w2);   // k1=2 n2=1
+    v[10] = rot90<INV>(v[10]); // k1=2 n2=2: W16^4
+    v[11] = cmul(v[11], w6);  // k1=2 n2=3
+    v[13] = cmul(v[13], w3);  // k1=3 n2=1
+    v[14] = cmul(v[14], w6);  // k1=3 n2=2
+    v[15] = cmul(v[15], w9);  // k1=3 n2=3
+#pragma unroll
+    for(int k1 = 0; k1 < 4; ++k1) dft4<INV>(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+    // X[k1 + 4 k2] = v[4 k1 + k2]: transpose the 4x4 register block
+    float2 o[16];
+#pragma unroll
+    for(int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+        for(int k2 = 0; k2 < 4; ++k2) o[k1 + 4 * k2] = v[4 * k1 + k2];
+#pragma unroll
+    for(int k = 0; k < 16; ++k) v[k] = o[k];
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2* v)
+{
+    if constexpr(R == 2) {
+        dft2<INV>(v[0], v[1]);
+    } else if constexpr(R == 4) {
+        dft4<INV>(v[0], v[1], v[2], v[3]);
+    } else if constexpr(R == 8) {
+        dft8<INV>(v);
+    } else {
+        static_assert(R == 16, "radix");
+        dft16<INV>(v);
+    }
+}
+
+// LDS index with one float2 of padding every 16 (breaks power-of-two strides)
+__device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int padded(int n) { return n + (n >> 4); }
+
+// One Stockham autosort pass of radix R over C columns of length L held in
+// LDS (column c at offset c*padded(L)).  Ns = product of the radices already
+// applied.  tw[m] = exp(-2 pi i m / L).  All threads of the block call it.
+template <int R, bool INV, int L, int C, int NT>
+__device__ __forceinline__ void stockham_pass(float2* buf, int Ns, const float2* __restrict__ tw)
+{
+    constexpr int NB  = L / R;                 // butterflies per column
+    constexpr int PER = (C * NB + NT - 1) / NT; // butterflies per thread
+    const int tid = threadIdx.x;
+    float2 v[PER][R];
+#pragma unroll
+    for(int q = 0; q < PER; ++q) {
+        const int item = tid + q * NT;
+        if(item < C * NB) {
+            const int c = item / NB, j = item % NB;
+            const float2* col = buf + c * padded(L);
+#pragma unroll
+            for(int k = 0; k < R; ++k) v[q][k] = col[pad(j + k * NB)];
+            if(Ns > 1) {
+                const int jm = j % Ns;
+                const int stride = jm * (L / (Ns * R));
+#pragma unroll
+                for(int k = 1; k < R; ++k) {
+                    float2 w = tw[stride * k];
+                    if(INV) w = cconj(w);
+                    v[q][k] = cmul(v[q][k], w);
+                }
+            }
+            dft<R, INV>(v[q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for(int q = 0; q < PER; ++q) {
+        const int item = tid + q * NT;
+        if(item < C * NB) {
+            const int c = item / NB, j = item % NB;
+            float2* col = buf + c * padded(L);
+            const int jm = j % Ns;
+            const int idxD = (j / Ns) * Ns * R + jm;
+#pragma unroll
+            for(int k = 0; k < R; ++k) col[pad(idxD + k * Ns)] = v[q][k];
+        }
+    }
+    __syncthreads();
+}
+
+// Full FFT of length L (power of two, 16..4096) on C columns in LDS, natural
+// order in and out, as radix-16 passes followed by one smaller pass.
+template <bool INV, int L, int C, int NT>
+__device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw)
+{
+    int Ns = 1;
+    constexpr int lg = __builtin_ctz(L);
+    constexpr int n16 = lg / 4;
+    constexpr int rem = lg % 4;
+#pragma unroll
+    for(int p = 0; p < n16; ++p) {
+        stockham_pass<16, INV, L, C, NT>(buf, Ns, tw);
+        Ns *= 16;
+    }
+    if constexpr(rem == 1) stockham_pass<2, INV, L, C, NT>(buf, Ns, tw);
+    if constexpr(rem == 2) stockham_pass<4, INV, L, C, NT>(buf, Ns, tw);
+    if constexpr(rem == 3) stockham_pass<8, INV, L, C, NT>(buf, Ns, tw);
+}
+
+} // namespace fft
+} // namespace sw

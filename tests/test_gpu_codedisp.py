@@ -1,0 +1,133 @@
+"""GPU coherent dedispersion against the FP64 overlap-save oracle and the
+reference's golden vectors.
+
+Tolerances: dedispersed voltages (y/N before quantisation) within 1e-5
+relative (L2 norm over each DM's output) of the FP64 oracle; int8 outputs
+within +-1 (modulo the int8 wrap) of the oracle, with mismatches confined to
+the rare values that lie within FP32 rounding of a half-integer.  The golden
+pair is checked at +-1, the reference test's own tolerance
+(CoherentDedisperserTester.cu:195-203)."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TSAMP = 4096 / 856e6
+BAND_BW = 13.375e6
+CHAN_BW = BAND_BW / 64
+F_LOW = 1284e6 - BAND_BW / 2
+
+
+def _wrapdiff(a, b):
+    return ((a.astype(np.int64) - b.astype(np.int64) + 128) % 256) - 128
+
+
+def _check(got_i8, got_f, exp_i8, exp_f, label):
+    gf = got_f[..., 0].astype(np.float64) + 1j * got_f[..., 1]
+    rel = np.linalg.norm(gf - exp_f) / np.linalg.norm(exp_f)
+    mx = np.abs(gf - exp_f).max()
+    d = _wrapdiff(got_i8, exp_i8)
+    frac = np.mean(d != 0)
+    print(f"{label}: rel L2 {rel:.3e} max abs {mx:.3e} int8 mismatch {frac:.2e}", flush=True)
+    assert rel <= 1e-5, label
+    assert np.abs(d).max() <= 1, label
+    assert frac <= 1e-3, label
+    return rel
+
+
+def _oracle_stream(orc, hist, cur, chirps, N, O, nthreads=16):
+    """hist [F][O][P][A][2], cur [F][T][P][A][2] -> (int8, complex) [F][T][P][A]."""
+    F, T = cur.shape[:2]
+    outs_i, outs_f = [], []
+    for f in range(F):
+        x = np.concatenate([hist[f, hist.shape[1] - O:], cur[f]], 0)
+        yf, yi = orc.codedisp_channel(x, chirps[f], N, O, T, nthreads=nthreads)
+        outs_i.append(yi)
+        outs_f.append(yf)
+    return np.stack(outs_i), np.stack(outs_f)
+
+
+def test_codedisp_golden_pair(sw, orc, dev, golden_dir):
+    from skyweaver_amd import DeviceBuffer as DB
+    inp = np.fromfile(os.path.join(golden_dir, "codedisp_input_DM1000_1chan.bin"), np.int8).reshape(1, 1024, 2, 64, 2)
+    gold = np.fromfile(os.path.join(golden_dir, "codedisp_output_DM1000_1chan.bin"), np.int8).reshape(1, 446, 2, 64, 2)
+    O, T = 578, 446
+    plan = sw.CoherentDedisperser(1, 64, T, 856e6, BAND_BW / 64, TSAMP, [1000.0], fft_len=[1024], overlap=[O])
+    hist = DB.from_host(np.ascontiguousarray(inp[:, :O]))
+    cur = DB.from_host(np.ascontiguousarray(inp[:, O:]))
+    out = DB.empty((1, T, 2, 64, 2), np.int8, fill=0x55)
+    outf = DB.empty((1, T, 2, 64, 2), np.float32)
+    plan.set_history(hist)
+    plan.run(cur, [out], [outf])
+    got = out.download()
+    d = _wrapdiff(got, gold)
+    print(f"golden: exact {np.mean(d == 0):.5f} max |d| {np.abs(d).max()}", flush=True)
+    assert np.abs(d).max() <= 1
+    chirp = orc.dm_chirp(1000.0, 856e6, BAND_BW / 64, 64, 1024)[:1]
+    ei, ef = _oracle_stream(orc, inp[:, :O], inp[:, O:], chirp, 1024, O)
+    _check(got, outf.download(), ei, ef, "golden DM1000 N1024")
+
+
+@pytest.mark.parametrize("dm,N", [(50.0, 16384), (50.0, 1024), (350.0, 32768), (100.0, 4096)])
+def test_codedisp_vs_fp64_oracle(sw, orc, dev, dm, N):
+    from skyweaver_amd import DeviceBuffer as DB
+    F, A, T = 2, 64, 40000
+    cur = sw.synth_voltages((F, T, 2, A, 2), seed=0x5EED0002, sigma=16.0)
+    plan = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, [dm], fft_len=[N])
+    _, O = plan.info(0)
+    print(f"codedisp dm={dm} N={N} O={O}", flush=True)
+    d_cur = DB.from_host(cur)
+    out = DB.empty(cur.shape, np.int8, fill=0x55)
+    outf = DB.empty(cur.shape, np.float32)
+    plan.run(d_cur, [out], [outf])
+    chirp = orc.dm_chirp(dm, F_LOW, CHAN_BW, F, N)
+    ei, ef = _oracle_stream(orc, np.zeros((F, O, 2, A, 2), np.int8), cur, chirp, N, O)
+    _check(out.download(), outf.download(), ei, ef, f"DM {dm} N {N}")
+
+
+def test_codedisp_dm0_is_identity(sw, dev):
+    from skyweaver_amd import DeviceBuffer as DB
+    F, A, T = 3, 64, 9000
+    cur = sw.synth_voltages((F, T, 2, A, 2), seed=11, sigma=30.0)
+    plan = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, [0.0, 0.0], fft_len=[4096, 16384],
+                                  overlap=[0, 20])
+    outs = [DB.empty(cur.shape, np.int8, fill=0x55) for _ in range(2)]
+    plan.run(DB.from_host(cur), outs)
+    np.testing.assert_array_equal(outs[0].download(), cur)
+    # with O = 20 the output is delayed by O/2 (zero history before the first gulp)
+    exp = np.concatenate([np.zeros((F, 10, 2, A, 2), np.int8), cur[:, :T - 10]], 1)
+    np.testing.assert_array_equal(outs[1].download(), exp)
+
+
+def test_codedisp_uneven_batch_and_history(sw, orc, dev):
+    """Eight DMs with uneven FFT lengths over two gulps: batched == one DM per
+    launch bit for bit, and both match the oracle on the continuous stream."""
+    from skyweaver_amd import DeviceBuffer as DB
+    F, A, T = 2, 64, 24576
+    dms = [0.0, 5.0, 20.0, 50.0, 100.0, 180.0, 260.0, 350.0]
+    Ns = [4096, 4096, 8192, 8192, 16384, 16384, 32768, 32768]
+    plan_b = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, fft_len=Ns)
+    plan_u = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, fft_len=Ns)
+    Os = [plan_b.info(d)[1] for d in range(8)]
+    assert Os == [0, 2, 4, 10, 18, 32, 46, 62]
+    g1 = sw.synth_voltages((F, T, 2, A, 2), seed=0x5EED0003, sigma=16.0)
+    g2 = sw.synth_voltages((F, T, 2, A, 2), seed=0x5EED0013, sigma=16.0)
+    for gi, g in enumerate((g1, g2)):
+        d_g = DB.from_host(g)
+        ob = [DB.empty(g.shape, np.int8, fill=0x55) for _ in dms]
+        obf = [DB.empty(g.shape, np.float32) for _ in dms]
+        ou = [DB.empty(g.shape, np.int8, fill=0x33) for _ in dms]
+        plan_b.run(d_g, ob, obf, batched=True)
+        for d in range(8):
+            plan_u.run(d_g, ou, dm_begin=d, dm_count=1, batched=False)
+        plan_b.push_history(d_g)
+        plan_u.push_history(d_g)
+        for d in range(8):
+            got = ob[d].download()
+            np.testing.assert_array_equal(got, ou[d].download())
+            if gi == 1 and d in (0, 3, 7):
+                chirp = orc.dm_chirp(dms[d], F_LOW, CHAN_BW, F, Ns[d])
+                ei, ef = _oracle_stream(orc, g1[:, T - Os[d]:], g2, chirp, Ns[d], Os[d])
+                _check(got, obf[d].download(), ei, ef, f"gulp 2 DM {dms[d]} N {Ns[d]}")

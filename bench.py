@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark of the skyweaver BeamformerPipeline hot path on one MI355X.
+
+Workload (one gulp = one step): 64 antennas x 64 channels (209 kHz around
+1284 MHz) x 2 pols x 65536 samples of seeded Gaussian complex int8 (sigma
+16, TAFTP order, resident in HBM), 800 random-phase int8 beams, one coherent
+DM of 50 pc/cm^3 (FFT length 16384, overlap 10 = intra-channel smear),
+tscrunch 16, Stokes I, with the incoherent beam and per-channel statistics.
+Stages per gulp: transpose, statistics, coherent dedispersion, incoherent
+beam, coherent beam.
+
+Two forms run on the same GPU and data:
+  chosen: int8 MFMA coherent beamformer, DM-batched dedispersion (the value)
+  plain:  VALU v_dot4 beamformer, one DM per dedispersion launch
+and the CPU oracle (oracle/, -O3, OpenMP/threads) runs a bounded sample of
+the same gulp on the host cores.
+
+Prints one JSON line.  metric: beam-samples/s (beams x channels x input
+samples per second, summed over coherent DMs); x-real-time = gulp duration
+(65536 x 4096/856 MHz) / wall time per gulp.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# workload: 64 antennas / 64 channels / 65536 samples / 800 beams / DM 50
+A, F, T, B, TS = 64, 64, 65536, 800, 16
+DMS, FFT_LEN = [50.0], [16384]
+CFREQ, BW = 1284e6, 13.375e6
+TSAMP = 4096 / 856e6
+SEED_V, SEED_W = 0x5EED0002, 0x5EED1002
+INT8_PEAK_TOPS = 5000.0     # MI355X dense int8 MFMA
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_form(sw, d_taftp, weights, form, batched, steps, warmup, capture=False):
+    p = sw.BeamformerPipeline(A, F, B, T, TS, DMS, CFREQ, BW, stokes="I", fft_len=FFT_LEN, form=form,
+                              batched=batched)
+    p.set_weights(weights)
+    p.init(A)
+    captured = {}
+    if capture:
+        def cb_sink(blk):
+            if captured.get("armed"):
+                captured["tfb"] = sw.download_block(blk)
+        p.set_sinks(cb=cb_sink)
+    for _ in range(warmup):
+        p(d_taftp)
+    stage = {k: 0.0 for k in sw.STAGES}
+    sw.synchronize()
+    e0, e1 = sw.Event(), sw.Event()
+    e0.record(p.stream)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        p(d_taftp)
+        for k, v in p.stage_times().items():
+            stage[k] += v / steps
+    e1.record(p.stream)
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    ms = e0.elapsed_ms(e1) / steps
+    if capture:
+        captured["armed"] = True
+        p(d_taftp)
+    p.close()
+    return ms, wall, stage, captured.get("tfb")
+
+
+def cpu_baseline(sw, orc, taftp, weights, nchan_sample, nthreads):
+    """The oracle's full per-gulp path on `nchan_sample` channels."""
+    from concurrent.futures import ThreadPoolExecutor
+    H = T // 256
+    sub = np.ascontiguousarray(taftp[:, :, :nchan_sample])
+    w = np.ascontiguousarray(weights[:nchan_sample])
+    lowf = CFREQ - BW / 2
+    chbw = BW / F
+    O = 10
+    chirp = orc.dm_chirp(DMS[0], lowf, chbw, F, FFT_LEN[0])
+    ts = TS
+    scale = np.full(nchan_sample, 1e8, np.float32)
+    offset = np.full(nchan_sample, 1e9, np.float32)
+    antw = np.ones((1, A), np.float32)
+    bmap = np.zeros(B, np.int32)
+    t0 = time.perf_counter()
+    ftpa = orc.transpose(sub, A)                               # [c][T][2][A][2]
+
+    def per_channel(c):
+        x = ftpa[c:c + 1]
+        orc.statistics(x)
+        hist = np.zeros((O, 2, A, 2), np.int8)
+        _, dd = orc.codedisp_channel(np.concatenate([hist, x[0]]), chirp[c], FFT_LEN[0], O, T,
+                                     want_float=False, nthreads=1)
+        dd = dd[None]
+        raw, _ = orc.ib_beamform(dd, ts, scale[c:c + 1], offset[c:c + 1], antw, 1, "I")
+        orc.cb_beamform(dd, w[c:c + 1], ts, scale[c:c + 1], offset[c:c + 1], antw, bmap, 1, "I", True)
+        return c
+
+    with ThreadPoolExecutor(max_workers=nthreads) as ex:
+        list(ex.map(per_channel, range(nchan_sample)))
+    return time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-channels", type=int, default=0, help="channels in the CPU sample (0 = auto)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the path is built for one GPU); --gpus {args.gpus} is not "
+              "supported", file=sys.stderr)
+        sys.exit(2)
+
+    import skyweaver_amd as sw
+    from skyweaver_amd import DeviceBuffer
+    if sw.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    H = T // 256
+    log("generating synthetic input")
+    taftp = sw.synth_voltages((H, A, F, 256, 2, 2), seed=SEED_V, sigma=16.0, nthreads=16)
+    weights = sw.synth_weights((F, B, A, 2), seed=SEED_W)
+    d_taftp = DeviceBuffer.from_host(taftp)
+
+    log("chosen form (MFMA beamformer, batched DMs)")
+    ms, wall, stage, tfb_c = run_form(sw, d_taftp, weights, "mfma", True, args.steps, args.warmup,
+                                      capture=not args.no_plain)
+    plain = None
+    if not args.no_plain:
+        log("plain form (v_dot4 beamformer, one DM per launch)")
+        psteps = max(2, args.steps // 4)
+        pms, pwall, pstage, tfb_p = run_form(sw, d_taftp, weights, "plain", False, psteps, 1, capture=True)
+        plain = {"ms_per_step": round(pms, 4), "value": B * F * T * len(DMS) / (pms * 1e-3),
+                 "steps": psteps, "stage_ms": {k: round(v, 4) for k, v in pstage.items()},
+                 "outputs_bit_identical_to_chosen": bool(np.array_equal(tfb_c, tfb_p))}
+
+    value = B * F * T * len(DMS) / (ms * 1e-3)
+    xrt = T * TSAMP / (ms * 1e-3)
+    cb_ms = stage["coherent_beam"]
+    ops = 8.0 * F * B * T * 2 * A           # 4 int8 MACs per complex MAC, 2 ops per MAC
+    achieved = ops / (cb_ms * 1e-3) / 1e12
+    roofline = {"kernel": "cb_mfma_kernel", "bound": "mfma", "achieved": round(achieved, 1),
+                "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4),
+                "traffic": None, "ops_per_launch": ops, "avg_launch_ms": round(cb_ms, 4),
+                "note": "int8 operations (1 MAC = 2 ops); peak is the dense int8 MFMA rate"}
+    hbm = {}
+    for k, nbytes in (("transpose", 2 * T * F * 2 * A * 2), ("statistics", T * F * 2 * A * 2),
+                      ("incoherent_beam", T * F * 2 * A * 2)):
+        if stage[k] > 0:
+            gbs = nbytes / (stage[k] * 1e-3) / 1e9
+            hbm[k] = {"GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        nthreads = min(16, os.cpu_count() or 1)
+        nchan = args.cpu_channels or max(1, min(F, nthreads))
+        log(f"CPU oracle sample: {nchan} channels on {nthreads} threads")
+        secs = cpu_baseline(sw, oracle, taftp, weights, nchan, nthreads)
+        cpu_value = B * nchan * T * len(DMS) / secs
+        cpu = {"value": cpu_value, "unit": "beam-samples/s", "cores": nthreads, "kind": "port",
+               "sample": f"{nchan} of {F} channels of one gulp (transpose, statistics, FP64 dedispersion, "
+                         f"incoherent and coherent beams), {secs:.1f} s; work is linear in channels",
+               "seconds": round(secs, 2), "xrealtime": T * TSAMP / (secs * F / nchan)}
+
+    out = {
+        "metric": "beam-samples/s",
+        "value": value,
+        "unit": "beam-samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic",
+        "config": {"workload": "64 antennas x 64 channels x 2 pols x 65536 samples, 800 beams, coherent DM 50 "
+                               "(FFT 16384, overlap 10), tscrunch 16, Stokes I, incoherent beam + statistics",
+                   "antennas": A, "channels": F, "samples_per_gulp": T, "beams": B, "tscrunch": TS,
+                   "coherent_dms": DMS, "fft_len": FFT_LEN, "form": "int8 MFMA + batched DMs",
+                   "parallelism": "single GPU"},
+        "xrealtime": round(xrt, 2),
+        "wall_ms_per_step": round(wall, 4),
+        "stage_ms": {k: round(v, 4) for k, v in stage.items()},
+        "roofline": roofline,
+        "hbm_kernels": hbm,
+        "plain": plain,
+        "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -176,6 +176,7 @@ typedef struct sw_codedisp_config {
     const float* dms;  /* [ndms] pc cm^-3 */
     const int* fft_len;/* [ndms] or NULL */
     const int* overlap;/* [ndms] or NULL */
+    int input_fpat;    /* 0: inputs and history are FTPA; 1: time-major FPAT */
 } sw_codedisp_config;
 typedef struct sw_codedisp sw_codedisp;
 
@@ -195,6 +196,104 @@ int sw_codedisp_run(sw_codedisp* plan, const void* d_ftpa, void* const* d_out,
 int sw_codedisp_push_history(sw_codedisp* plan, const void* d_ftpa, void* stream);
 /* Replace the history ([F][Omax][2][A] complex int8, Omax = largest overlap). */
 int sw_codedisp_set_history(sw_codedisp* plan, const void* d_hist, void* stream);
+
+/* ----------------------------------------------------- time-major layout
+ * Inside the pipeline a gulp is held time-major, FPAT [F][P][A][T] complex
+ * int8, so that the dedisperser's transforms read contiguous time runs.
+ * No reference counterpart (the reference keeps FTPA and transposes per
+ * channel inside cuFFT's strided plans, sw/src/CoherentDedisperser.cu:106-131). */
+int sw_transpose_fpat(const void* d_taftp, void* d_fpat, int nheap_groups, int nant_in,
+                      int nant_out, int nchans, void* stream);
+/* [F][rows][cols] -> [F][cols][rows] of 2-byte elements (FTPA <-> FPAT). */
+int sw_corner_turn(const void* d_src, void* d_dst, int nchans, int rows, int cols, void* stream);
+/* Statistics of an FPAT gulp; same results as sw_statistics on FTPA. */
+int sw_statistics_fpat(const void* d_fpat, float* d_stats, int nchans, int nsamples, int npol,
+                       int nantennas, void* d_workspace, void* stream);
+
+/* ---------------------------------------------------- BeamformerPipeline
+ * BeamformerPipeline<CBH, IBH, StatsH, Traits>: ctor(PipelineConfig const&,
+ * handlers), init(ObservationHeader const&, utc_offset),
+ * operator()(TAFTPVoltagesH<char2> const&), sw/BeamformerPipeline.cuh:48-67,
+ * sw/detail/BeamformerPipeline.cu:44-301.  Per gulp: transpose, statistics,
+ * scalings (first gulp), coherent dedispersion of every DM, then per DM the
+ * incoherent and coherent beams and the handler calls.  All buffers are
+ * allocated at creation; processing runs on one stream owned by the
+ * pipeline and returns after the gulp completes. */
+enum sw_block_kind {
+    SW_BLOCK_CB_TFB = 0,    /* coherent beams  [T/ts][F][B] int8 (x4 IQUV)   */
+    SW_BLOCK_IB_BTF = 1,    /* incoherent beam [S][T/ts][F] int8 (x4 IQUV)   */
+    SW_BLOCK_STATS_FPA = 2, /* statistics      [F][P][A] {mean,std,skew,kurt} */
+    SW_BLOCK_TDB = 3        /* incoherently dedispersed [T'][D][B] int8      */
+};
+
+/* Metadata of a block handed to a sink (the DescribedVector fields the
+ * reference handlers read, sw/DescribedVector.hpp:97-573). */
+typedef struct sw_block {
+    int kind;              /* enum sw_block_kind */
+    const void* data;      /* valid only during the sink call */
+    int on_device;         /* 1: device pointer, 0: host pointer */
+    size_t bytes;
+    int ndims;
+    size_t dims[4];        /* slowest -> fastest */
+    char order[8];         /* "TFB", "BTF", "FPA", "TDB" */
+    int dm_idx;            /* coherent DM index */
+    float reference_dm;
+    double tsamp;          /* s */
+    double utc_offset;     /* s, relative to the observation's UTC_START */
+    const float* dms;      /* incoherent DMs of a TDB block, else NULL */
+    int ndms;
+} sw_block;
+
+typedef int (*sw_sink_fn)(void* user, const sw_block* block);
+
+typedef struct sw_pipeline_config {
+    int nantennas;          /* A (padded; multiple of 4) */
+    int nchans;             /* F */
+    int nbeams;             /* B */
+    int nsamples;           /* gulp length T, multiple of 256 */
+    int tscrunch;           /* coherent and incoherent beam tscrunch */
+    int stokes;             /* enum sw_stokes */
+    int ib_subtract;        /* 1: subtract the incoherent beam from the coherent beams */
+    float output_level;     /* PipelineConfig::output_level (default 24) */
+    double centre_frequency;/* Hz, centre of the subband */
+    double bandwidth;       /* Hz, bandwidth of the subband */
+    int total_nchans;       /* channels of the full band (timestamp step), e.g. 4096 */
+    int ndms;
+    const float* coherent_dms;
+    const int* fft_len;     /* per DM or NULL (see sw_codedisp_config) */
+    const int* overlap;     /* per DM or NULL */
+    int cb_form;            /* enum sw_cb_form */
+    int batched_dms;        /* 1: share forward FFTs across DMs */
+} sw_pipeline_config;
+
+typedef struct sw_obs_header {
+    int nantennas;          /* antennas present in the TAFTP input (<= nantennas) */
+    double utc_start;       /* unix seconds */
+    double sample_clock;    /* Hz */
+} sw_obs_header;
+
+typedef struct sw_pipeline sw_pipeline;
+
+int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** pipeline);
+int sw_pipeline_destroy(sw_pipeline* pipeline);
+int sw_pipeline_set_sinks(sw_pipeline* pipeline, sw_sink_fn cb, void* cb_user, sw_sink_fn ib,
+                          void* ib_user, sw_sink_fn stats, void* stats_user);
+/* Static beamforming weights [F][B][A] complex int8 (host), used for every
+ * gulp until replaced. */
+int sw_pipeline_set_weights(sw_pipeline* pipeline, const void* h_weights);
+/* Beamsets: nbeamsets, beam -> beamset map [B], antenna weights [S][A]. */
+int sw_pipeline_set_beamsets(sw_pipeline* pipeline, int nbeamsets, const int* h_map,
+                             const float* h_antenna_weights);
+int sw_pipeline_init(sw_pipeline* pipeline, const sw_obs_header* header);
+/* One gulp of TAFTP [T/256][A_in][F][256][2] complex int8; on_device = 1 if
+ * taftp is a device pointer (no host transfer).  Returns 0 (the reference's
+ * operator() always returns false). */
+int sw_pipeline_process(sw_pipeline* pipeline, const void* taftp, size_t bytes, int on_device);
+/* Per-stage device times of the last gulp (ms), in the order transpose,
+ * statistics, dedispersion, incoherent beam, coherent beam, other. */
+int sw_pipeline_stage_times(const sw_pipeline* pipeline, float* ms6);
+/* The processing stream (hipStream_t) the pipeline enqueues work on. */
+void* sw_pipeline_stream(const sw_pipeline* pipeline);
 
 /* -------------------------------------------------- synthetic input (bench)
  * Seeded Gaussian complex int8: xoshiro256** seeded through splitmix64,

@@ -57,7 +57,8 @@ class CodedispConfig(ctypes.Structure):
     _fields_ = [("nchans", ctypes.c_int), ("nantennas", ctypes.c_int), ("nsamples", ctypes.c_int),
                 ("low_freq", ctypes.c_double), ("chan_bw", ctypes.c_double), ("tsamp", ctypes.c_double),
                 ("ndms", ctypes.c_int), ("dms", ctypes.POINTER(ctypes.c_float)),
-                ("fft_len", ctypes.POINTER(ctypes.c_int)), ("overlap", ctypes.POINTER(ctypes.c_int))]
+                ("fft_len", ctypes.POINTER(ctypes.c_int)), ("overlap", ctypes.POINTER(ctypes.c_int)),
+                ("input_fpat", ctypes.c_int)]
 
 
 _vp = ctypes.c_void_p
@@ -83,6 +84,9 @@ def _declare(L):
     L.sw_cb_beamform_packed.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(BfDims), _vp]
     L.sw_synth_gaussian_c8.argtypes = [_vp, _sz, ctypes.c_uint64, ctypes.c_double, _c]
     L.sw_synth_phase_weights.argtypes = [_vp, _sz, ctypes.c_uint64]
+    L.sw_transpose_fpat.argtypes = [_vp, _vp, _c, _c, _c, _c, _vp]
+    L.sw_corner_turn.argtypes = [_vp, _vp, _c, _c, _c, _vp]
+    L.sw_statistics_fpat.argtypes = [_vp, _vp, _c, _c, _c, _c, _vp, _vp]
     _pvp = ctypes.POINTER(ctypes.c_void_p)
     L.sw_codedisp_create.argtypes = [ctypes.POINTER(CodedispConfig), _vp, _pvp]
     L.sw_codedisp_destroy.argtypes = [_vp]
@@ -256,6 +260,19 @@ def transpose(taftp, ftpa, nheaps, nant_in, nant_out, nchans, stream=None):
     check(lib().sw_transpose(_ptr(taftp), _ptr(ftpa), nheaps, nant_in, nant_out, nchans, _stream(stream)))
 
 
+def transpose_fpat(taftp, fpat, nheaps, nant_in, nant_out, nchans, stream=None):
+    check(lib().sw_transpose_fpat(_ptr(taftp), _ptr(fpat), nheaps, nant_in, nant_out, nchans, _stream(stream)))
+
+
+def corner_turn(src, dst, nchans, rows, cols, stream=None):
+    check(lib().sw_corner_turn(_ptr(src), _ptr(dst), nchans, rows, cols, _stream(stream)))
+
+
+def statistics_fpat(fpat, stats, nchans, nsamples, nantennas, workspace=None, stream=None):
+    check(lib().sw_statistics_fpat(_ptr(fpat), _ptr(stats), nchans, nsamples, 2, nantennas, _ptr(workspace),
+                                   _stream(stream)))
+
+
 def statistics(ftpa, stats, nchans, nsamples, nantennas, workspace=None, stream=None):
     check(lib().sw_statistics(_ptr(ftpa), _ptr(stats), nchans, nsamples, 2, nantennas, _ptr(workspace),
                               _stream(stream)))
@@ -319,7 +336,7 @@ class CoherentDedisperser:
     """Overlap-save coherent dedispersion plan (sw_codedisp_* in the C ABI)."""
 
     def __init__(self, nchans, nantennas, nsamples, low_freq, chan_bw, tsamp, dms, fft_len=None,
-                 overlap=None, stream=None):
+                 overlap=None, stream=None, input_fpat=False):
         import numpy as np
         self._dms = np.ascontiguousarray(dms, np.float32)
         n = len(self._dms)
@@ -328,7 +345,7 @@ class CoherentDedisperser:
         cfg = CodedispConfig(nchans, nantennas, nsamples, low_freq, chan_bw, tsamp, n,
                              self._dms.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                              self._fft.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                             self._ov.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+                             self._ov.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 1 if input_fpat else 0)
         h = ctypes.c_void_p()
         check(lib().sw_codedisp_create(ctypes.byref(cfg), _stream(stream), ctypes.byref(h)))
         self.handle = h.value
@@ -365,3 +382,140 @@ class CoherentDedisperser:
             self.close()
         except Exception:
             pass
+
+
+class PipelineConfigC(ctypes.Structure):
+    _fields_ = [("nantennas", ctypes.c_int), ("nchans", ctypes.c_int), ("nbeams", ctypes.c_int),
+                ("nsamples", ctypes.c_int), ("tscrunch", ctypes.c_int), ("stokes", ctypes.c_int),
+                ("ib_subtract", ctypes.c_int), ("output_level", ctypes.c_float),
+                ("centre_frequency", ctypes.c_double), ("bandwidth", ctypes.c_double),
+                ("total_nchans", ctypes.c_int), ("ndms", ctypes.c_int),
+                ("coherent_dms", ctypes.POINTER(ctypes.c_float)), ("fft_len", ctypes.POINTER(ctypes.c_int)),
+                ("overlap", ctypes.POINTER(ctypes.c_int)), ("cb_form", ctypes.c_int), ("batched_dms", ctypes.c_int)]
+
+
+class ObsHeaderC(ctypes.Structure):
+    _fields_ = [("nantennas", ctypes.c_int), ("utc_start", ctypes.c_double), ("sample_clock", ctypes.c_double)]
+
+
+class BlockC(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("data", ctypes.c_void_p), ("on_device", ctypes.c_int),
+                ("bytes", ctypes.c_size_t), ("ndims", ctypes.c_int), ("dims", ctypes.c_size_t * 4),
+                ("order", ctypes.c_char * 8), ("dm_idx", ctypes.c_int), ("reference_dm", ctypes.c_float),
+                ("tsamp", ctypes.c_double), ("utc_offset", ctypes.c_double),
+                ("dms", ctypes.POINTER(ctypes.c_float)), ("ndms", ctypes.c_int)]
+
+
+SINK_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(BlockC))
+
+
+def _declare_pipeline(L):
+    _pvp = ctypes.POINTER(ctypes.c_void_p)
+    L.sw_pipeline_create.argtypes = [ctypes.POINTER(PipelineConfigC), _pvp]
+    L.sw_pipeline_destroy.argtypes = [_vp]
+    L.sw_pipeline_set_sinks.argtypes = [_vp, SINK_FN, _vp, SINK_FN, _vp, SINK_FN, _vp]
+    L.sw_pipeline_set_weights.argtypes = [_vp, _vp]
+    L.sw_pipeline_set_beamsets.argtypes = [_vp, _c, _vp, _vp]
+    L.sw_pipeline_init.argtypes = [_vp, ctypes.POINTER(ObsHeaderC)]
+    L.sw_pipeline_process.argtypes = [_vp, _vp, _sz, _c]
+    L.sw_pipeline_stage_times.argtypes = [_vp, ctypes.POINTER(ctypes.c_float)]
+    L.sw_pipeline_stream.argtypes = [_vp]
+    L.sw_pipeline_stream.restype = _vp
+
+
+STAGES = ("transpose", "statistics", "dedispersion", "incoherent_beam", "coherent_beam", "other")
+
+
+class BeamformerPipeline:
+    """Native per-gulp pipeline (sw_pipeline_* in the C ABI)."""
+
+    def __init__(self, nantennas, nchans, nbeams, nsamples, tscrunch, dms, centre_frequency, bandwidth,
+                 stokes="I", ib_subtract=True, output_level=24.0, fft_len=None, overlap=None, form="mfma",
+                 batched=True, total_nchans=4096):
+        import numpy as np
+        L = lib()
+        if not hasattr(L, "_pipeline_declared"):
+            _declare_pipeline(L)
+            L._pipeline_declared = True
+        n = len(dms)
+        self._dms = np.ascontiguousarray(dms, np.float32)
+        self._fft = np.ascontiguousarray(fft_len, np.int32) if fft_len is not None else None
+        self._ov = np.ascontiguousarray(overlap, np.int32) if overlap is not None else None
+        ip = ctypes.POINTER(ctypes.c_int)
+        cfg = PipelineConfigC(nantennas, nchans, nbeams, nsamples, tscrunch, STOKES[stokes], 1 if ib_subtract else 0,
+                              output_level, centre_frequency, bandwidth, total_nchans, n,
+                              self._dms.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                              self._fft.ctypes.data_as(ip) if self._fft is not None else None,
+                              self._ov.ctypes.data_as(ip) if self._ov is not None else None,
+                              FORMS[form], 1 if batched else 0)
+        h = ctypes.c_void_p()
+        check(L.sw_pipeline_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h.value
+        self._sinks = []
+
+    def set_weights(self, weights_np):
+        import numpy as np
+        w = np.ascontiguousarray(weights_np, np.int8)
+        check(lib().sw_pipeline_set_weights(self.handle, w.ctypes.data))
+
+    def set_beamsets(self, bmap, antenna_weights):
+        import numpy as np
+        m = np.ascontiguousarray(bmap, np.int32)
+        w = np.ascontiguousarray(antenna_weights, np.float32)
+        check(lib().sw_pipeline_set_beamsets(self.handle, w.shape[0], m.ctypes.data, w.ctypes.data))
+
+    def set_sinks(self, cb=None, ib=None, stats=None):
+        """Python callables f(block: BlockC) -> None, invoked synchronously."""
+        def wrap(fn):
+            if fn is None:
+                return SINK_FN()
+            def _f(user, blk):
+                fn(blk.contents)
+                return 0
+            return SINK_FN(_f)
+        self._sinks = [wrap(cb), wrap(ib), wrap(stats)]
+        check(lib().sw_pipeline_set_sinks(self.handle, self._sinks[0], None, self._sinks[1], None,
+                                          self._sinks[2], None))
+
+    def init(self, nantennas, utc_start=1708082229.0, sample_clock=1712e6):
+        hdr = ObsHeaderC(nantennas, utc_start, sample_clock)
+        check(lib().sw_pipeline_init(self.handle, ctypes.byref(hdr)))
+
+    def __call__(self, taftp, nbytes=None):
+        """Process one gulp: a DeviceBuffer (device-resident) or a numpy array (host)."""
+        if isinstance(taftp, DeviceBuffer):
+            check(lib().sw_pipeline_process(self.handle, taftp.ptr, nbytes or taftp.nbytes, 1))
+        else:
+            import numpy as np
+            a = np.ascontiguousarray(taftp)
+            check(lib().sw_pipeline_process(self.handle, a.ctypes.data, a.nbytes, 0))
+        return False
+
+    def stage_times(self):
+        buf = (ctypes.c_float * 6)()
+        check(lib().sw_pipeline_stage_times(self.handle, buf))
+        return dict(zip(STAGES, list(buf)))
+
+    @property
+    def stream(self):
+        return lib().sw_pipeline_stream(self.handle)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sw_pipeline_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def download_block(blk):
+    """Copy a sink block (device memory) to a numpy array of its shape."""
+    import numpy as np
+    out = np.empty(blk.bytes, np.uint8)
+    check(lib().sw_memcpy_dtoh(out.ctypes.data, blk.data, blk.bytes, None))
+    check(lib().sw_stream_synchronize(None))
+    return out

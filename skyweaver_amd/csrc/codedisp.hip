@@ -6,23 +6,24 @@
 // keep [O/2, N-O/2).  The chirp is DMResponse (:313-335):
 //   nu0 = f_low + c*bw_c - bw_c/2, nu = k*bw_c/N for all k in [0, N),
 //   phase = nu^2 (-2 pi K DM) / ((nu0 + nu) nu0^2), H = exp(i phase).
-// Unlike the reference pipeline (FFT length tied to the gulp, F4/F5 of the
-// design notes), the FFT length N and the overlap O are per-DM parameters
-// and the stream is processed as true overlap-save: the input of channel f
-// is the previous gulp's last O samples followed by the current gulp;
-// segment s covers input [s(N-O), s(N-O)+N) (zero past the end) and yields
-// output samples [s(N-O), (s+1)(N-O)) from its positions [O/2, N-O/2).
-// Output sample j is the dedispersed input sample j - O/2.
+// The FFT length N and the overlap O are per-DM parameters and the stream is
+// processed as true overlap-save: the input of a channel is the previous
+// gulp's last O samples followed by the current gulp; segment s covers input
+// [s(N-O), s(N-O)+N) (zero past the end) and yields output samples
+// [s(N-O), (s+1)(N-O)) from its positions [O/2, N-O/2).  Output sample j is
+// the dedispersed input sample j - O/2.
 //
-// Kernels (all columns = (pol, antenna) pairs of one channel):
-//  * N <= 4096: one workgroup per (channel, segment, column group) keeps C
-//    columns x N points in LDS: load int8 -> forward FFT -> for each DM of the
-//    batch: multiply by the chirp, inverse FFT, quantise, store.  The forward
-//    FFT is shared by every DM of a launch.
-//  * N > 4096 = RP x L (L = 4096): a decimation-in-frequency radix-RP stage
-//    (int8 in, FP32 out), the same LDS kernel on each of the RP sub-sequences
-//    (chirp permuted to match), and a decimation-in-time radix-RP stage that
-//    quantises.
+// Data are time-major ([F][P*A][T] complex int8, one column per (pol,
+// antenna)) so that every transform reads and writes contiguous runs:
+//  * N <= 16384: one workgroup per (channel, segment, group of C columns)
+//    keeps C x N points in LDS: load -> forward FFT -> per DM of the launch:
+//    chirp, inverse FFT, quantise, store.  With two LDS buffers (C*N <= 8192)
+//    the forward FFT is shared by every DM of the launch; with one (N = 16384)
+//    it is recomputed per DM.
+//  * N > 16384 = RP x L: a radix-RP decimation-in-frequency stage, the LDS
+//    kernel on each of the RP sub-sequences (chirp permuted to match) and a
+//    radix-RP decimation-in-time stage that quantises.
+// Outputs are corner-turned to FTPA for the beamformers.
 #include "fft.hpp"
 
 #include <algorithm>
@@ -32,26 +33,26 @@
 #include <vector>
 
 namespace sw {
+int corner_turn(const void* src, void* dst, int nchans, int R, int C, hipStream_t st);
+
 namespace {
 
 using namespace fft;
 
-constexpr int kNT      = 256;
-constexpr int kTile    = 4096;   // C * L complex values per LDS buffer
-constexpr int kMaxL    = 4096;
-constexpr int kMaxDm   = 8;      // DMs per launch
+constexpr int kMaxL  = 16384;   // longest single-workgroup transform
+constexpr int kMaxDm = 8;       // DMs per launch
 constexpr double kDmConst = 4.148806423e15;   // Hz^2 pc^-1 cm^3 s
 
 struct CdParams {
-    const int8_t* cur;   // [F][T][PA] complex int8
-    const int8_t* hist;  // [F][Oh][PA]
-    const float2* zin;   // [F*nseg][RP][L][PA] (float input stage)
+    const int8_t* cur;   // [F][PA][T] complex int8
+    const int8_t* hist;  // [F][PA][Oh]
+    float2* z;           // [F*nseg][RP][PA][L] (N > kMaxL)
     int T, O, Oh, hop, nseg, PA, F, RP, N;
     int ndm;
     const float2* chirp[kMaxDm];  // [F][RP][L] (RP == 1: natural order)
-    int8_t* out[kMaxDm];          // [F][T][PA] complex int8
-    float2* outf[kMaxDm];         // optional [F][T][PA] y/N
-    float2* vout[kMaxDm];         // [F*nseg][RP][L][PA] (float output stage)
+    int8_t* out[kMaxDm];          // [F][PA][T] complex int8
+    float2* outf[kMaxDm];         // optional FTPA-ordered [F][T][PA] y/N
+    float2* v[kMaxDm];            // [F*nseg][RP][PA][L] (N > kMaxL)
     const float2* tw;             // exp(-2 pi i m / L)
     const float2* twN;            // exp(-2 pi i m / N)
 };
@@ -69,86 +70,73 @@ __device__ __forceinline__ void store_c8(int8_t* p, float2 y)
     *reinterpret_cast<uint16_t*>(p) = (uint16_t)((re & 0xff) | ((im & 0xff) << 8));
 }
 
-// Map a workgroup index to (unit, column group) so that all column groups of
-// a unit (which read the same input rows) run on the same XCD.
-__device__ __forceinline__ void decode_block(int units, int G, int& u, int& g)
+// input sample tau (relative to the gulp start) of column col of channel f
+__device__ __forceinline__ float2 input_sample(const CdParams& p, int f, int col, int tau)
 {
-    const int bid = blockIdx.x;
-    if((units & 7) == 0) {
-        const int xcd = bid & 7, k = bid >> 3;
-        u = xcd + 8 * (k / G);
-        g = k % G;
-    } else {
-        u = bid / G;
-        g = bid % G;
-    }
+    if(tau < 0) return load_c8(p.hist + (((size_t)f * p.PA + col) * p.Oh + p.Oh + tau) * 2);
+    if(tau < p.T) return load_c8(p.cur + (((size_t)f * p.PA + col) * p.T + tau) * 2);
+    return make_float2(0.0f, 0.0f);
 }
 
-template <int L, int C, bool IN_F32, bool OUT_F32>
-__global__ void __launch_bounds__(kNT) codedisp_lds_kernel(CdParams p)
+template <int L, int C, int NBUF, int NT, bool F32>
+__global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
 {
-    __shared__ float2 X[C * padded(L)];
-    __shared__ float2 W[C * padded(L)];
-    const int G     = p.PA / C;
-    const int units = p.F * p.nseg * p.RP;
-    int u, g;
-    decode_block(units, G, u, g);
+    __shared__ float2 lds[NBUF * C * padded(L)];
+    float2* X = lds;
+    float2* W = (NBUF == 2) ? lds + C * padded(L) : lds;
+    const int G  = p.PA / C;
+    const int bid = blockIdx.x;
+    const int u = bid / G, g = bid % G;  // unit = (channel, segment, sub-sequence)
     const int fs = u / p.RP, r = u % p.RP;
     const int f = fs / p.nseg, s = fs % p.nseg;
     const int tid = threadIdx.x;
+    const float inv_n = 1.0f / (float)p.N;  // exact: N is a power of two
 
-    // ---- load C columns x L points
-    for(int i = tid; i < C * L; i += kNT) {
-        const int n = i / C, c = i % C;
-        const int col = g * C + c;
-        float2 v;
-        if constexpr(IN_F32) {
-            v = p.zin[(((size_t)fs * p.RP + r) * L + n) * p.PA + col];
-        } else {
-            const int tau = s * p.hop - p.O + n;
-            if(tau < 0)
-                v = load_c8(p.hist + (((size_t)f * p.Oh + p.Oh + tau) * p.PA + col) * 2);
-            else if(tau < p.T)
-                v = load_c8(p.cur + (((size_t)f * p.T + tau) * p.PA + col) * 2);
+    auto load = [&]() {
+        for(int i = tid; i < C * L; i += NT) {
+            const int c = i / L, n = i % L;
+            const int col = g * C + c;
+            float2 v;
+            if constexpr(F32)
+                v = p.z[(((size_t)fs * p.RP + r) * p.PA + col) * L + n];
             else
-                v = make_float2(0.0f, 0.0f);
+                v = input_sample(p, f, col, s * p.hop - p.O + n);
+            X[c * padded(L) + pad(n)] = v;
         }
-        X[c * padded(L) + pad(n)] = v;
-    }
-    __syncthreads();
-    fft_lds<false, L, C, kNT>(X, p.tw);
+        __syncthreads();
+        fft_lds<false, L, C, NT>(X, p.tw);
+    };
 
-    const float inv_n = 1.0f / (float)p.N;   // exact: N is a power of two
+    load();
     for(int d = 0; d < p.ndm; ++d) {
+        if(NBUF == 1 && d > 0) load();
         const float2* H = p.chirp[d] + ((size_t)f * p.RP + r) * L;
-        for(int i = tid; i < C * L; i += kNT) {
+        for(int i = tid; i < C * L; i += NT) {
             const int c = i / L, k = i % L;
             W[c * padded(L) + pad(k)] = cmul(X[c * padded(L) + pad(k)], H[k]);
         }
         __syncthreads();
-        fft_lds<true, L, C, kNT>(W, p.tw);
-        if constexpr(OUT_F32) {
-            float2* V = p.vout[d];
-            for(int i = tid; i < C * L; i += kNT) {
-                const int n = i / C, c = i % C;
+        fft_lds<true, L, C, NT>(W, p.tw);
+        if constexpr(F32) {
+            float2* V = p.v[d];
+            for(int i = tid; i < C * L; i += NT) {
+                const int c = i / L, n = i % L;
                 const int col = g * C + c;
-                // exp(+2 pi i r n / N)
-                const float2 w = cconj(p.twN[r * n]);
-                V[(((size_t)fs * p.RP + r) * L + n) * p.PA + col] = cmul(W[c * padded(L) + pad(n)], w);
+                const float2 w = cconj(p.twN[r * n]);  // exp(+2 pi i r n / N)
+                V[(((size_t)fs * p.RP + r) * p.PA + col) * L + n] = cmul(W[c * padded(L) + pad(n)], w);
             }
         } else {
-            int8_t* out = p.out[d];
+            int8_t* out  = p.out[d];
             float2* outf = p.outf[d];
             const int half = p.O / 2;
-            for(int i = tid; i < C * p.hop; i += kNT) {
-                const int m = i / C, c = i % C;
+            for(int i = tid; i < C * p.hop; i += NT) {
+                const int c = i / p.hop, m = i % p.hop;
                 const int j = s * p.hop + m;
                 if(j < p.T) {
                     const int col = g * C + c;
                     const float2 y = cscale(W[c * padded(L) + pad(half + m)], inv_n);
-                    const size_t o = ((size_t)f * p.T + j) * p.PA + col;
-                    store_c8(out + o * 2, y);
-                    if(outf) outf[o] = y;
+                    store_c8(out + (((size_t)f * p.PA + col) * p.T + j) * 2, y);
+                    if(outf) outf[((size_t)f * p.T + j) * p.PA + col] = y;
                 }
             }
         }
@@ -162,50 +150,38 @@ template <int RP>
 __global__ void __launch_bounds__(256) codedisp_dif_kernel(CdParams p, int L)
 {
     const size_t idx   = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t total = (size_t)p.F * p.nseg * L * p.PA;
+    const size_t total = (size_t)p.F * p.nseg * p.PA * L;
     if(idx >= total) return;
-    const int col  = (int)(idx % p.PA);
-    const size_t t = idx / p.PA;
-    const int n    = (int)(t % L);
-    const int fs   = (int)(t / L);
+    const int n    = (int)(idx % L);
+    const size_t t = idx / L;
+    const int col  = (int)(t % p.PA);
+    const int fs   = (int)(t / p.PA);
     const int f = fs / p.nseg, s = fs % p.nseg;
     float2 v[RP];
 #pragma unroll
-    for(int q = 0; q < RP; ++q) {
-        const int tau = s * p.hop - p.O + n + q * L;
-        if(tau < 0)
-            v[q] = load_c8(p.hist + (((size_t)f * p.Oh + p.Oh + tau) * p.PA + col) * 2);
-        else if(tau < p.T)
-            v[q] = load_c8(p.cur + (((size_t)f * p.T + tau) * p.PA + col) * 2);
-        else
-            v[q] = make_float2(0.0f, 0.0f);
-    }
+    for(int q = 0; q < RP; ++q) v[q] = input_sample(p, f, col, s * p.hop - p.O + n + q * L);
     dft<RP, false>(v);
-    float2* Z = const_cast<float2*>(p.zin);
 #pragma unroll
-    for(int rr = 0; rr < RP; ++rr) {
-        const float2 w = (rr == 0) ? make_float2(1.0f, 0.0f) : p.twN[n * rr];
-        Z[(((size_t)fs * RP + rr) * L + n) * p.PA + col] = rr == 0 ? v[rr] : cmul(v[rr], w);
-    }
+    for(int rr = 0; rr < RP; ++rr)
+        p.z[(((size_t)fs * RP + rr) * p.PA + col) * L + n] = rr == 0 ? v[0] : cmul(v[rr], p.twN[n * rr]);
 }
 
 // Decimation-in-time last stage: y[n + q L] = sum_r V_r[n] exp(+2 pi i q r / RP),
-// then /N, quantise and keep output positions [O/2, N - O/2).
+// then /N, quantise and keep segment positions [O/2, N - O/2).
 template <int RP>
 __global__ void __launch_bounds__(256) codedisp_dit_kernel(CdParams p, int L, int d)
 {
     const size_t idx   = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t total = (size_t)p.F * p.nseg * L * p.PA;
+    const size_t total = (size_t)p.F * p.nseg * p.PA * L;
     if(idx >= total) return;
-    const int col  = (int)(idx % p.PA);
-    const size_t t = idx / p.PA;
-    const int n    = (int)(t % L);
-    const int fs   = (int)(t / L);
+    const int n    = (int)(idx % L);
+    const size_t t = idx / L;
+    const int col  = (int)(t % p.PA);
+    const int fs   = (int)(t / p.PA);
     const int f = fs / p.nseg, s = fs % p.nseg;
-    const float2* V = p.vout[d];
     float2 v[RP];
 #pragma unroll
-    for(int rr = 0; rr < RP; ++rr) v[rr] = V[(((size_t)fs * RP + rr) * L + n) * p.PA + col];
+    for(int rr = 0; rr < RP; ++rr) v[rr] = p.v[d][(((size_t)fs * RP + rr) * p.PA + col) * L + n];
     dft<RP, true>(v);
     const float inv_n = 1.0f / (float)p.N;
     const int half = p.O / 2;
@@ -215,10 +191,9 @@ __global__ void __launch_bounds__(256) codedisp_dit_kernel(CdParams p, int L, in
         if(pos >= half && pos < p.N - half) {
             const int j = s * p.hop + pos - half;
             if(j < p.T) {
-                const size_t o = ((size_t)f * p.T + j) * p.PA + col;
                 const float2 y = cscale(v[q], inv_n);
-                store_c8(p.out[d] + o * 2, y);
-                if(p.outf[d]) p.outf[d][o] = y;
+                store_c8(p.out[d] + (((size_t)f * p.PA + col) * p.T + j) * 2, y);
+                if(p.outf[d]) p.outf[d][((size_t)f * p.T + j) * p.PA + col] = y;
             }
         }
     }
@@ -252,32 +227,41 @@ __global__ void chirp_kernel(float2* out, int nchans, int N, int RP, double low_
     out[((size_t)chan * RP + r) * L + m] = make_float2((float)c, (float)s);
 }
 
-__global__ void history_kernel(int8_t* hist, const int8_t* cur, int F, int T, int Oh, int PA)
+// history[f][col][i] = gulp[f][col][T - Oh + i]
+__global__ void history_kernel(int8_t* hist, const int8_t* cur, int ncols, int T, int Oh)
 {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t total = (size_t)F * Oh * PA;
-    if(idx >= total) return;
-    const int col = (int)(idx % PA);
-    const size_t t = idx / PA;
-    const int i = (int)(t % Oh), f = (int)(t / Oh);
-    const uint16_t v =
-        *reinterpret_cast<const uint16_t*>(cur + (((size_t)f * T + T - Oh + i) * PA + col) * 2);
-    *reinterpret_cast<uint16_t*>(hist + idx * 2) = v;
+    if(idx >= (size_t)ncols * Oh) return;
+    const int i = (int)(idx % Oh);
+    const size_t col = idx / Oh;
+    *reinterpret_cast<uint16_t*>(hist + idx * 2) =
+        *reinterpret_cast<const uint16_t*>(cur + (col * T + T - Oh + i) * 2);
 }
 
 // ------------------------------------------------------------------ host
-template <bool IN_F32, bool OUT_F32>
+struct LdsShape {
+    int C, NBUF, NT;
+};
+
+LdsShape lds_shape(int L)
+{
+    if(L >= 16384) return {1, 1, 512};
+    return {8192 / L, 2, 512};
+}
+
+template <bool F32>
 int launch_lds(int L, const CdParams& p, int blocks, hipStream_t st)
 {
-#define SW_CD(LL)                                                                           \
-    case LL:                                                                                \
-        hipLaunchKernelGGL((codedisp_lds_kernel<LL, kTile / LL, IN_F32, OUT_F32>), dim3(blocks), \
-                           dim3(kNT), 0, st, p);                                            \
+#define SW_CD(LL, CC, NB, NTT)                                                               \
+    case LL:                                                                                 \
+        hipLaunchKernelGGL((codedisp_tm_kernel<LL, CC, NB, NTT, F32>), dim3(blocks), dim3(NTT), \
+                           0, st, p);                                                        \
         break;
     switch(L) {
-        SW_CD(16) SW_CD(32) SW_CD(64) SW_CD(128) SW_CD(256) SW_CD(512) SW_CD(1024) SW_CD(2048)
-        SW_CD(4096)
-    default: set_error("codedisp: unsupported sub-FFT length"); return SW_ERR_UNSUPPORTED;
+        SW_CD(64, 128, 2, 512) SW_CD(128, 64, 2, 512) SW_CD(256, 32, 2, 512)
+        SW_CD(512, 16, 2, 512) SW_CD(1024, 8, 2, 512) SW_CD(2048, 4, 2, 512)
+        SW_CD(4096, 2, 2, 512) SW_CD(8192, 1, 2, 512) SW_CD(16384, 1, 1, 512)
+    default: set_error("codedisp: unsupported transform length"); return SW_ERR_UNSUPPORTED;
     }
 #undef SW_CD
     SW_LAUNCH_CHECK();
@@ -295,6 +279,7 @@ struct CodedispGroup {
 
 struct sw_codedisp {
     int F, A, PA, T;
+    int input_fpat;
     double low_freq, chan_bw, tsamp;
     std::vector<float> dms;
     std::vector<int> N, O;
@@ -302,15 +287,18 @@ struct sw_codedisp {
     std::vector<sw::CodedispGroup> groups;
     std::vector<float2*> chirp;          // per DM
     std::map<int, float2*> twiddles;     // per length
-    int8_t* hist = nullptr;              // [F][Omax][PA] complex int8
-    float2* zbuf = nullptr;              // DIF stage output (N > 4096)
-    std::vector<float2*> vbuf;           // per DM slot of a launch (N > 4096)
-    size_t zbytes = 0;
+    int8_t* hist = nullptr;              // [F][PA][Omax] complex int8
+    int8_t* xtm  = nullptr;              // time-major copy of an FTPA input
+    std::vector<int8_t*> ytm;            // time-major outputs, one per DM slot
+    float2* zbuf = nullptr;              // DIF stage output (N > kMaxL)
+    std::vector<float2*> vbuf;           // per DM slot (N > kMaxL)
     ~sw_codedisp()
     {
         for(auto* c : chirp) (void)hipFree(c);
         for(auto& kv : twiddles) (void)hipFree(kv.second);
         (void)hipFree(hist);
+        (void)hipFree(xtm);
+        for(auto* y : ytm) (void)hipFree(y);
         (void)hipFree(zbuf);
         for(auto* v : vbuf) (void)hipFree(v);
     }
@@ -361,6 +349,7 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
     p->A = cfg->nantennas;
     p->PA = 2 * cfg->nantennas;
     p->T = cfg->nsamples;
+    p->input_fpat = cfg->input_fpat;
     p->low_freq = cfg->low_freq;
     p->chan_bw = cfg->chan_bw;
     p->tsamp = cfg->tsamp;
@@ -376,14 +365,15 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
             N = 1024;
             while(N < 16 * O) N *= 2;
         }
-        SW_CHECK_ARG((N & (N - 1)) == 0 && N >= 16 && N <= 65536, "FFT length must be a power of two in [16, 65536]");
+        SW_CHECK_ARG((N & (N - 1)) == 0 && N >= 64 && N <= 262144,
+                     "FFT length must be a power of two in [64, 262144]");
         SW_CHECK_ARG(O < N, "overlap must be smaller than the FFT length");
         p->dms.push_back(dm);
         p->N.push_back(N);
         p->O.push_back(O);
         p->Omax = std::max(p->Omax, O);
     }
-    // group DMs sharing (N, O): their forward transforms are shared
+    // group DMs sharing (N, O): one forward transform serves the whole group
     std::map<std::pair<int, int>, int> gid;
     for(int d = 0; d < (int)p->dms.size(); ++d) {
         auto key = std::make_pair(p->N[d], p->O[d]);
@@ -395,21 +385,21 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
             g.RP = g.N / g.L;
             g.hop = g.N - g.O;
             g.nseg = (p->T + g.hop - 1) / g.hop;
-            const int C = sw::kTile / g.L;
-            SW_CHECK_ARG(p->PA % C == 0, "2*nantennas must be a multiple of the column tile");
-            SW_CHECK_ARG(g.RP <= 16, "FFT length too large");
+            SW_CHECK_ARG(p->PA % sw::lds_shape(g.L).C == 0,
+                         "2*nantennas must be a multiple of the transform's column tile");
             gid[key] = (int)p->groups.size();
             p->groups.push_back(g);
         }
         p->groups[gid[key]].dms.push_back(d);
     }
-    // device tables
-    size_t zmax = 0;
+    size_t zmax = 0, slots = 1, vslots = 0;
     for(auto& g : p->groups) {
         if(!sw::twiddle_table(p.get(), g.L, st)) return SW_ERR_ALLOC;
+        slots = std::max(slots, std::min<size_t>(sw::kMaxDm, g.dms.size()));
         if(g.RP > 1) {
             if(!sw::twiddle_table(p.get(), g.N, st)) return SW_ERR_ALLOC;
             zmax = std::max(zmax, (size_t)p->F * g.nseg * g.N * p->PA * sizeof(float2));
+            vslots = std::max(vslots, std::min<size_t>(sw::kMaxDm, g.dms.size()));
         }
     }
     p->chirp.resize(p->dms.size(), nullptr);
@@ -426,20 +416,19 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
                            p->chirp[d], p->F, N, RP, p->low_freq, p->chan_bw, pref);
         SW_LAUNCH_CHECK();
     }
+    const size_t vol = (size_t)p->F * p->PA * p->T * 2;
     if(p->Omax > 0) {
-        const size_t hb = (size_t)p->F * p->Omax * p->PA * 2;
+        const size_t hb = (size_t)p->F * p->PA * p->Omax * 2;
         SW_HIP_TRY(hipMalloc(&p->hist, hb));
         SW_HIP_TRY(hipMemsetAsync(p->hist, 0, hb, st));
     }
+    if(!p->input_fpat) SW_HIP_TRY(hipMalloc(&p->xtm, vol));
+    p->ytm.resize(slots, nullptr);
+    for(auto& y : p->ytm) SW_HIP_TRY(hipMalloc(&y, vol));
     if(zmax) {
-        // DIF-stage output plus one inverse-stage buffer per DM slot of a launch
-        size_t slots = 0;
-        for(auto& g : p->groups)
-            if(g.RP > 1) slots = std::max(slots, std::min<size_t>(sw::kMaxDm, g.dms.size()));
-        p->zbytes = zmax;
         SW_HIP_TRY(hipMalloc(&p->zbuf, zmax));
-        p->vbuf.resize(sw::kMaxDm, nullptr);
-        for(size_t k = 0; k < slots; ++k) SW_HIP_TRY(hipMalloc(&p->vbuf[k], zmax));
+        p->vbuf.resize(vslots, nullptr);
+        for(auto& v : p->vbuf) SW_HIP_TRY(hipMalloc(&v, zmax));
     }
     SW_HIP_TRY(hipStreamSynchronize(st));
     *out = p.release();
@@ -467,34 +456,51 @@ extern "C" int sw_codedisp_set_history(sw_codedisp* plan, const void* d_hist, vo
 {
     SW_CHECK_ARG(plan && d_hist, "null argument");
     if(plan->Omax == 0) return SW_OK;
-    SW_HIP_TRY(hipMemcpyAsync(plan->hist, d_hist, (size_t)plan->F * plan->Omax * plan->PA * 2,
-                              hipMemcpyDeviceToDevice, sw::as_stream(stream)));
-    return SW_OK;
+    hipStream_t st = sw::as_stream(stream);
+    if(plan->input_fpat) {
+        SW_HIP_TRY(hipMemcpyAsync(plan->hist, d_hist, (size_t)plan->F * plan->PA * plan->Omax * 2,
+                                  hipMemcpyDeviceToDevice, st));
+        return SW_OK;
+    }
+    // FTPA [F][Omax][PA] -> [F][PA][Omax]
+    return sw::corner_turn(d_hist, plan->hist, plan->F, plan->Omax, plan->PA, st);
 }
 
-extern "C" int sw_codedisp_push_history(sw_codedisp* plan, const void* d_ftpa, void* stream)
+static const int8_t* time_major_input(sw_codedisp* plan, const void* d_in, hipStream_t st, int& rc)
 {
-    SW_CHECK_ARG(plan && d_ftpa, "null argument");
+    rc = SW_OK;
+    if(plan->input_fpat) return static_cast<const int8_t*>(d_in);
+    rc = sw::corner_turn(d_in, plan->xtm, plan->F, plan->T, plan->PA, st);
+    return plan->xtm;
+}
+
+extern "C" int sw_codedisp_push_history(sw_codedisp* plan, const void* d_in, void* stream)
+{
+    SW_CHECK_ARG(plan && d_in, "null argument");
     if(plan->Omax == 0) return SW_OK;
     SW_CHECK_ARG(plan->T >= plan->Omax, "gulp shorter than the overlap");
-    const size_t n = (size_t)plan->F * plan->Omax * plan->PA;
-    hipLaunchKernelGGL(sw::history_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                       sw::as_stream(stream), plan->hist, static_cast<const int8_t*>(d_ftpa), plan->F,
-                       plan->T, plan->Omax, plan->PA);
+    hipStream_t st = sw::as_stream(stream);
+    // an FTPA input was turned into xtm by the run of this gulp
+    const int8_t* cur = plan->input_fpat ? static_cast<const int8_t*>(d_in) : plan->xtm;
+    const size_t n = (size_t)plan->F * plan->PA * plan->Omax;
+    hipLaunchKernelGGL(sw::history_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       plan->hist, cur, plan->F * plan->PA, plan->T, plan->Omax);
     SW_LAUNCH_CHECK();
     return SW_OK;
 }
 
-extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_ftpa, void* const* d_out,
+extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const* d_out,
                                void* const* d_out_f32, int dm_begin, int dm_count, int batched,
                                void* stream)
 {
-    SW_CHECK_ARG(plan && d_ftpa && d_out, "null argument");
+    SW_CHECK_ARG(plan && d_in && d_out, "null argument");
     const int nd = (int)plan->dms.size();
     SW_CHECK_ARG(dm_begin >= 0 && dm_count >= 0 && dm_begin + dm_count <= nd, "bad DM range");
     hipStream_t st = sw::as_stream(stream);
+    int rc;
+    const int8_t* cur = time_major_input(plan, d_in, st, rc);
+    if(rc) return rc;
     for(auto& g : plan->groups) {
-        // DMs of this group inside [dm_begin, dm_begin + dm_count)
         std::vector<int> todo;
         for(int d : g.dms)
             if(d >= dm_begin && d < dm_begin + dm_count) todo.push_back(d);
@@ -502,8 +508,9 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_ftpa, void* cons
         for(size_t b0 = 0; b0 < todo.size(); b0 += per_launch) {
             const int nb = (int)std::min<size_t>(per_launch, todo.size() - b0);
             sw::CdParams p{};
-            p.cur = static_cast<const int8_t*>(d_ftpa);
+            p.cur  = cur;
             p.hist = plan->hist;
+            p.z    = plan->zbuf;
             p.T = plan->T;
             p.O = g.O;
             p.Oh = plan->Omax;
@@ -514,46 +521,47 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_ftpa, void* cons
             p.RP = g.RP;
             p.N = g.N;
             p.ndm = nb;
-            p.tw = plan->twiddles.at(g.L);
+            p.tw  = plan->twiddles.at(g.L);
             p.twN = g.RP > 1 ? plan->twiddles.at(g.N) : nullptr;
             for(int k = 0; k < nb; ++k) {
                 const int d = todo[b0 + k];
                 SW_CHECK_ARG(d_out[d], "null output buffer");
                 p.chirp[k] = plan->chirp[d];
-                p.out[k] = static_cast<int8_t*>(d_out[d]);
-                p.outf[k] = d_out_f32 ? static_cast<float2*>(d_out_f32[d]) : nullptr;
+                p.out[k]   = plan->ytm[k];
+                p.outf[k]  = d_out_f32 ? static_cast<float2*>(d_out_f32[d]) : nullptr;
+                if(g.RP > 1) p.v[k] = plan->vbuf[k];
             }
-            const int C = sw::kTile / g.L;
-            const int G = plan->PA / C;
+            const int G = plan->PA / sw::lds_shape(g.L).C;
             if(g.RP == 1) {
-                const int blocks = plan->F * g.nseg * G;
-                int rc = sw::launch_lds<false, false>(g.L, p, blocks, st);
+                rc = sw::launch_lds<false>(g.L, p, plan->F * g.nseg * G, st);
                 if(rc) return rc;
-                continue;
-            }
-            // N > 4096: DIF stage, LDS stage per sub-sequence, DIT stage
-            for(int k = 0; k < nb; ++k) p.vout[k] = plan->vbuf[k];
-            p.zin = plan->zbuf;
-            const size_t items = (size_t)plan->F * g.nseg * g.L * plan->PA;
-            const unsigned eb = (unsigned)((items + 255) / 256);
-            switch(g.RP) {
-            case 2: hipLaunchKernelGGL(sw::codedisp_dif_kernel<2>, dim3(eb), dim3(256), 0, st, p, g.L); break;
-            case 4: hipLaunchKernelGGL(sw::codedisp_dif_kernel<4>, dim3(eb), dim3(256), 0, st, p, g.L); break;
-            case 8: hipLaunchKernelGGL(sw::codedisp_dif_kernel<8>, dim3(eb), dim3(256), 0, st, p, g.L); break;
-            default: hipLaunchKernelGGL(sw::codedisp_dif_kernel<16>, dim3(eb), dim3(256), 0, st, p, g.L); break;
-            }
-            SW_LAUNCH_CHECK();
-            const int blocks = plan->F * g.nseg * g.RP * G;
-            int rc = sw::launch_lds<true, true>(g.L, p, blocks, st);
-            if(rc) return rc;
-            for(int k = 0; k < nb; ++k) {
+            } else {
+                const size_t items = (size_t)plan->F * g.nseg * plan->PA * g.L;
+                const unsigned eb = (unsigned)((items + 255) / 256);
                 switch(g.RP) {
-                case 2: hipLaunchKernelGGL(sw::codedisp_dit_kernel<2>, dim3(eb), dim3(256), 0, st, p, g.L, k); break;
-                case 4: hipLaunchKernelGGL(sw::codedisp_dit_kernel<4>, dim3(eb), dim3(256), 0, st, p, g.L, k); break;
-                case 8: hipLaunchKernelGGL(sw::codedisp_dit_kernel<8>, dim3(eb), dim3(256), 0, st, p, g.L, k); break;
-                default: hipLaunchKernelGGL(sw::codedisp_dit_kernel<16>, dim3(eb), dim3(256), 0, st, p, g.L, k); break;
+                case 2: hipLaunchKernelGGL(sw::codedisp_dif_kernel<2>, dim3(eb), dim3(256), 0, st, p, g.L); break;
+                case 4: hipLaunchKernelGGL(sw::codedisp_dif_kernel<4>, dim3(eb), dim3(256), 0, st, p, g.L); break;
+                case 8: hipLaunchKernelGGL(sw::codedisp_dif_kernel<8>, dim3(eb), dim3(256), 0, st, p, g.L); break;
+                case 16: hipLaunchKernelGGL(sw::codedisp_dif_kernel<16>, dim3(eb), dim3(256), 0, st, p, g.L); break;
+                default: sw::set_error("codedisp: FFT length too large"); return SW_ERR_UNSUPPORTED;
                 }
                 SW_LAUNCH_CHECK();
+                rc = sw::launch_lds<true>(g.L, p, plan->F * g.nseg * g.RP * G, st);
+                if(rc) return rc;
+                for(int k = 0; k < nb; ++k) {
+                    switch(g.RP) {
+                    case 2: hipLaunchKernelGGL(sw::codedisp_dit_kernel<2>, dim3(eb), dim3(256), 0, st, p, g.L, k); break;
+                    case 4: hipLaunchKernelGGL(sw::codedisp_dit_kernel<4>, dim3(eb), dim3(256), 0, st, p, g.L, k); break;
+                    case 8: hipLaunchKernelGGL(sw::codedisp_dit_kernel<8>, dim3(eb), dim3(256), 0, st, p, g.L, k); break;
+                    default: hipLaunchKernelGGL(sw::codedisp_dit_kernel<16>, dim3(eb), dim3(256), 0, st, p, g.L, k); break;
+                    }
+                    SW_LAUNCH_CHECK();
+                }
+            }
+            // time-major result -> FTPA for the beamformers
+            for(int k = 0; k < nb; ++k) {
+                rc = sw::corner_turn(plan->ytm[k], d_out[todo[b0 + k]], plan->F, plan->PA, plan->T, st);
+                if(rc) return rc;
             }
         }
     }

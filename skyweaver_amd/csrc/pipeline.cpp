@@ -1,0 +1,404 @@
+// pipeline.cpp -- the per-gulp BeamformerPipeline behind sw_pipeline_*.
+//
+// Reference: BeamformerPipeline::process / operator(),
+// sw/detail/BeamformerPipeline.cu:127-301.  Stage order per gulp:
+//   (H2D) -> transpose -> statistics -> scalings (first gulp only) ->
+//   coherent dedispersion of all DMs -> per DM: incoherent beam, coherent
+//   beam, CB handler, IB handler -> stats handler.
+// Differences by design: every buffer is allocated once at creation, the
+// stages are enqueued back to back on one stream with no intermediate host
+// synchronisation (only when a sink must read a block, and at the end of
+// the gulp), and the scalings are indexed per beamset (beamset*F + f).
+#include "sw_common.hpp"
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace sw {
+int cb_mfma_supported(const sw_bf_dims& d);
+size_t fpat_stats_workspace(int nchans, int npol, int nant);
+}
+
+struct sw_pipeline {
+    sw_pipeline_config cfg{};
+    std::vector<float> dms;
+    std::vector<int> fft_len, overlap;
+    hipStream_t stream = nullptr;
+    sw_codedisp* cd    = nullptr;
+    // device buffers
+    void* taftp_d      = nullptr;
+    int8_t* ftpa       = nullptr;
+    float* stats       = nullptr;
+    void* stats_ws     = nullptr;
+    std::vector<int8_t*> ftpa_dd;
+    float* ib_raw      = nullptr;
+    int8_t* ib_q       = nullptr;
+    int8_t* cb_out     = nullptr;
+    int8_t* weights    = nullptr;
+    uint8_t* packed    = nullptr;
+    float* scl         = nullptr;  // cb_offsets | cb_scaling | ib_offsets | ib_scaling, [4][S][F]
+    int* bmap          = nullptr;
+    float* antw        = nullptr;
+    int nbeamsets      = 1;
+    std::vector<int> h_bmap;
+    std::vector<float> h_antw;
+    std::vector<float> h_stats;
+    bool weights_set   = false;
+    bool use_mfma      = false;
+    // sinks
+    sw_sink_fn cb_sink = nullptr, ib_sink = nullptr, st_sink = nullptr;
+    void *cb_user = nullptr, *ib_user = nullptr, *st_user = nullptr;
+    // state
+    sw_obs_header header{};
+    bool inited        = false;
+    size_t call_count  = 0;
+    double tick_per_block = 0;
+    hipEvent_t ev[6]{};
+    std::vector<hipEvent_t> ev_dm;  // per DM: ib start, cb start, cb end
+    float stage_ms[6]{};
+
+    size_t ftpa_bytes() const { return (size_t)cfg.nchans * cfg.nsamples * 2 * cfg.nantennas * 2; }
+    int nv() const { return cfg.stokes == SW_STOKES_IQUV ? 4 : 1; }
+    size_t tout() const { return (size_t)cfg.nsamples / cfg.tscrunch; }
+
+    ~sw_pipeline()
+    {
+        if(stream) (void)hipStreamSynchronize(stream);
+        if(cd) sw_codedisp_destroy(cd);
+        void* bufs[] = {taftp_d, ftpa, stats, stats_ws, ib_raw, ib_q, cb_out, weights, packed, scl, bmap, antw};
+        for(void* b : bufs) (void)hipFree(b);
+        for(auto* b : ftpa_dd) (void)hipFree(b);
+        for(auto e : ev) if(e) (void)hipEventDestroy(e);
+        for(auto e : ev_dm) if(e) (void)hipEventDestroy(e);
+        if(stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+sw_bf_dims bf_dims(const sw_pipeline* p)
+{
+    sw_bf_dims d{};
+    d.nchans      = p->cfg.nchans;
+    d.nsamples    = p->cfg.nsamples;
+    d.nantennas   = p->cfg.nantennas;
+    d.nbeams      = p->cfg.nbeams;
+    d.tscrunch    = p->cfg.tscrunch;
+    d.stokes      = p->cfg.stokes;
+    d.nbeamsets   = p->nbeamsets;
+    d.ib_subtract = p->cfg.ib_subtract;
+    return d;
+}
+
+int upload_beamsets(sw_pipeline* p)
+{
+    const int A = p->cfg.nantennas, B = p->cfg.nbeams;
+    if(p->bmap) (void)hipFree(p->bmap);
+    if(p->antw) (void)hipFree(p->antw);
+    if(p->scl) (void)hipFree(p->scl);
+    p->bmap = nullptr;
+    p->antw = nullptr;
+    p->scl  = nullptr;
+    SW_HIP_TRY(hipMalloc(&p->bmap, sizeof(int) * B));
+    SW_HIP_TRY(hipMalloc(&p->antw, sizeof(float) * p->nbeamsets * A));
+    SW_HIP_TRY(hipMalloc(&p->scl, sizeof(float) * 4 * p->nbeamsets * p->cfg.nchans));
+    SW_HIP_TRY(hipMemcpyAsync(p->bmap, p->h_bmap.data(), sizeof(int) * B, hipMemcpyHostToDevice, p->stream));
+    SW_HIP_TRY(hipMemcpyAsync(p->antw, p->h_antw.data(), sizeof(float) * p->nbeamsets * A,
+                              hipMemcpyHostToDevice, p->stream));
+    SW_HIP_TRY(hipStreamSynchronize(p->stream));
+    // scalings depend on the beamset weights: recompute on the next gulp
+    p->call_count = 0;
+    return SW_OK;
+}
+
+int call_sink(sw_pipeline* p, sw_sink_fn fn, void* user, sw_block& blk)
+{
+    if(!fn) return SW_OK;
+    SW_HIP_TRY(hipStreamSynchronize(p->stream));
+    const int rc = fn(user, &blk);
+    if(rc != 0) {
+        sw::set_error("sw_pipeline_process: handler returned " + std::to_string(rc));
+        return SW_ERR_IO;
+    }
+    return SW_OK;
+}
+
+void fill_block(sw_block& b, int kind, const void* data, size_t bytes, const char* order,
+                std::initializer_list<size_t> dims)
+{
+    std::memset(&b, 0, sizeof(b));
+    b.kind      = kind;
+    b.data      = data;
+    b.on_device = 1;
+    b.bytes     = bytes;
+    b.ndims     = (int)dims.size();
+    int i = 0;
+    for(size_t v : dims) b.dims[i++] = v;
+    std::strncpy(b.order, order, sizeof(b.order) - 1);
+}
+
+} // namespace
+
+extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** out)
+{
+    SW_CHECK_ARG(cfg && out, "null argument");
+    *out = nullptr;
+    const sw_pipeline_config& c = *cfg;
+    SW_CHECK_ARG(c.ndms > 0 && c.coherent_dms, "No DDPlan set for pipeline");
+    SW_CHECK_ARG(c.nsamples > 0 && c.nsamples % 256 == 0,
+                 "Gulp size is not a multiple of the number of samples per heap");
+    SW_CHECK_ARG(c.nantennas > 0 && c.nantennas % 4 == 0, "nantennas must be a multiple of 4");
+    SW_CHECK_ARG(c.nchans > 0 && c.nbeams > 0, "empty dimensions");
+    SW_CHECK_ARG(c.tscrunch > 0 && c.nsamples % c.tscrunch == 0, "gulp must be a multiple of tscrunch");
+    SW_CHECK_ARG(c.bandwidth > 0 && c.centre_frequency > 0, "bad band description");
+    auto p = std::make_unique<sw_pipeline>();
+    p->cfg = c;
+    p->dms.assign(c.coherent_dms, c.coherent_dms + c.ndms);
+    if(c.fft_len) p->fft_len.assign(c.fft_len, c.fft_len + c.ndms);
+    if(c.overlap) p->overlap.assign(c.overlap, c.overlap + c.ndms);
+    p->cfg.coherent_dms = p->dms.data();
+    p->cfg.fft_len      = c.fft_len ? p->fft_len.data() : nullptr;
+    p->cfg.overlap      = c.overlap ? p->overlap.data() : nullptr;
+    if(p->cfg.output_level <= 0) p->cfg.output_level = 24.0f;
+    if(p->cfg.total_nchans <= 0) p->cfg.total_nchans = 4096;
+    SW_HIP_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    for(auto& e : p->ev) SW_HIP_TRY(hipEventCreate(&e));
+    p->ev_dm.resize(3 * c.ndms);
+    for(auto& e : p->ev_dm) SW_HIP_TRY(hipEventCreate(&e));
+
+    // coherent dedispersion plan: tsamp = F / bw, band bottom = centre - bw/2
+    // (create_coherent_dedisperser_config, sw/src/CoherentDedisperser.cu:19-54)
+    sw_codedisp_config cd{};
+    cd.nchans    = c.nchans;
+    cd.nantennas = c.nantennas;
+    cd.nsamples  = c.nsamples;
+    cd.low_freq  = c.centre_frequency - c.bandwidth / 2.0;
+    cd.chan_bw   = c.bandwidth / c.nchans;
+    cd.tsamp     = c.nchans / c.bandwidth;
+    cd.ndms      = c.ndms;
+    cd.dms       = p->dms.data();
+    cd.fft_len   = p->cfg.fft_len;
+    cd.overlap   = p->cfg.overlap;
+    cd.input_fpat = 1;
+    int rc = sw_codedisp_create(&cd, p->stream, &p->cd);
+    if(rc) return rc;
+
+    const size_t fb = p->ftpa_bytes();
+    SW_HIP_TRY(hipMalloc(&p->taftp_d, fb));
+    SW_HIP_TRY(hipMalloc(&p->ftpa, fb));
+
+    p->ftpa_dd.resize(c.ndms, nullptr);
+    for(auto& b : p->ftpa_dd) SW_HIP_TRY(hipMalloc(&b, fb));
+    SW_HIP_TRY(hipMalloc(&p->stats, sizeof(float) * 4 * c.nchans * 2 * c.nantennas));
+    SW_HIP_TRY(hipMalloc(&p->stats_ws, sw::fpat_stats_workspace(c.nchans, 2, c.nantennas)));
+    p->h_stats.resize((size_t)4 * c.nchans * 2 * c.nantennas);
+    const int nv = p->nv();
+    SW_HIP_TRY(hipMalloc(&p->cb_out, p->tout() * c.nchans * c.nbeams * nv));
+    // incoherent beam outputs sized for the largest supported beamset count
+    SW_HIP_TRY(hipMalloc(&p->ib_raw, sizeof(float) * 16 * p->tout() * c.nchans * nv));
+    SW_HIP_TRY(hipMalloc(&p->ib_q, (size_t)16 * p->tout() * c.nchans * nv));
+    SW_HIP_TRY(hipMalloc(&p->weights, (size_t)c.nchans * c.nbeams * c.nantennas * 2));
+    SW_HIP_TRY(hipMalloc(&p->packed, (size_t)c.nchans * c.nbeams * c.nantennas * 4));
+    // one beamset of all antennas until told otherwise
+    p->nbeamsets = 1;
+    p->h_bmap.assign(c.nbeams, 0);
+    p->h_antw.assign(c.nantennas, 1.0f);
+    rc = upload_beamsets(p.get());
+    if(rc) return rc;
+    sw_bf_dims d = bf_dims(p.get());
+    p->use_mfma = c.cb_form == SW_CB_MFMA && sw::cb_mfma_supported(d);
+    SW_HIP_TRY(hipStreamSynchronize(p->stream));
+    *out = p.release();
+    return SW_OK;
+}
+
+extern "C" int sw_pipeline_destroy(sw_pipeline* p)
+{
+    delete p;
+    return SW_OK;
+}
+
+extern "C" void* sw_pipeline_stream(const sw_pipeline* p) { return p ? p->stream : nullptr; }
+
+extern "C" int sw_pipeline_set_sinks(sw_pipeline* p, sw_sink_fn cb, void* cb_user, sw_sink_fn ib,
+                                     void* ib_user, sw_sink_fn stats, void* stats_user)
+{
+    SW_CHECK_ARG(p, "null pipeline");
+    p->cb_sink = cb;
+    p->cb_user = cb_user;
+    p->ib_sink = ib;
+    p->ib_user = ib_user;
+    p->st_sink = stats;
+    p->st_user = stats_user;
+    return SW_OK;
+}
+
+extern "C" int sw_pipeline_set_weights(sw_pipeline* p, const void* h_weights)
+{
+    SW_CHECK_ARG(p && h_weights, "null argument");
+    const auto& c = p->cfg;
+    SW_HIP_TRY(hipMemcpyAsync(p->weights, h_weights, (size_t)c.nchans * c.nbeams * c.nantennas * 2,
+                              hipMemcpyHostToDevice, p->stream));
+    int rc = sw_cb_pack_weights(p->weights, p->packed, c.nchans, c.nbeams, c.nantennas, p->stream);
+    if(rc) return rc;
+    SW_HIP_TRY(hipStreamSynchronize(p->stream));
+    p->weights_set = true;
+    return SW_OK;
+}
+
+extern "C" int sw_pipeline_set_beamsets(sw_pipeline* p, int nbeamsets, const int* h_map,
+                                        const float* h_antenna_weights)
+{
+    SW_CHECK_ARG(p && h_map && h_antenna_weights, "null argument");
+    SW_CHECK_ARG(nbeamsets > 0 && nbeamsets <= 16, "between 1 and 16 beamsets are supported");
+    for(int b = 0; b < p->cfg.nbeams; ++b)
+        SW_CHECK_ARG(h_map[b] >= 0 && h_map[b] < nbeamsets, "beamset map entry out of range");
+    p->nbeamsets = nbeamsets;
+    p->h_bmap.assign(h_map, h_map + p->cfg.nbeams);
+    p->h_antw.assign(h_antenna_weights, h_antenna_weights + (size_t)nbeamsets * p->cfg.nantennas);
+    return upload_beamsets(p);
+}
+
+extern "C" int sw_pipeline_init(sw_pipeline* p, const sw_obs_header* header)
+{
+    SW_CHECK_ARG(p && header, "null argument");
+    SW_CHECK_ARG(header->nantennas > 0 && header->nantennas <= p->cfg.nantennas,
+                 "Input data contains too many antennas for the configured nantennas");
+    p->header = *header;
+    p->inited = true;
+    // timestamp step per block in sampler ticks (BeamformerPipeline.cu:67)
+    p->tick_per_block = 2.0 * p->cfg.total_nchans * p->cfg.nsamples;
+    return SW_OK;
+}
+
+extern "C" int sw_pipeline_stage_times(const sw_pipeline* p, float* ms6)
+{
+    SW_CHECK_ARG(p && ms6, "null argument");
+    for(int i = 0; i < 6; ++i) ms6[i] = p->stage_ms[i];
+    return SW_OK;
+}
+
+extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t bytes, int on_device)
+{
+    SW_CHECK_ARG(p && taftp, "null argument");
+    SW_CHECK_ARG(p->inited, "pipeline not initialised (call sw_pipeline_init)");
+    SW_CHECK_ARG(p->weights_set, "no beamforming weights set");
+    const auto& c      = p->cfg;
+    const int nant_in  = p->header.nantennas;
+    const size_t expect = (size_t)c.nsamples * nant_in * c.nchans * 2 * 2;
+    if(bytes != expect) {
+        sw::set_error("Unexpected buffer size, expected " + std::to_string(expect) + " but got " +
+                      std::to_string(bytes));
+        return SW_ERR_INVALID;
+    }
+    hipStream_t st = p->stream;
+    const void* in = taftp;
+    if(!on_device) {
+        SW_HIP_TRY(hipMemcpyAsync(p->taftp_d, taftp, bytes, hipMemcpyHostToDevice, st));
+        in = p->taftp_d;
+    }
+    const sw_bf_dims d = bf_dims(p);
+    const int F = c.nchans, A = c.nantennas, S = p->nbeamsets;
+    const double utc_offset_gulp = (double)p->call_count * p->tick_per_block / p->header.sample_clock;
+    int rc;
+    SW_HIP_TRY(hipEventRecord(p->ev[0], st));
+    // the gulp is held time-major (FPAT) up to the dedisperser
+    rc = sw_transpose_fpat(in, p->ftpa, c.nsamples / 256, nant_in, A, F, st);
+    if(rc) return rc;
+    SW_HIP_TRY(hipEventRecord(p->ev[1], st));
+    rc = sw_statistics_fpat(p->ftpa, p->stats, F, c.nsamples, 2, A, p->stats_ws, st);
+    if(rc) return rc;
+    SW_HIP_TRY(hipEventRecord(p->ev[2], st));
+    if(p->call_count == 0) {
+        // update_scalings on the first gulp (BeamformerPipeline.cu:171-176)
+        SW_HIP_TRY(hipMemcpyAsync(p->h_stats.data(), p->stats, sizeof(float) * p->h_stats.size(),
+                                  hipMemcpyDeviceToHost, st));
+        SW_HIP_TRY(hipStreamSynchronize(st));
+        std::vector<float> h(4 * (size_t)S * F);
+        rc = sw_update_scalings(p->h_stats.data(), p->h_antw.data(), S, F, 2, A, c.tscrunch, c.tscrunch,
+                                c.output_level, h.data(), h.data() + S * F, h.data() + 2 * S * F,
+                                h.data() + 3 * S * F);
+        if(rc) return rc;
+        SW_HIP_TRY(hipMemcpyAsync(p->scl, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, st));
+    }
+    float* cb_offsets = p->scl;
+    float* cb_scaling = p->scl + S * F;
+    float* ib_offsets = p->scl + 2 * S * F;
+    float* ib_scaling = p->scl + 3 * S * F;
+
+    std::vector<void*> outs(p->ftpa_dd.begin(), p->ftpa_dd.end());
+    rc = sw_codedisp_run(p->cd, p->ftpa, outs.data(), nullptr, 0, c.ndms, c.batched_dms, st);
+    if(rc) return rc;
+    rc = sw_codedisp_push_history(p->cd, p->ftpa, st);
+    if(rc) return rc;
+    SW_HIP_TRY(hipEventRecord(p->ev[3], st));
+
+    const int nv = p->nv();
+    const size_t T2 = p->tout();
+    for(int dm = 0; dm < c.ndms; ++dm) {
+        int fl = 0, ov = 0;
+        sw_codedisp_info(p->cd, dm, &fl, &ov);
+        const double tsamp = c.nchans / c.bandwidth;
+        SW_HIP_TRY(hipEventRecord(p->ev_dm[3 * dm], st));
+        rc = sw_ib_beamform(p->ftpa_dd[dm], p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, &d, st);
+        if(rc) return rc;
+        SW_HIP_TRY(hipEventRecord(p->ev_dm[3 * dm + 1], st));
+        if(p->use_mfma)
+            rc = sw_cb_beamform_packed(p->ftpa_dd[dm], p->packed, cb_scaling, cb_offsets, p->bmap,
+                                       p->ib_raw, p->cb_out, &d, st);
+        else
+            rc = sw_cb_beamform(p->ftpa_dd[dm], p->weights, cb_scaling, cb_offsets, p->bmap, p->ib_raw,
+                                p->cb_out, &d, SW_CB_PLAIN, nullptr, st);
+        if(rc) return rc;
+        SW_HIP_TRY(hipEventRecord(p->ev_dm[3 * dm + 2], st));
+        // the dedispersed block is delayed by the filter delay tsamp*O/2
+        const double utc = utc_offset_gulp + tsamp * ov / 2.0;
+        sw_block blk;
+        fill_block(blk, SW_BLOCK_CB_TFB, p->cb_out, T2 * F * c.nbeams * nv, "TFB", {T2, (size_t)F, (size_t)c.nbeams});
+        blk.dm_idx = dm;
+        blk.reference_dm = p->dms[dm];
+        blk.tsamp = tsamp * c.tscrunch;
+        blk.utc_offset = utc;
+        rc = call_sink(p, p->cb_sink, p->cb_user, blk);
+        if(rc) return rc;
+        fill_block(blk, SW_BLOCK_IB_BTF, p->ib_q, (size_t)S * T2 * F * nv, "BTF", {(size_t)S, T2, (size_t)F});
+        blk.dm_idx = dm;
+        blk.reference_dm = p->dms[dm];
+        blk.tsamp = tsamp * c.tscrunch;
+        blk.utc_offset = utc;
+        rc = call_sink(p, p->ib_sink, p->ib_user, blk);
+        if(rc) return rc;
+    }
+    SW_HIP_TRY(hipEventRecord(p->ev[4], st));
+    {
+        sw_block blk;
+        fill_block(blk, SW_BLOCK_STATS_FPA, p->stats, sizeof(float) * 4 * F * 2 * A, "FPA",
+                   {(size_t)F, 2, (size_t)A});
+        blk.tsamp = c.nchans / c.bandwidth * c.nsamples;
+        blk.utc_offset = utc_offset_gulp;
+        rc = call_sink(p, p->st_sink, p->st_user, blk);
+        if(rc) return rc;
+    }
+    SW_HIP_TRY(hipStreamSynchronize(st));
+    // per-stage device times of this gulp
+    float t = 0;
+    for(int i = 0; i < 6; ++i) p->stage_ms[i] = 0;
+    SW_HIP_TRY(hipEventElapsedTime(&t, p->ev[0], p->ev[1]));
+    p->stage_ms[0] = t;
+    SW_HIP_TRY(hipEventElapsedTime(&t, p->ev[1], p->ev[2]));
+    p->stage_ms[1] = t;
+    SW_HIP_TRY(hipEventElapsedTime(&t, p->ev[2], p->ev[3]));
+    p->stage_ms[2] = t;
+    for(int dm = 0; dm < c.ndms; ++dm) {
+        SW_HIP_TRY(hipEventElapsedTime(&t, p->ev_dm[3 * dm], p->ev_dm[3 * dm + 1]));
+        p->stage_ms[3] += t;
+        SW_HIP_TRY(hipEventElapsedTime(&t, p->ev_dm[3 * dm + 1], p->ev_dm[3 * dm + 2]));
+        p->stage_ms[4] += t;
+    }
+    SW_HIP_TRY(hipEventElapsedTime(&t, p->ev[0], p->ev[4]));
+    p->stage_ms[5] = t - (p->stage_ms[0] + p->stage_ms[1] + p->stage_ms[2] + p->stage_ms[3] + p->stage_ms[4]);
+    ++p->call_count;
+    return SW_OK;
+}

@@ -61,7 +61,7 @@ const char* sw_version(void);
  * Thin wrappers over the HIP runtime this library is linked against, so a
  * caller (a test harness, a host language without HIP bindings) needs no
  * second GPU runtime in its process.  No reference counterpart: the
- * reference allocates through thrust containers (sw/DescribedVector.hpp). */
+ * reference allocates through thrust containers (cpp/skyweaver/DescribedVector.hpp). */
 int sw_device_count(int* count);
 int sw_malloc(void** d_ptr, size_t bytes);
 int sw_free(void* d_ptr);
@@ -81,14 +81,14 @@ int sw_event_elapsed_ms(float* ms, void* start, void* stop);
 
 /* ---------------------------------------------------------------- Transposer
  * Transposer::transpose(TAFTPVoltagesD const&, FTPAVoltagesD&, size_t
- * input_nantennas, cudaStream_t), sw/Transposer.cuh:66-69,
- * sw/src/Transposer.cu:117-184.  Antennas >= nant_in are written as zeros. */
+ * input_nantennas, cudaStream_t), cpp/skyweaver/Transposer.cuh:66-69,
+ * cpp/skyweaver/src/Transposer.cu:117-184.  Antennas >= nant_in are written as zeros. */
 int sw_transpose(const void* d_taftp, void* d_ftpa, int nheap_groups,
                  int nant_in, int nant_out, int nchans, void* stream);
 
 /* ------------------------------------------------------ StatisticsCalculator
  * StatisticsCalculator::calculate_statistics(FTPAVoltagesD const&),
- * sw/StatisticsCalculator.cuh:65, sw/src/StatisticsCalculator.cu:54-130.
+ * cpp/skyweaver/StatisticsCalculator.cuh:65, cpp/skyweaver/src/StatisticsCalculator.cu:54-130.
  * d_stats: FPA of struct {float mean, std, skew, kurtosis}.
  * d_workspace: sw_statistics_workspace_size() bytes of device scratch, or
  * NULL to use a library-owned buffer. */
@@ -97,7 +97,7 @@ int sw_statistics(const void* d_ftpa, float* d_stats, int nchans, int nsamples,
                   int npol, int nantennas, void* d_workspace, void* stream);
 
 /* StatisticsCalculator::update_scalings(beamset_weights, nbeamsets),
- * sw/src/StatisticsCalculator.cu:132-239 (host arithmetic).  All arrays are
+ * cpp/skyweaver/src/StatisticsCalculator.cu:132-239 (host arithmetic).  All arrays are
  * host memory; outputs are [nbeamsets][nchans], indexed beamset*F + f. */
 int sw_update_scalings(const float* h_stats, const float* h_beamset_weights,
                        int nbeamsets, int nchans, int npol, int nantennas,
@@ -120,7 +120,7 @@ typedef struct sw_bf_dims {
 /* ------------------------------------------------------ IncoherentBeamformer
  * IncoherentBeamformer::beamform(input, output_raw, output, output_scale,
  * output_offset, antenna_weights, nbeamsets, stream),
- * sw/IncoherentBeamformer.cuh:84-91, sw/src/IncoherentBeamformer.cu:16-183.
+ * cpp/skyweaver/IncoherentBeamformer.cuh:84-91, cpp/skyweaver/src/IncoherentBeamformer.cu:16-183.
  * Per-beamset antenna weights [S][A] are honoured for every beamset.
  * d_raw: BTF float (float4 for IQUV); d_out: BTF int8 (char4 for IQUV);
  * d_scale / d_offset: [S][F]; d_ant_weights: [S][A]. */
@@ -132,7 +132,7 @@ int sw_ib_beamform(const void* d_ftpa, float* d_raw, void* d_out,
 /* -------------------------------------------------------- CoherentBeamformer
  * CoherentBeamformer::beamform(input, weights, output_scale, output_offset,
  * beamset_mapping, ib_powers, output, nbeamsets, stream),
- * sw/CoherentBeamformer.cuh:84-92, sw/src/CoherentBeamformer.cu:13-269.
+ * cpp/skyweaver/CoherentBeamformer.cuh:84-92, cpp/skyweaver/src/CoherentBeamformer.cu:13-269.
  * d_weights: [F][B][A] complex int8; d_scale / d_offset: [S][F];
  * d_beamset_map: [B] int; d_ib_raw: BTF from sw_ib_beamform of the same
  * input; d_out: TFB int8 (char4 for IQUV).
@@ -146,7 +146,7 @@ int sw_cb_beamform(const void* d_ftpa, const void* d_weights,
                    void* d_workspace, void* stream);
 
 /* The MFMA form in two steps: repack [F][B][A] weights once per weight
- * update (WeightsManager::weights, sw/src/WeightsManager.cu:110-139 is where
+ * update (WeightsManager::weights, cpp/skyweaver/src/WeightsManager.cu:110-139 is where
  * the reference produces them), then beamform from the packed buffer. */
 int sw_cb_pack_weights(const void* d_weights, void* d_packed, int nchans,
                        int nbeams, int nantennas, void* stream);
@@ -155,9 +155,42 @@ int sw_cb_beamform_packed(const void* d_ftpa, const void* d_packed,
                           const int* d_beamset_map, const float* d_ib_raw,
                           void* d_out, const sw_bf_dims* dims, void* stream);
 
+/* ----------------------------------------------------------- WeightsManager
+ * WeightsManager::weights(delays, current_epoch, delay_epoch) and
+ * generate_weights_k, cpp/skyweaver/src/WeightsManager.cu:16-87,109-139.
+ * d_delays: [B][A] float3 {weight, offset (s), rate (s/s)};
+ * d_chan_freqs: [F] Hz (FP64); d_weights out: [F][B][A] complex int8,
+ * w = clamp(rint(127 wt (cos 2 pi phi, -sin 2 pi phi))),
+ * phi = ((current - delay_epoch) rate + offset) nu_f. */
+int sw_generate_weights(const float* d_delays, void* d_weights,
+                        const double* d_chan_freqs, int nantennas, int nbeams,
+                        int nchans, double current_epoch, double delay_epoch,
+                        void* stream);
+
+/* ------------------------------------------------------------ DelayManager
+ * cpp/skyweaver/src/DelayManager.cu:16-250, file format
+ * cpp/skyweaver/DelayManager.cuh:40-51: records of
+ * {uint32 nbeams, uint32 nantennas, double start_epoch, double end_epoch}
+ * followed by float3 [nbeams][nantennas].  open() reads the first model and
+ * parses its beamsets (DelayManager::DelayManager / parse_beamsets);
+ * get() = DelayManager::delays(epoch): advances to the model valid at epoch
+ * (start <= epoch <= end) and writes it zero-padded to [nbeams][nantennas]
+ * float3 on the host.  Errors: SW_ERR_DELAY_EPOCH (epoch before the current
+ * model; InvalidDelayEpoch), SW_ERR_EOF (file exhausted), SW_ERR_INVALID
+ * (too many beams/antennas, model dimensions changing), SW_ERR_IO. */
+typedef struct sw_delay_model sw_delay_model;
+int sw_delay_model_open(const char* path, int nbeams, int nantennas, sw_delay_model** out);
+int sw_delay_model_close(sw_delay_model* model);
+/* DelayManager::nbeamsets / beamset_mapping / beamset_weights: h_map [nbeams]
+ * and h_weights [max_beamsets][nantennas] may be NULL. */
+int sw_delay_model_beamsets(const sw_delay_model* model, int* nbeamsets, int* h_map,
+                            float* h_weights, int max_beamsets);
+int sw_delay_model_get(sw_delay_model* model, double epoch, float* h_delays,
+                       double* model_epoch);
+
 /* ------------------------------------------------------ CoherentDedisperser
  * create_coherent_dedisperser_config + CoherentDedisperser::dedisperse,
- * sw/CoherentDedisperser.cuh:55-92, sw/src/CoherentDedisperser.cu:19-353.
+ * cpp/skyweaver/CoherentDedisperser.cuh:55-92, cpp/skyweaver/src/CoherentDedisperser.cu:19-353.
  * A plan holds, per coherent DM, the FFT length N, the overlap O (even),
  * the FP64-generated chirp tables and the overlap-save history (the previous
  * gulp's last O samples of every channel; zeros before the first gulp).
@@ -201,7 +234,7 @@ int sw_codedisp_set_history(sw_codedisp* plan, const void* d_hist, void* stream)
  * Inside the pipeline a gulp is held time-major, FPAT [F][P][A][T] complex
  * int8, so that the dedisperser's transforms read contiguous time runs.
  * No reference counterpart (the reference keeps FTPA and transposes per
- * channel inside cuFFT's strided plans, sw/src/CoherentDedisperser.cu:106-131). */
+ * channel inside cuFFT's strided plans, cpp/skyweaver/src/CoherentDedisperser.cu:106-131). */
 int sw_transpose_fpat(const void* d_taftp, void* d_fpat, int nheap_groups, int nant_in,
                       int nant_out, int nchans, void* stream);
 /* [F][rows][cols] -> [F][cols][rows] of 2-byte elements (FTPA <-> FPAT). */
@@ -213,8 +246,8 @@ int sw_statistics_fpat(const void* d_fpat, float* d_stats, int nchans, int nsamp
 /* ---------------------------------------------------- BeamformerPipeline
  * BeamformerPipeline<CBH, IBH, StatsH, Traits>: ctor(PipelineConfig const&,
  * handlers), init(ObservationHeader const&, utc_offset),
- * operator()(TAFTPVoltagesH<char2> const&), sw/BeamformerPipeline.cuh:48-67,
- * sw/detail/BeamformerPipeline.cu:44-301.  Per gulp: transpose, statistics,
+ * operator()(TAFTPVoltagesH<char2> const&), cpp/skyweaver/BeamformerPipeline.cuh:48-67,
+ * cpp/skyweaver/detail/BeamformerPipeline.cu:44-301.  Per gulp: transpose, statistics,
  * scalings (first gulp), coherent dedispersion of every DM, then per DM the
  * incoherent and coherent beams and the handler calls.  All buffers are
  * allocated at creation; processing runs on one stream owned by the
@@ -227,7 +260,7 @@ enum sw_block_kind {
 };
 
 /* Metadata of a block handed to a sink (the DescribedVector fields the
- * reference handlers read, sw/DescribedVector.hpp:97-573). */
+ * reference handlers read, cpp/skyweaver/DescribedVector.hpp:97-573). */
 typedef struct sw_block {
     int kind;              /* enum sw_block_kind */
     const void* data;      /* valid only during the sink call */
@@ -294,6 +327,12 @@ int sw_pipeline_process(sw_pipeline* pipeline, const void* taftp, size_t bytes, 
 int sw_pipeline_stage_times(const sw_pipeline* pipeline, float* ms6);
 /* The processing stream (hipStream_t) the pipeline enqueues work on. */
 void* sw_pipeline_stream(const sw_pipeline* pipeline);
+
+/* -------------------------------------------------------- microbenchmarks
+ * Roofline peaks of the device, measured: which = 0 dense int8 MFMA (TOP/s),
+ * 1 HBM copy of 1 GiB (GB/s, read + write), 2 FP32 FMA (TFLOP/s).  No
+ * reference counterpart. */
+int sw_microbench(int which, double* result, double* seconds, void* stream);
 
 /* -------------------------------------------------- synthetic input (bench)
  * Seeded Gaussian complex int8: xoshiro256** seeded through splitmix64,

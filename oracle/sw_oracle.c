@@ -20,7 +20,7 @@
 
 /* ------------------------------------------------------------------------ */
 /* Complex float helpers: cuCmulf / cuCaddf / cuConjf semantics (textbook),
- * sw/beamformer_utils.cuh:103-106 for the squared magnitude. */
+ * cpp/skyweaver/beamformer_utils.cuh:103-106 for the squared magnitude. */
 typedef struct { float x, y; } cf32;
 
 static inline cf32 cf_mul(cf32 a, cf32 b)
@@ -46,7 +46,7 @@ static inline cf32 cf_conj(cf32 a)
 }
 static inline float cf_mag2(cf32 a) { return a.x * a.x + a.y * a.y; }
 
-/* calculate_stokes<S>, sw/beamformer_utils.cuh:125-140 */
+/* calculate_stokes<S>, cpp/skyweaver/beamformer_utils.cuh:125-140 */
 static inline float stokes_param(int s, cf32 p0, cf32 p1)
 {
     switch(s) {
@@ -57,13 +57,13 @@ static inline float stokes_param(int s, cf32 p0, cf32 p1)
     }
 }
 
-/* SingleStokesBeamformerTraits::clamp, sw/beamformer_utils.cuh:186-195 */
+/* SingleStokesBeamformerTraits::clamp, cpp/skyweaver/beamformer_utils.cuh:186-195 */
 static inline int8_t clamp_i8(float v)
 {
     return (int8_t)fmaxf(-128.0f, fminf(127.0f, v));
 }
 
-/* rescale, sw/beamformer_utils.cuh:174-184 */
+/* rescale, cpp/skyweaver/beamformer_utils.cuh:174-184 */
 static inline float rescale_param(int s, float power, float offset, float scale)
 {
     if(s == ORC_STOKES_I)
@@ -71,7 +71,7 @@ static inline float rescale_param(int s, float power, float offset, float scale)
     return rintf(power / scale);
 }
 
-/* ib_subtract, sw/beamformer_utils.cuh:165-172 */
+/* ib_subtract, cpp/skyweaver/beamformer_utils.cuh:165-172 */
 static inline float ib_subtract_param(float power, float ib_power, float mult,
                                       float scale)
 {
@@ -90,7 +90,7 @@ static inline cf32 load_c8(const int8_t* p)
 void orc_transpose(const int8_t* taftp, int8_t* ftpa, int nheaps, int nant_in,
                    int nant_out, int nchans, int nsamps_per_heap, int npol)
 {
-    /* sw/test/src/TransposerTester.cu:37-71 */
+    /* cpp/skyweaver/test/src/TransposerTester.cu:37-71 */
     const size_t tp   = (size_t)nsamps_per_heap * npol;
     const size_t ftp  = (size_t)nchans * tp;
     const size_t aftp = (size_t)nant_in * ftp;
@@ -116,7 +116,7 @@ void orc_ib_beamform(const int8_t* ftpa, float* raw, int8_t* out, int nchans,
                      const float* scale, const float* offset,
                      const float* beamset_weights, int nbeamsets, int stokes)
 {
-    /* sw/test/src/IncoherentBeamformerTester.cu:55-102 (fscrunch = 1) */
+    /* cpp/skyweaver/test/src/IncoherentBeamformerTester.cu:55-102 (fscrunch = 1) */
     const int nvec         = (stokes == ORC_STOKES_IQUV) ? 4 : 1;
     const int nsamples_out = nsamples / tscrunch;
     const size_t a         = (size_t)nantennas;
@@ -157,7 +157,7 @@ void orc_cb_beamform(const int8_t* ftpa, const int8_t* weights, int8_t* out,
                      const float* antenna_weights, const int* beamset_mapping,
                      int nbeamsets, int stokes, int ib_subtract)
 {
-    /* sw/test/src/CoherentBeamformerTester.cu:59-151 (fscrunch = 1).
+    /* cpp/skyweaver/test/src/CoherentBeamformerTester.cu:59-151 (fscrunch = 1).
      * Per beam the float operations are performed in exactly the reference
      * oracle's order: complex sums over antennas in ascending order, Stokes
      * accumulated over the tscrunch window in ascending time.  The loops are
@@ -257,7 +257,7 @@ typedef struct {
     double M1, M2, M3, M4;
 } running_stats;
 
-/* RunningStats::push, sw/test/src/StatisticsCalculatorTester.cu:54-69 */
+/* RunningStats::push, cpp/skyweaver/test/src/StatisticsCalculatorTester.cu:54-69 */
 static inline void rs_push(running_stats* s, double x)
 {
     double delta, delta_n, delta_n2, term1;
@@ -278,7 +278,7 @@ static inline void rs_push(running_stats* s, double x)
 void orc_statistics(const int8_t* ftpa, double* out, int nchans, int nsamples,
                     int npol, int nantennas)
 {
-    /* sw/test/src/StatisticsCalculatorTester.cu:120-153 */
+    /* cpp/skyweaver/test/src/StatisticsCalculatorTester.cu:120-153 */
     const size_t pa = (size_t)npol * nantennas;
     running_stats* st = (running_stats*)calloc(pa, sizeof(running_stats));
     for(int f = 0; f < nchans; ++f) {
@@ -310,7 +310,7 @@ void orc_update_scalings(const float* stds, const float* beamset_weights,
                          float* cb_offsets, float* cb_scaling,
                          float* ib_offsets, float* ib_scaling)
 {
-    /* sw/src/StatisticsCalculator.cu:140-229.  std::pow(float, int) promotes
+    /* cpp/skyweaver/src/StatisticsCalculator.cu:140-229.  std::pow(float, int) promotes
      * to double; std::sqrt(float) stays in float. */
     const float weights_amp = 127.0f;
     const int pa = npol * nantennas;
@@ -343,13 +343,13 @@ void orc_update_scalings(const float* stds, const float* beamset_weights,
 }
 
 /* ------------------------------------------------------------------------ */
-#define ORC_DM_CONST_HZ_S 4.148806423e15 /* sw/dedispersion_utils.cuh:14-16 */
+#define ORC_DM_CONST_HZ_S 4.148806423e15 /* cpp/skyweaver/dedispersion_utils.cuh:14-16 */
 
 int orc_id_delays(const float* dms, int ndms, const double* chan_freqs,
                   int nchans, double tsamp, int* delays)
 {
     /* DMSampleDelay(dm, f_ref = lowest channel, tsamp)(f),
-     * sw/dedispersion_utils.cuh:19-40; dm_delay = K (1/f_ref^2 - 1/f^2) dm */
+     * cpp/skyweaver/dedispersion_utils.cuh:19-40; dm_delay = K (1/f_ref^2 - 1/f^2) dm */
     int maxd = 0;
     const double f_ref = chan_freqs[0];
     for(int d = 0; d < ndms; ++d)
@@ -370,7 +370,7 @@ int orc_incoherent_dedisperse(const int8_t* tfb, long nsamples, int nchans,
                               int max_delay, int tscrunch, int8_t* out,
                               int nthreads)
 {
-    /* sw/src/IncoherentDedisperser.cu:67-125.  Sums are exact integers in
+    /* cpp/skyweaver/src/IncoherentDedisperser.cu:67-125.  Sums are exact integers in
      * float, so accumulation order is immaterial; the division by the float
      * sqrt(F*ts) and the clamp-then-truncate are reproduced exactly. */
     if(nsamples <= max_delay)
@@ -456,7 +456,7 @@ void orc_dm_chirp(float dm, double low_freq, double coarse_chan_bw, int nchans,
                   int fft_len, double* out)
 {
     /* DMPrefactor: -1.0f * 2 pi * K * dm (double), dedispersion_utils.cuh:42-48;
-     * DMResponse, sw/src/CoherentDedisperser.cu:313-335. */
+     * DMResponse, cpp/skyweaver/src/CoherentDedisperser.cu:313-335. */
     const double two_pi = 2 * 3.1415926535897932385;
     const double pref   = -1.0f * two_pi * ORC_DM_CONST_HZ_S * (double)dm;
     const double fine_chan_bw = coarse_chan_bw / fft_len;
@@ -475,7 +475,7 @@ void orc_dm_chirp(float dm, double low_freq, double coarse_chan_bw, int nchans,
 
 static inline int8_t wrap8(long v)
 {
-    /* static_cast<char>(int): modulo 2^8, sw/src/CoherentDedisperser.cu:217-221 */
+    /* static_cast<char>(int): modulo 2^8, cpp/skyweaver/src/CoherentDedisperser.cu:217-221 */
     return (int8_t)(uint8_t)(v & 0xFF);
 }
 
@@ -540,9 +540,9 @@ void orc_weights(const float* delays, int8_t* out, const double* chan_freqs,
                  int nantennas, int nbeams, int nchans, double current_epoch,
                  double delay_epoch)
 {
-    /* sw/test/src/WeightsManagerTester.cu:45-70 loop; the product kernel's
+    /* cpp/skyweaver/test/src/WeightsManagerTester.cu:45-70 loop; the product kernel's
      * rounding (__double2int_rn = half even) and clamp to [-128, 127]
-     * (sw/src/WeightsManager.cu:70-79). */
+     * (cpp/skyweaver/src/WeightsManager.cu:70-79). */
     const double TWOPI = 6.283185307179586;
     for(int a = 0; a < nantennas; ++a)
         for(int b = 0; b < nbeams; ++b) {
@@ -567,7 +567,7 @@ int orc_parse_beamsets(const float* model, int nbeams_model, int nant_model,
                        int nbeams_cfg, int nant_cfg, int* beamset_map,
                        float* beamset_weights, int max_beamsets)
 {
-    /* sw/src/DelayManager.cu:162-232: beamsets are runs of contiguous beams
+    /* cpp/skyweaver/src/DelayManager.cu:162-232: beamsets are runs of contiguous beams
      * whose per-antenna scalar weights (x of the float3) are identical to the
      * previous beam's. */
     for(int b = 0; b < nbeams_cfg; ++b)

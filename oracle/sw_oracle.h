@@ -47,17 +47,17 @@
 extern "C" {
 #endif
 
-/* Stokes modes, sw/beamformer_utils.cuh:108 plus Full (IQUV). */
+/* Stokes modes, cpp/skyweaver/beamformer_utils.cuh:108 plus Full (IQUV). */
 enum { ORC_STOKES_I = 0, ORC_STOKES_Q = 1, ORC_STOKES_U = 2, ORC_STOKES_V = 3, ORC_STOKES_IQUV = 4 };
 
 /* TAFTP -> FTPA transpose with zero padding of antennas >= nant_in.
- * sw/test/src/TransposerTester.cu:30-72.  Arrays are interleaved complex
+ * cpp/skyweaver/test/src/TransposerTester.cu:30-72.  Arrays are interleaved complex
  * int8 (re, im).  The output must be pre-zeroed by the caller for the
  * padded antennas (reference: thrust resize value-initialises). */
 void orc_transpose(const int8_t* taftp, int8_t* ftpa, int nheaps, int nant_in,
                    int nant_out, int nchans, int nsamps_per_heap, int npol);
 
-/* Incoherent beamformer, sw/test/src/IncoherentBeamformerTester.cu:41-103.
+/* Incoherent beamformer, cpp/skyweaver/test/src/IncoherentBeamformerTester.cu:41-103.
  * ftpa [F][T][2][A] complex int8; raw [S][T/ts][F] float (x4 for IQUV);
  * out [S][T/ts][F] int8 (x4 for IQUV); scale/offset [S*F];
  * beamset_weights [S*A]. */
@@ -66,7 +66,7 @@ void orc_ib_beamform(const int8_t* ftpa, float* raw, int8_t* out, int nchans,
                      const float* scale, const float* offset,
                      const float* beamset_weights, int nbeamsets, int stokes);
 
-/* Coherent beamformer, sw/test/src/CoherentBeamformerTester.cu:43-152.
+/* Coherent beamformer, cpp/skyweaver/test/src/CoherentBeamformerTester.cu:43-152.
  * weights [F][B][A] complex int8; out TFB [T/ts][F][B] int8 (x4 for IQUV);
  * scales/offsets [S*F]; antenna_weights [S*A]; beamset_mapping [B].
  * The incoherent power used for the IB subtraction is recomputed here in
@@ -77,12 +77,12 @@ void orc_cb_beamform(const int8_t* ftpa, const int8_t* weights, int8_t* out,
                      const float* antenna_weights, const int* beamset_mapping,
                      int nbeamsets, int stokes, int ib_subtract);
 
-/* Per-(f,p,a) Welford statistics, sw/test/src/StatisticsCalculatorTester.cu:25-99,
+/* Per-(f,p,a) Welford statistics, cpp/skyweaver/test/src/StatisticsCalculatorTester.cu:25-99,
  * 120-171.  out [F][P][A][4] doubles = {mean, std, skew, kurtosis}. */
 void orc_statistics(const int8_t* ftpa, double* out, int nchans, int nsamples,
                     int npol, int nantennas);
 
-/* StatisticsCalculator::update_scalings, sw/src/StatisticsCalculator.cu:132-239,
+/* StatisticsCalculator::update_scalings, cpp/skyweaver/src/StatisticsCalculator.cu:132-239,
  * with the per-beamset index (beamset*F + f) the reference omits.
  * stds [F][P][A] float; beamset_weights [S][A]; outputs [S*F]. */
 void orc_update_scalings(const float* stds, const float* beamset_weights,
@@ -91,13 +91,13 @@ void orc_update_scalings(const float* stds, const float* beamset_weights,
                          float* cb_offsets, float* cb_scaling,
                          float* ib_offsets, float* ib_scaling);
 
-/* IncoherentDedisperser::prepare, sw/src/IncoherentDedisperser.cu:36-54 with
- * DMSampleDelay, sw/dedispersion_utils.cuh:20-40.  delays [D][F]; returns
+/* IncoherentDedisperser::prepare, cpp/skyweaver/src/IncoherentDedisperser.cu:36-54 with
+ * DMSampleDelay, cpp/skyweaver/dedispersion_utils.cuh:20-40.  delays [D][F]; returns
  * the maximum delay. */
 int orc_id_delays(const float* dms, int ndms, const double* chan_freqs,
                   int nchans, double tsamp, int* delays);
 
-/* IncoherentDedisperser::dedisperse, sw/src/IncoherentDedisperser.cu:67-125.
+/* IncoherentDedisperser::dedisperse, cpp/skyweaver/src/IncoherentDedisperser.cu:67-125.
  * tfb [T][F][B] int8 (x4 for IQUV, nvec=4); out [(T-maxd)/ts][D][B] (xnvec).
  * Returns 0, or -1 when T <= max_delay, -2 when (T-maxd) % ts != 0. */
 int orc_incoherent_dedisperse(const int8_t* tfb, long nsamples, int nchans,
@@ -109,8 +109,8 @@ int orc_incoherent_dedisperse(const int8_t* tfb, long nsamples, int nchans,
  * data [N][2]; N a power of two.  Iterative radix-2. */
 void orc_fft(double* data, int n, int inverse);
 
-/* Chirp H[c][k], sw/src/CoherentDedisperser.cu:313-335 and
- * DMPrefactor sw/dedispersion_utils.cuh:42-48.  out [nchans][N][2] FP64. */
+/* Chirp H[c][k], cpp/skyweaver/src/CoherentDedisperser.cu:313-335 and
+ * DMPrefactor cpp/skyweaver/dedispersion_utils.cuh:42-48.  out [nchans][N][2] FP64. */
 void orc_dm_chirp(float dm, double low_freq, double coarse_chan_bw, int nchans,
                   int fft_len, double* out);
 
@@ -118,22 +118,22 @@ void orc_dm_chirp(float dm, double low_freq, double coarse_chan_bw, int nchans,
  * tpa [nin][pa] complex int8 (pa = P*A columns); chirp [N][2].
  * Output sample j corresponds to input sample j + O/2: segment s covers
  * input [s(N-O), s(N-O)+N) (zero beyond nin); its samples [O/2, N-O/2) are
- * kept (sw/src/CoherentDedisperser.cu:196-221).  out_f [nout][pa][2]
+ * kept (cpp/skyweaver/src/CoherentDedisperser.cu:196-221).  out_f [nout][pa][2]
  * doubles (y/N) may be NULL; out_i8 [nout][pa][2] = wrap8(rint(y/N)) may be
  * NULL. */
 void orc_codedisp_channel(const int8_t* tpa, long nin, int pa,
                           const double* chirp, int N, int O, long nout,
                           double* out_f, int8_t* out_i8, int nthreads);
 
-/* Weight generation, sw/src/WeightsManager.cu:16-87 (the kernel semantics:
+/* Weight generation, cpp/skyweaver/src/WeightsManager.cu:16-87 (the kernel semantics:
  * round-half-even and clamp to int8) with the host oracle's loop
- * (sw/test/src/WeightsManagerTester.cu:33-71).
+ * (cpp/skyweaver/test/src/WeightsManagerTester.cu:33-71).
  * delays [B][A][3] floats {weight, offset, rate}; out [F][B][A] complex int8. */
 void orc_weights(const float* delays, int8_t* out, const double* chan_freqs,
                  int nantennas, int nbeams, int nchans, double current_epoch,
                  double delay_epoch);
 
-/* DelayManager::parse_beamsets, sw/src/DelayManager.cu:162-232.
+/* DelayManager::parse_beamsets, cpp/skyweaver/src/DelayManager.cu:162-232.
  * model [nbeams_model][nant_model][3]; beamset_map [nbeams_cfg];
  * beamset_weights [max_beamsets][nant_cfg].  Returns nbeamsets. */
 int orc_parse_beamsets(const float* model, int nbeams_model, int nant_model,

@@ -84,6 +84,7 @@ def _declare(L):
     L.sw_cb_beamform_packed.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(BfDims), _vp]
     L.sw_synth_gaussian_c8.argtypes = [_vp, _sz, ctypes.c_uint64, ctypes.c_double, _c]
     L.sw_synth_phase_weights.argtypes = [_vp, _sz, ctypes.c_uint64]
+    L.sw_microbench.argtypes = [_c, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _vp]
     L.sw_transpose_fpat.argtypes = [_vp, _vp, _c, _c, _c, _c, _vp]
     L.sw_corner_turn.argtypes = [_vp, _vp, _c, _c, _c, _vp]
     L.sw_statistics_fpat.argtypes = [_vp, _vp, _c, _c, _c, _c, _vp, _vp]
@@ -110,6 +111,12 @@ def _declare(L):
     L.sw_event_destroy.argtypes = [_vp]
     L.sw_event_record.argtypes = [_vp, _vp]
     L.sw_event_elapsed_ms.argtypes = [ctypes.POINTER(ctypes.c_float), _vp, _vp]
+    _d = ctypes.c_double
+    L.sw_generate_weights.argtypes = [_vp, _vp, _vp, _c, _c, _c, _d, _d, _vp]
+    L.sw_delay_model_open.argtypes = [ctypes.c_char_p, _c, _c, _pvp]
+    L.sw_delay_model_close.argtypes = [_vp]
+    L.sw_delay_model_beamsets.argtypes = [_vp, ctypes.POINTER(_c), _vp, _vp, _c]
+    L.sw_delay_model_get.argtypes = [_vp, _d, _vp, ctypes.POINTER(_d)]
 
 
 def last_error() -> str:
@@ -260,6 +267,14 @@ def transpose(taftp, ftpa, nheaps, nant_in, nant_out, nchans, stream=None):
     check(lib().sw_transpose(_ptr(taftp), _ptr(ftpa), nheaps, nant_in, nant_out, nchans, _stream(stream)))
 
 
+def microbench(which: str) -> float:
+    """Measured device peak: 'mfma_i8' (TOP/s), 'hbm_copy' (GB/s), 'fma_f32' (TFLOP/s)."""
+    code = {"mfma_i8": 0, "hbm_copy": 1, "fma_f32": 2}[which]
+    r, s = ctypes.c_double(), ctypes.c_double()
+    check(lib().sw_microbench(code, ctypes.byref(r), ctypes.byref(s), None))
+    return r.value
+
+
 def transpose_fpat(taftp, fpat, nheaps, nant_in, nant_out, nchans, stream=None):
     check(lib().sw_transpose_fpat(_ptr(taftp), _ptr(fpat), nheaps, nant_in, nant_out, nchans, _stream(stream)))
 
@@ -330,6 +345,52 @@ def synth_weights(shape, seed):
     assert out.shape[-1] == 2
     check(lib().sw_synth_phase_weights(out.ctypes.data, out.size // 2, seed))
     return out
+
+
+def generate_weights(delays, weights, chan_freqs, nantennas, nbeams, nchans, current_epoch, delay_epoch,
+                     stream=None):
+    """WeightsManager::weights: device [B][A] float3 delays -> [F][B][A] complex int8."""
+    check(lib().sw_generate_weights(_ptr(delays), _ptr(weights), _ptr(chan_freqs), nantennas, nbeams, nchans,
+                                    float(current_epoch), float(delay_epoch), _stream(stream)))
+
+
+class DelayModelFile:
+    """DelayManager over a delay-model file (sw_delay_model_* in the C ABI)."""
+
+    def __init__(self, path, nbeams, nantennas):
+        self.nbeams, self.nantennas = nbeams, nantennas
+        h = ctypes.c_void_p()
+        check(lib().sw_delay_model_open(str(path).encode(), nbeams, nantennas, ctypes.byref(h)))
+        self._h = h
+
+    def beamsets(self):
+        """(nbeamsets, beamset map [B] int32, beamset weights [S][A] float32)."""
+        import numpy as np
+        n = ctypes.c_int()
+        check(lib().sw_delay_model_beamsets(self._h, ctypes.byref(n), None, None, 0))
+        bmap = np.zeros(self.nbeams, np.int32)
+        bw = np.zeros((n.value, self.nantennas), np.float32)
+        check(lib().sw_delay_model_beamsets(self._h, ctypes.byref(n), bmap.ctypes.data, bw.ctypes.data, n.value))
+        return n.value, bmap, bw
+
+    def delays(self, epoch):
+        """DelayManager::delays(epoch) -> ([B][A][3] float32 on the host, model start epoch)."""
+        import numpy as np
+        out = np.zeros((self.nbeams, self.nantennas, 3), np.float32)
+        ep = ctypes.c_double()
+        check(lib().sw_delay_model_get(self._h, float(epoch), out.ctypes.data, ctypes.byref(ep)))
+        return out, ep.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sw_delay_model_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class CoherentDedisperser:

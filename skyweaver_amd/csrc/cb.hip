@@ -2,8 +2,8 @@
 // detection, time scrunch, incoherent-beam subtraction and int8 requantisation.
 //
 // Reference: CoherentBeamformer::beamform / bf_ftpa_general_k,
-// sw/src/CoherentBeamformer.cu:13-269 (dp4a form); oracle
-// sw/test/src/CoherentBeamformerTester.cu:43-152.  Per (f, b, t, p):
+// cpp/skyweaver/src/CoherentBeamformer.cu:13-269 (dp4a form); oracle
+// cpp/skyweaver/test/src/CoherentBeamformerTester.cu:43-152.  Per (f, b, t, p):
 //   V = sum_a W[f][b][a] * v[f][t][p][a]           (exact integers)
 //   power(t') = sum_{t in window, ascending} Stokes(V_p0, V_p1)   (float)
 //   out[t'][f][b] = clamp(rint((power - ib[s][t'][f]*127^2) / scale[s][f]))

@@ -1,7 +1,7 @@
 // codedisp.hip -- overlap-save coherent dedispersion: FFT, chirp multiply,
 // inverse FFT in FP32 with FP64-generated chirps and twiddles.
 //
-// Reference: CoherentDedisperser (sw/src/CoherentDedisperser.cu:19-353):
+// Reference: CoherentDedisperser (cpp/skyweaver/src/CoherentDedisperser.cu:19-353):
 // int8 -> c64, FFT, x H[dm][f][k], IFFT, /N, round half-even, wrap to int8,
 // keep [O/2, N-O/2).  The chirp is DMResponse (:313-335):
 //   nu0 = f_low + c*bw_c - bw_c/2, nu = k*bw_c/N for all k in [0, N),
@@ -81,9 +81,10 @@ __device__ __forceinline__ float2 input_sample(const CdParams& p, int f, int col
 template <int L, int C, int NBUF, int NT, bool F32>
 __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
 {
-    __shared__ float2 lds[NBUF * C * padded(L)];
+    __shared__ float2 lds[NBUF * C * padded(L) + TwiddleLds<L>::SIZE];
     float2* X = lds;
     float2* W = (NBUF == 2) ? lds + C * padded(L) : lds;
+    const Twiddles tw = load_twiddles<L, NT>(lds + NBUF * C * padded(L), p.tw);
     const int G  = p.PA / C;
     const int bid = blockIdx.x;
     const int u = bid / G, g = bid % G;  // unit = (channel, segment, sub-sequence)
@@ -104,7 +105,7 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
             X[c * padded(L) + pad(n)] = v;
         }
         __syncthreads();
-        fft_lds<false, L, C, NT>(X, p.tw);
+        fft_lds<false, L, C, NT>(X, tw);
     };
 
     load();
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
             W[c * padded(L) + pad(k)] = cmul(X[c * padded(L) + pad(k)], H[k]);
         }
         __syncthreads();
-        fft_lds<true, L, C, NT>(W, p.tw);
+        fft_lds<true, L, C, NT>(W, tw);
         if constexpr(F32) {
             float2* V = p.v[d];
             for(int i = tid; i < C * L; i += NT) {
@@ -315,7 +316,7 @@ int even_ceil(double x)
 }
 
 // Intra-channel dispersion smear at the bottom of the band, in samples:
-// dm_delay(f_low, f_low + bw_c, DM) / tsamp (sw/dedispersion_utils.cuh:20-23).
+// dm_delay(f_low, f_low + bw_c, DM) / tsamp (cpp/skyweaver/dedispersion_utils.cuh:20-23).
 double smear_samples(double f_low, double chan_bw, double tsamp, double dm)
 {
     const double f1 = f_low, f2 = f_low + chan_bw;
@@ -409,7 +410,7 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
             sw::set_error("codedisp: chirp allocation failed");
             return SW_ERR_ALLOC;
         }
-        // DMPrefactor: -1.0f * 2 pi * K * dm (sw/dedispersion_utils.cuh:42-48)
+        // DMPrefactor: -1.0f * 2 pi * K * dm (cpp/skyweaver/dedispersion_utils.cuh:42-48)
         const double pref = -1.0f * (2 * 3.1415926535897932385) * sw::kDmConst * (double)p->dms[d];
         const size_t n = (size_t)p->F * N;
         hipLaunchKernelGGL(sw::chirp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,

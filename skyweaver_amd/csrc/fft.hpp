@@ -134,11 +134,63 @@ __device__ __forceinline__ void dft(float2* v)
 __device__ __forceinline__ int pad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int padded(int n) { return n + (n >> 4); }
 
+// Twiddle source for the passes: exp(-2 pi i m / L) = A[m / LO] * B[m % LO]
+// with two small tables (LO = min(128, L) entries and L / LO entries) held
+// in LDS; both are FP64-generated values rounded once to FP32.
+struct Twiddles {
+    const float2* A;  // exp(-2 pi i (LO j) / L), j < L / LO
+    const float2* B;  // exp(-2 pi i j / L),      j < LO
+    int lo_bits;
+    __device__ __forceinline__ float2 operator()(int m) const
+    {
+        return cmul(A[m >> lo_bits], B[m & ((1 << lo_bits) - 1)]);
+    }
+};
+
+template <int L>
+struct TwiddleLds {
+    static constexpr int LO = L < 128 ? L : 128;
+    static constexpr int HI = L / LO;
+    static constexpr int SIZE = LO + HI;
+};
+
+// Fill the two LDS twiddle tables from the full table tw[m] = exp(-2 pi i m / L).
+template <int L, int NT>
+__device__ __forceinline__ Twiddles load_twiddles(float2* lds, const float2* __restrict__ tw)
+{
+    using TL = TwiddleLds<L>;
+    for(int i = threadIdx.x; i < TL::LO; i += NT) lds[i] = tw[i];
+    for(int i = threadIdx.x; i < TL::HI; i += NT) lds[TL::LO + i] = tw[i * TL::LO];
+    return Twiddles{lds + TL::LO, lds, __builtin_ctz(TL::LO)};
+}
+
+// w[k] = w1^k for k = 1..R-1 by a product tree of depth <= 4
+template <int R>
+__device__ __forceinline__ void twiddle_powers(float2 w1, float2* w)
+{
+    w[1] = w1;
+    if constexpr(R > 2) {
+        w[2] = cmul(w1, w1);
+        w[3] = cmul(w[2], w1);
+    }
+    if constexpr(R > 4) {
+        w[4] = cmul(w[2], w[2]);
+        w[5] = cmul(w[4], w1);
+        w[6] = cmul(w[4], w[2]);
+        w[7] = cmul(w[4], w[3]);
+    }
+    if constexpr(R > 8) {
+        w[8] = cmul(w[4], w[4]);
+#pragma unroll
+        for(int k = 1; k < 8; ++k) w[8 + k] = cmul(w[8], w[k]);
+    }
+}
+
 // One Stockham autosort pass of radix R over C columns of length L held in
 // LDS (column c at offset c*padded(L)).  Ns = product of the radices already
-// applied.  tw[m] = exp(-2 pi i m / L).  All threads of the block call it.
+// applied.  All threads of the block call it.
 template <int R, bool INV, int L, int C, int NT>
-__device__ __forceinline__ void stockham_pass(float2* buf, int Ns, const float2* __restrict__ tw)
+__device__ __forceinline__ void stockham_pass(float2* buf, int Ns, const Twiddles& tw)
 {
     constexpr int NB  = L / R;                 // butterflies per column
     constexpr int PER = (C * NB + NT - 1) / NT; // butterflies per thread
@@ -154,13 +206,12 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int Ns, const float2*
             for(int k = 0; k < R; ++k) v[q][k] = col[pad(j + k * NB)];
             if(Ns > 1) {
                 const int jm = j % Ns;
-                const int stride = jm * (L / (Ns * R));
+                float2 w1 = tw(jm * (L / (Ns * R)));
+                if(INV) w1 = cconj(w1);
+                float2 w[R];
+                twiddle_powers<R>(w1, w);
 #pragma unroll
-                for(int k = 1; k < R; ++k) {
-                    float2 w = tw[stride * k];
-                    if(INV) w = cconj(w);
-                    v[q][k] = cmul(v[q][k], w);
-                }
+                for(int k = 1; k < R; ++k) v[q][k] = cmul(v[q][k], w[k]);
             }
             dft<R, INV>(v[q]);
         }
@@ -181,10 +232,10 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int Ns, const float2*
     __syncthreads();
 }
 
-// Full FFT of length L (power of two, 16..4096) on C columns in LDS, natural
+// Full FFT of length L (power of two, 16..16384) on C columns in LDS, natural
 // order in and out, as radix-16 passes followed by one smaller pass.
 template <bool INV, int L, int C, int NT>
-__device__ __forceinline__ void fft_lds(float2* buf, const float2* __restrict__ tw)
+__device__ __forceinline__ void fft_lds(float2* buf, const Twiddles& tw)
 {
     int Ns = 1;
     constexpr int lg = __builtin_ctz(L);

@@ -2,8 +2,8 @@
 // antennas and time-scrunched, for every beamset in one launch.
 //
 // Reference: IncoherentBeamformer::beamform / icbf_ftpa_general_k,
-// sw/src/IncoherentBeamformer.cu:16-183; oracle
-// sw/test/src/IncoherentBeamformerTester.cu:41-103:
+// cpp/skyweaver/src/IncoherentBeamformer.cu:16-183; oracle
+// cpp/skyweaver/test/src/IncoherentBeamformerTester.cu:41-103:
 //   P[s][t'][f] = sum_{t in window} sum_a Stokes(v[f][t][0][a], v[f][t][1][a]) * w[s][a]
 //   raw = P, out = clamp(rescale(P, offset[s][f], scale[s][f]))
 // Unlike the reference kernel (which applies antenna_weights[a] of beamset 0
@@ -14,68 +14,85 @@
 // oracle's order (time ascending, antenna ascending, one rounding for the
 // product and one for the sum), so raw powers are bit-identical to the
 // oracle for any weights, not only when the sums happen to be exact.  The
-// arithmetic is tiny next to the HBM read of the FTPA block.
+// arithmetic is tiny next to the HBM read of the FTPA block; the rows are
+// staged through LDS so that the HBM reads are coalesced.
 #include "sw_common.hpp"
 
 namespace sw {
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kOut = 64;  // output samples per workgroup (one wave)
 
+// One wave per (64 consecutive output samples, channel).  For each step tt of
+// the scrunch window the 64 rows t'*ts + tt (one per output) are staged in
+// LDS with coalesced 16-byte loads; each lane then accumulates its own row,
+// antennas ascending, for every beamset.
 template <int S, int SMAX>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kOut)
 ib_kernel(const uint8_t* __restrict__ ftpa, float* __restrict__ raw, int8_t* __restrict__ out,
           const float* __restrict__ scale, const float* __restrict__ offset,
           const float* __restrict__ antw, int nchans, int nsamples, int nant, int tscrunch,
           int nbeamsets)
 {
     constexpr int NV = nstokes(S);
-    const int f   = blockIdx.y;
-    const int tp  = blockIdx.x * kThreads + threadIdx.x;   // output sample
+    extern __shared__ uint4 rows[];           // [kOut][rowq + 1]
+    const int f    = blockIdx.y;
+    const int lane = threadIdx.x;
     const int nsamps_out = nsamples / tscrunch;
-    if(tp >= nsamps_out) return;
+    const int tp0  = blockIdx.x * kOut;
+    const int tp   = tp0 + lane;
+    const int rowq = nant / 4;                // 16-byte chunks per (t) row: 4*nant bytes
+    const int ldsq = rowq + 1;
+    const int nrows = min(kOut, nsamps_out - tp0);
     float power[SMAX][NV];
 #pragma unroll
     for(int s = 0; s < SMAX; ++s)
 #pragma unroll
         for(int k = 0; k < NV; ++k) power[s][k] = 0.0f;
-
-    const size_t row_bytes = (size_t)4 * nant;       // P * A * 2
-    const uint8_t* base = ftpa + ((size_t)f * nsamples + (size_t)tp * tscrunch) * row_bytes;
+    const uint4* chan = reinterpret_cast<const uint4*>(ftpa + (size_t)f * nsamples * 4 * nant);
     for(int tt = 0; tt < tscrunch; ++tt) {
-        const uint8_t* row = base + (size_t)tt * row_bytes;
-        for(int a0 = 0; a0 < nant; a0 += 4) {
-            // 4 antennas of each polarisation: 8 bytes each
-            const uint2 q0 = *reinterpret_cast<const uint2*>(row + (size_t)a0 * 2);
-            const uint2 q1 = *reinterpret_cast<const uint2*>(row + (size_t)(nant + a0) * 2);
-            const uint32_t w0[2] = {q0.x, q0.y};
-            const uint32_t w1[2] = {q1.x, q1.y};
+        __syncthreads();
+        for(int i = lane; i < nrows * rowq; i += kOut) {
+            const int r = i / rowq, q = i % rowq;
+            rows[r * ldsq + q] = chan[((size_t)(tp0 + r) * tscrunch + tt) * rowq + q];
+        }
+        __syncthreads();
+        if(lane < nrows) {
+            const uint8_t* rb = reinterpret_cast<const uint8_t*>(rows + lane * ldsq);
+            for(int a4 = 0; a4 < nant; a4 += 4) {
+                // 4 antennas of pol 0 and of pol 1 (8 bytes each, 8-byte aligned)
+                const uint2 q0 = *reinterpret_cast<const uint2*>(rb + a4 * 2);
+                const uint2 q1 = *reinterpret_cast<const uint2*>(rb + (nant + a4) * 2);
+                const uint32_t w0[2] = {q0.x, q0.y};
+                const uint32_t w1[2] = {q1.x, q1.y};
 #pragma unroll
-            for(int j = 0; j < 4; ++j) {
-                const uint32_t d0 = w0[j >> 1] >> (16 * (j & 1));
-                const uint32_t d1 = w1[j >> 1] >> (16 * (j & 1));
-                const float r0 = (float)(int8_t)(d0), i0 = (float)(int8_t)(d0 >> 8);
-                const float r1 = (float)(int8_t)(d1), i1 = (float)(int8_t)(d1 >> 8);
-                float st[NV];
-                if constexpr(NV == 1) {
-                    st[0] = stokes<S>(r0, i0, r1, i1);
-                } else {
-                    st[0] = stokes<SW_STOKES_I>(r0, i0, r1, i1);
-                    st[1] = stokes<SW_STOKES_Q>(r0, i0, r1, i1);
-                    st[2] = stokes<SW_STOKES_U>(r0, i0, r1, i1);
-                    st[3] = stokes<SW_STOKES_V>(r0, i0, r1, i1);
-                }
+                for(int j = 0; j < 4; ++j) {
+                    const uint32_t d0 = w0[j >> 1] >> (16 * (j & 1));
+                    const uint32_t d1 = w1[j >> 1] >> (16 * (j & 1));
+                    const float r0 = (float)(int8_t)(d0), i0 = (float)(int8_t)(d0 >> 8);
+                    const float r1 = (float)(int8_t)(d1), i1 = (float)(int8_t)(d1 >> 8);
+                    float st[NV];
+                    if constexpr(NV == 1) {
+                        st[0] = stokes<S>(r0, i0, r1, i1);
+                    } else {
+                        st[0] = stokes<SW_STOKES_I>(r0, i0, r1, i1);
+                        st[1] = stokes<SW_STOKES_Q>(r0, i0, r1, i1);
+                        st[2] = stokes<SW_STOKES_U>(r0, i0, r1, i1);
+                        st[3] = stokes<SW_STOKES_V>(r0, i0, r1, i1);
+                    }
 #pragma unroll
-                for(int s = 0; s < SMAX; ++s) {
-                    if(s < nbeamsets) {
-                        const float w = antw[s * nant + a0 + j];
+                    for(int s = 0; s < SMAX; ++s) {
+                        if(s < nbeamsets) {
+                            const float w = antw[s * nant + a4 + j];
 #pragma unroll
-                        for(int k = 0; k < NV; ++k) power[s][k] += st[k] * w;
+                            for(int k = 0; k < NV; ++k) power[s][k] += st[k] * w;
+                        }
                     }
                 }
             }
         }
     }
+    if(tp >= nsamps_out) return;
 #pragma unroll
     for(int s = 0; s < SMAX; ++s) {
         if(s < nbeamsets) {
@@ -104,9 +121,10 @@ int launch_ib(const uint8_t* ftpa, float* raw, int8_t* out, const float* scale,
               const float* offset, const float* antw, const sw_bf_dims& d, hipStream_t st)
 {
     const int nout = d.nsamples / d.tscrunch;
-    dim3 grid((nout + kThreads - 1) / kThreads, d.nchans);
+    dim3 grid((nout + kOut - 1) / kOut, d.nchans);
+    const size_t lds = (size_t)kOut * (d.nantennas / 4 + 1) * 16;
 #define SW_IB_LAUNCH(SM)                                                                     \
-    hipLaunchKernelGGL((ib_kernel<S, SM>), grid, dim3(kThreads), 0, st, ftpa, raw, out,       \
+    hipLaunchKernelGGL((ib_kernel<S, SM>), grid, dim3(kOut), lds, st, ftpa, raw, out,         \
                        scale, offset, antw, d.nchans, d.nsamples, d.nantennas, d.tscrunch,   \
                        d.nbeamsets)
     if(d.nbeamsets <= 1)
@@ -140,7 +158,8 @@ extern "C" int sw_ib_beamform(const void* d_ftpa, float* d_raw, void* d_out,
     SW_CHECK_ARG(d.tscrunch > 0 && d.nsamples > 0 && d.nsamples % d.tscrunch == 0,
                  "nsamples must be a positive multiple of tscrunch");
     SW_CHECK_ARG(d.stokes >= SW_STOKES_I && d.stokes <= SW_STOKES_IQUV, "bad stokes mode");
-    SW_CHECK_ARG((reinterpret_cast<uintptr_t>(d_ftpa) & 7) == 0, "input must be 8-byte aligned");
+    SW_CHECK_ARG((reinterpret_cast<uintptr_t>(d_ftpa) & 15) == 0, "input must be 16-byte aligned");
+    SW_CHECK_ARG(d.nantennas <= 512, "at most 512 antennas");
     auto* in  = static_cast<const uint8_t*>(d_ftpa);
     auto* out = static_cast<int8_t*>(d_out);
     hipStream_t st = sw::as_stream(stream);

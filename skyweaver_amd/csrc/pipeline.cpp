@@ -1,7 +1,7 @@
 // pipeline.cpp -- the per-gulp BeamformerPipeline behind sw_pipeline_*.
 //
 // Reference: BeamformerPipeline::process / operator(),
-// sw/detail/BeamformerPipeline.cu:127-301.  Stage order per gulp:
+// cpp/skyweaver/detail/BeamformerPipeline.cu:127-301.  Stage order per gulp:
 //   (H2D) -> transpose -> statistics -> scalings (first gulp only) ->
 //   coherent dedispersion of all DMs -> per DM: incoherent beam, coherent
 //   beam, CB handler, IB handler -> stats handler.
@@ -169,7 +169,7 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
     for(auto& e : p->ev_dm) SW_HIP_TRY(hipEventCreate(&e));
 
     // coherent dedispersion plan: tsamp = F / bw, band bottom = centre - bw/2
-    // (create_coherent_dedisperser_config, sw/src/CoherentDedisperser.cu:19-54)
+    // (create_coherent_dedisperser_config, cpp/skyweaver/src/CoherentDedisperser.cu:19-54)
     sw_codedisp_config cd{};
     cd.nchans    = c.nchans;
     cd.nantennas = c.nantennas;

@@ -140,7 +140,7 @@ extern "C" int sw_update_scalings(const float* h_stats, const float* h_beamset_w
                                   float* h_cb_offsets, float* h_cb_scaling,
                                   float* h_ib_offsets, float* h_ib_scaling)
 {
-    // StatisticsCalculator::update_scalings, sw/src/StatisticsCalculator.cu:132-239,
+    // StatisticsCalculator::update_scalings, cpp/skyweaver/src/StatisticsCalculator.cu:132-239,
     // with outputs indexed beamset*F + f for every beamset.
     SW_CHECK_ARG(h_stats && h_beamset_weights && h_cb_offsets && h_cb_scaling && h_ib_offsets &&
                      h_ib_scaling,

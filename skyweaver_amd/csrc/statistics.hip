@@ -1,9 +1,9 @@
 // statistics.hip -- per-input (f, p, a) mean / std / skew / kurtosis.
 //
 // Reference: StatisticsCalculator::calculate_statistics and the
-// calculate_statistics kernel, sw/src/StatisticsCalculator.cu:20-130 (FP64
+// calculate_statistics kernel, cpp/skyweaver/src/StatisticsCalculator.cu:20-130 (FP64
 // Welford over the re and im parts of every sample), checked against the
-// host RunningStats oracle (sw/test/src/StatisticsCalculatorTester.cu:25-99)
+// host RunningStats oracle (cpp/skyweaver/test/src/StatisticsCalculatorTester.cu:25-99)
 // at 1e-5 relative.
 //
 // The inputs are int8, so the raw power sums S1..S4 of each column are

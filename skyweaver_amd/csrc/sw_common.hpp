@@ -57,13 +57,13 @@ struct cf32 {
     float x, y;
 };
 
-// |x|^2 = x.x*x.x + x.y*x.y (sw/beamformer_utils.cuh:103-106)
+// |x|^2 = x.x*x.x + x.y*x.y (cpp/skyweaver/beamformer_utils.cuh:103-106)
 __host__ __device__ __forceinline__ float mag2(float re, float im)
 {
     return re * re + im * im;
 }
 
-// Stokes parameters of two polarisations (sw/beamformer_utils.cuh:125-140).
+// Stokes parameters of two polarisations (cpp/skyweaver/beamformer_utils.cuh:125-140).
 // U and V are 2*Re / 2*Im of p0 * conj(p1) with the textbook complex product.
 template <int S>
 __host__ __device__ __forceinline__ float stokes(float r0, float i0, float r1,
@@ -82,13 +82,13 @@ __host__ __device__ __forceinline__ float stokes(float r0, float i0, float r1,
     }
 }
 
-// clamp<int8_t>(float), sw/beamformer_utils.cuh:186-195
+// clamp<int8_t>(float), cpp/skyweaver/beamformer_utils.cuh:186-195
 __host__ __device__ __forceinline__ int8_t clamp_i8(float v)
 {
     return static_cast<int8_t>(fmaxf(-128.0f, fminf(127.0f, v)));
 }
 
-// rescale, sw/beamformer_utils.cuh:174-184
+// rescale, cpp/skyweaver/beamformer_utils.cuh:174-184
 template <int S>
 __host__ __device__ __forceinline__ float rescale(float power, float offset,
                                                   float scale)
@@ -99,7 +99,7 @@ __host__ __device__ __forceinline__ float rescale(float power, float offset,
         return rintf(power / scale);
 }
 
-// ib_subtract, sw/beamformer_utils.cuh:165-172 (multiplier 127^2 = 16129)
+// ib_subtract, cpp/skyweaver/beamformer_utils.cuh:165-172 (multiplier 127^2 = 16129)
 __host__ __device__ __forceinline__ float ib_subtract(float power, float ib,
                                                       float scale)
 {

@@ -1,7 +1,7 @@
 // transpose.hip -- TAFTP -> FTPA split transpose with antenna zero padding.
 //
-// Reference: Transposer::transpose / transpose_k, sw/src/Transposer.cu:13-184,
-// oracle sw/test/src/TransposerTester.cu:30-72:
+// Reference: Transposer::transpose / transpose_k, cpp/skyweaver/src/Transposer.cu:13-184,
+// oracle cpp/skyweaver/test/src/TransposerTester.cu:30-72:
 //   out[f][h*256 + s][p][a] = in[h][a][f][s][p]  (a < nant_in), 0 otherwise.
 //
 // One workgroup per (heap h, channel f).  For each heap the input holds, per

@@ -38,6 +38,17 @@ TSAMP = 4096 / 856e6
 SEED_V, SEED_W = 0x5EED0002, 0x5EED1002
 INT8_PEAK_TOPS = 5000.0     # MI355X dense int8 MFMA
 HBM_PEAK_GBS = 8000.0
+VALU_ADD16_PEAK = 78.6e12   # 16-bit lane adds/s: 256 CUs x 64 lanes x 2 lanes per 32-bit add x 2.4 GHz
+# HBM bytes per cb_mfma_kernel launch at this workload, from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE calibrated on a 1 GiB copy kernel
+# (profiles/hbm_traffic_pipeline_64ant_64ch_65536samp_800beam_dm50.md)
+CB_TRAFFIC_BYTES = 1.330e9
+
+# incoherent dedispersion workload: 256 channels (4 x 64 over 53.5 MHz at
+# 1284 MHz), 512 beams, CB tscrunch 8, 1024 DM trials 0:1000:0.9765625, one
+# dispatch of the aggregation buffer (max(10 max_delay, 8192) samples)
+ID_F, ID_B, ID_CBTS, ID_BW = 256, 512, 8, 53.5e6
+ID_DMS = np.arange(1024, dtype=np.float32) * np.float32(0.9765625)
 
 
 def log(msg):
@@ -110,6 +121,43 @@ def cpu_baseline(sw, orc, taftp, weights, nchan_sample, nthreads):
     return time.perf_counter() - t0
 
 
+def bench_idisp(sw, steps):
+    """One aggregation-buffer dispatch of the incoherent dedisperser, timed
+    with HIP events on its stream; inputs resident in HBM."""
+    from skyweaver_amd import DeviceBuffer
+    idd = sw.IncoherentDedisperser(ID_F, CFREQ, ID_BW, ID_CBTS, ID_DMS, tscrunch=1)
+    disp = max(10 * idd.max_delay, 8192)
+    nsamp = disp + idd.max_delay
+    nout = idd.nout(nsamp)
+    # input: a 64 MiB seeded block tiled over the dispatch (sums are data-independent in cost)
+    tile = sw.synth_voltages((1 << 25, 2), seed=0x5EED0005, sigma=20.0, nthreads=16).reshape(-1)
+    d_in = DeviceBuffer(nsamp * ID_F * ID_B)
+    d_tile = DeviceBuffer.from_host(tile)
+    for off in range(0, d_in.nbytes, tile.nbytes):
+        n = min(tile.nbytes, d_in.nbytes - off)
+        sw.check(sw.lib().sw_memcpy_dtod(d_in.ptr + off, d_tile.ptr, n, None))
+    d_out = DeviceBuffer(nout * len(ID_DMS) * ID_B)
+    st = sw.Stream()
+    idd.dedisperse(d_in, nsamp, ID_B, 1, d_out, stream=st)   # warm-up
+    e0, e1 = sw.Event(), sw.Event()
+    e0.record(st)
+    for _ in range(steps):
+        idd.dedisperse(d_in, nsamp, ID_B, 1, d_out, stream=st)
+    e1.record(st)
+    st.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    adds = float(nout) * len(ID_DMS) * ID_B * ID_F
+    nbytes = d_in.nbytes + d_out.nbytes
+    tsamp_cb = ID_CBTS * ID_F / ID_BW
+    return {"kernel": "idisp_kernel", "ms_per_dispatch": round(ms, 3), "steps": steps,
+            "dispatch_samples": disp, "max_delay": idd.max_delay, "ndms": len(ID_DMS), "nchans": ID_F,
+            "nbeams": ID_B, "adds_per_dispatch": adds,
+            "adds_per_s": adds / (ms * 1e-3), "valu_add16_peak": VALU_ADD16_PEAK,
+            "frac_valu": round(adds / (ms * 1e-3) / VALU_ADD16_PEAK, 4),
+            "hbm_GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+            "xrealtime": round(disp * tsamp_cb / (ms * 1e-3), 2)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +166,7 @@ def main():
     ap.add_argument("--cpu-channels", type=int, default=0, help="channels in the CPU sample (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-idisp", action="store_true", help="skip the incoherent dedispersion measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path is built for one GPU); --gpus {args.gpus} is not "
@@ -154,7 +203,7 @@ def main():
     achieved = ops / (cb_ms * 1e-3) / 1e12
     roofline = {"kernel": "cb_mfma_kernel", "bound": "mfma", "achieved": round(achieved, 1),
                 "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4),
-                "traffic": None, "ops_per_launch": ops, "avg_launch_ms": round(cb_ms, 4),
+                "traffic": CB_TRAFFIC_BYTES, "ops_per_launch": ops, "avg_launch_ms": round(cb_ms, 4),
                 "note": "int8 operations (1 MAC = 2 ops); peak is the dense int8 MFMA rate"}
     hbm = {}
     for k, nbytes in (("transpose", 2 * T * F * 2 * A * 2), ("statistics", T * F * 2 * A * 2),
@@ -162,6 +211,11 @@ def main():
         if stage[k] > 0:
             gbs = nbytes / (stage[k] * 1e-3) / 1e9
             hbm[k] = {"GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 3)}
+
+    idisp = None
+    if not args.no_idisp:
+        log("incoherent dedispersion (1024 DMs x 256 channels x 512 beams)")
+        idisp = bench_idisp(sw, max(1, args.steps // 5))
 
     cpu = None
     if not args.no_cpu:
@@ -201,6 +255,7 @@ def main():
         "hbm_kernels": hbm,
         "plain": plain,
         "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
+        "incoherent_dedispersion": idisp,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

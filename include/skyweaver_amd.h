@@ -17,8 +17,7 @@
  *   TDB    [T'][D][B]                   incoherently dedispersed beams
  * Streams are hipStream_t passed as void* (NULL = default stream).
  *
- * Reference paths below are relative to the skyweaver repository root
- * (sw/ = cpp/skyweaver/).
+ * Reference paths below are relative to the skyweaver repository root.
  */
 #ifndef SKYWEAVER_AMD_H
 #define SKYWEAVER_AMD_H
@@ -188,6 +187,50 @@ int sw_delay_model_beamsets(const sw_delay_model* model, int* nbeamsets, int* h_
 int sw_delay_model_get(sw_delay_model* model, double epoch, float* h_delays,
                        double* model_epoch);
 
+/* ---------------------------------------------------- IncoherentDedisperser
+ * PipelineConfig::calculate_channel_frequencies,
+ * cpp/skyweaver/src/PipelineConfig.cpp:188-203: centre - bw/2 + chbw/2 + c chbw. */
+int sw_channel_frequencies(double centre_frequency, double bandwidth, int nchans,
+                           double* h_freqs);
+/* DMSampleDelay over a DM list, cpp/skyweaver/dedispersion_utils.cuh:19-40
+ * (f_ref = chan_freqs[0]): h_delays [ndms][nchans] samples. */
+int sw_id_delays(const float* dms, int ndms, const double* chan_freqs, int nchans,
+                 double tsamp, int* h_delays, int* max_delay);
+/* DedispersionPlan::add_block(std::string),
+ * cpp/skyweaver/src/DedispersionPlan.cpp:17-78: "cdm", "cdm:tscrunch" or
+ * "cdm:begin:end:step:tscrunch" -> coherent DM, tscrunch and the incoherent
+ * DM list (h_dms may be NULL to query ndms). */
+int sw_ddplan_parse_block(const char* spec, float* coherent_dm, int* tscrunch,
+                          float* h_dms, int max_dms, int* ndms);
+
+/* IncoherentDedisperser(config, dms, tscrunch) + dedisperse(tfb, tdb),
+ * cpp/skyweaver/IncoherentDedisperser.cuh:11-39,
+ * cpp/skyweaver/src/IncoherentDedisperser.cu:15-125.  Channel frequencies
+ * are chan_freqs if given, else PipelineConfig's; the delay sampling time is
+ * cb_tscrunch / (bandwidth / nchans). */
+typedef struct {
+    int nchans;
+    double centre_frequency; /* Hz */
+    double bandwidth;        /* Hz */
+    const double* chan_freqs; /* optional [nchans] Hz */
+    int cb_tscrunch;
+    const float* dms;
+    int ndms;
+    int tscrunch;
+} sw_idisp_config;
+typedef struct sw_idisp sw_idisp;
+int sw_idisp_create(const sw_idisp_config* config, sw_idisp** out);
+int sw_idisp_destroy(sw_idisp* plan);
+/* max_sample_delay(), delays() [ndms][nchans] and the scale sqrt(F ts);
+ * any output may be NULL. */
+int sw_idisp_info(const sw_idisp* plan, int* max_delay, int* h_delays, float* scale);
+/* d_tfb: [nsamples][nchans][nbeams * nvec] int8 (nvec = 4: char4 IQUV);
+ * d_tdb: [(nsamples - max_delay) / tscrunch][ndms][nbeams * nvec] int8.
+ * nsamples must exceed max_delay and (nsamples - max_delay) be a multiple
+ * of tscrunch (the reference's errors); nbeams * nvec % 16 == 0. */
+int sw_idisp_dedisperse(sw_idisp* plan, const void* d_tfb, size_t nsamples, int nbeams,
+                        int nvec, void* d_tdb, void* stream);
+
 /* ------------------------------------------------------ CoherentDedisperser
  * create_coherent_dedisperser_config + CoherentDedisperser::dedisperse,
  * cpp/skyweaver/CoherentDedisperser.cuh:55-92, cpp/skyweaver/src/CoherentDedisperser.cu:19-353.
@@ -279,6 +322,43 @@ typedef struct sw_block {
 
 typedef int (*sw_sink_fn)(void* user, const sw_block* block);
 
+/* ------------------------------------------ IncoherentDedispersionPipeline
+ * IncoherentDedispersionPipeline(config, handler) + AggregationBuffer,
+ * cpp/skyweaver/detail/IncoherentDedispersionPipeline.cu:6-113,
+ * cpp/skyweaver/detail/AggregationBuffer.cu:23-116.  One block per
+ * dedispersion-plan entry (coherent DM, tscrunch, incoherent DMs).  Each
+ * block aggregates pushed TFB samples ([T][F][B] int8, x4 for IQUV) in a
+ * device ring of dispatch + max_delay samples, with dispatch =
+ * max(10 max_delay, 8192) rounded down to a multiple of tscrunch (or
+ * dispatch_size if > 0); a full ring is dedispersed on the GPU and handed to
+ * the sink as a device TDB block, then its last max_delay samples move to
+ * the front. */
+typedef struct {
+    int nchans;
+    double centre_frequency;  /* Hz */
+    double bandwidth;         /* Hz */
+    const double* chan_freqs; /* optional [nchans] */
+    int cb_tscrunch;
+    int nbeams;
+    int nvec;                 /* 1: int8, 4: char4 (IQUV) */
+    int nblocks;
+    const float* coherent_dms;   /* [nblocks] */
+    const int* tscrunch;         /* [nblocks] */
+    const float* const* dms;     /* [nblocks][ndms[i]] incoherent DMs */
+    const int* ndms;             /* [nblocks] */
+    int dispatch_size;           /* 0: the reference's rule */
+} sw_idpipe_config;
+typedef struct sw_idpipe sw_idpipe;
+int sw_idpipe_create(const sw_idpipe_config* config, sw_idpipe** out);
+int sw_idpipe_destroy(sw_idpipe* pipeline);
+int sw_idpipe_set_sink(sw_idpipe* pipeline, sw_sink_fn sink, void* user);
+int sw_idpipe_info(const sw_idpipe* pipeline, int block, int* max_delay, size_t* dispatch_size,
+                   size_t* filled);
+/* IncoherentDedispersionPipeline::operator()(TFB block, dm_idx): tsamp and
+ * utc_offset describe the pushed TFB samples (the first one's time). */
+int sw_idpipe_push(sw_idpipe* pipeline, int block, const void* tfb, size_t nsamples,
+                   int on_device, double utc_offset, double tsamp, void* stream);
+
 typedef struct sw_pipeline_config {
     int nantennas;          /* A (padded; multiple of 4) */
     int nchans;             /* F */
@@ -297,12 +377,24 @@ typedef struct sw_pipeline_config {
     const int* overlap;     /* per DM or NULL */
     int cb_form;            /* enum sw_cb_form */
     int batched_dms;        /* 1: share forward FFTs across DMs */
+    /* Incoherent dedispersion of the coherent beams (PipelineConfig::
+     * enable_incoherent_dedispersion and the DedispersionPlan blocks): when
+     * id_enable is set, every coherent DM's TFB output is also pushed into an
+     * IncoherentDedispersionPipeline whose TDB blocks go to the TDB sink.
+     * Per coherent DM: id_tscrunch[i] (NULL: 1) and the incoherent DM list
+     * id_dms[i][id_ndms[i]] (NULL: the coherent DM alone). */
+    int id_enable;
+    const int* id_tscrunch;
+    const float* const* id_dms;
+    const int* id_ndms;
+    int id_dispatch_size;   /* 0: max(10 max_delay, 8192) */
 } sw_pipeline_config;
 
 typedef struct sw_obs_header {
     int nantennas;          /* antennas present in the TAFTP input (<= nantennas) */
     double utc_start;       /* unix seconds */
     double sample_clock;    /* Hz */
+    double utc_offset;      /* s; BeamformerPipeline::init's utc_offset (file start offset) */
 } sw_obs_header;
 
 typedef struct sw_pipeline sw_pipeline;
@@ -314,6 +406,14 @@ int sw_pipeline_set_sinks(sw_pipeline* pipeline, sw_sink_fn cb, void* cb_user, s
 /* Static beamforming weights [F][B][A] complex int8 (host), used for every
  * gulp until replaced. */
 int sw_pipeline_set_weights(sw_pipeline* pipeline, const void* h_weights);
+/* Weights from a delay-model file instead (the reference's DelayManager +
+ * WeightsManager path, cpp/skyweaver/detail/BeamformerPipeline.cu:133-147):
+ * beamsets come from the first model; on every gulp the model valid at the
+ * gulp's unix time utc_start + utc_offset + n 2 total_nchans T / sample_clock
+ * is read and the weights are generated on the device for that time. */
+int sw_pipeline_set_delay_file(sw_pipeline* pipeline, const char* path);
+/* TDB blocks of the incoherent dedispersion (id_enable). */
+int sw_pipeline_set_tdb_sink(sw_pipeline* pipeline, sw_sink_fn tdb, void* tdb_user);
 /* Beamsets: nbeamsets, beam -> beamset map [B], antenna weights [S][A]. */
 int sw_pipeline_set_beamsets(sw_pipeline* pipeline, int nbeamsets, const int* h_map,
                              const float* h_antenna_weights);
@@ -323,8 +423,11 @@ int sw_pipeline_init(sw_pipeline* pipeline, const sw_obs_header* header);
  * operator() always returns false). */
 int sw_pipeline_process(sw_pipeline* pipeline, const void* taftp, size_t bytes, int on_device);
 /* Per-stage device times of the last gulp (ms), in the order transpose,
- * statistics, dedispersion, incoherent beam, coherent beam, other. */
-int sw_pipeline_stage_times(const sw_pipeline* pipeline, float* ms6);
+ * statistics, dedispersion, incoherent beam, coherent beam, incoherent
+ * dedispersion (with its aggregation), weights (delay-model path), other;
+ * n <= SW_NSTAGES entries are written. */
+#define SW_NSTAGES 8
+int sw_pipeline_stage_times(const sw_pipeline* pipeline, float* ms, int n);
 /* The processing stream (hipStream_t) the pipeline enqueues work on. */
 void* sw_pipeline_stream(const sw_pipeline* pipeline);
 

@@ -9,7 +9,7 @@
  * cpu_baseline leg of bench.py may load it.
  *
  * Every function cites the reference file:line it follows.  Paths are
- * relative to the reference repository root; "sw/" = cpp/skyweaver/.
+ * relative to the reference repository root.
  *
  * Floating-point contract: compiled with -ffp-contract=off (no FMA
  * contraction) and IEEE semantics, like the host compilation of the

@@ -53,6 +53,12 @@ class BfDims(ctypes.Structure):
         "nchans", "nsamples", "nantennas", "nbeams", "tscrunch", "stokes", "nbeamsets", "ib_subtract")]
 
 
+class IdispConfig(ctypes.Structure):
+    _fields_ = [("nchans", ctypes.c_int), ("centre_frequency", ctypes.c_double), ("bandwidth", ctypes.c_double),
+                ("chan_freqs", ctypes.c_void_p), ("cb_tscrunch", ctypes.c_int), ("dms", ctypes.c_void_p),
+                ("ndms", ctypes.c_int), ("tscrunch", ctypes.c_int)]
+
+
 class CodedispConfig(ctypes.Structure):
     _fields_ = [("nchans", ctypes.c_int), ("nantennas", ctypes.c_int), ("nsamples", ctypes.c_int),
                 ("low_freq", ctypes.c_double), ("chan_bw", ctypes.c_double), ("tsamp", ctypes.c_double),
@@ -117,6 +123,14 @@ def _declare(L):
     L.sw_delay_model_close.argtypes = [_vp]
     L.sw_delay_model_beamsets.argtypes = [_vp, ctypes.POINTER(_c), _vp, _vp, _c]
     L.sw_delay_model_get.argtypes = [_vp, _d, _vp, ctypes.POINTER(_d)]
+    L.sw_channel_frequencies.argtypes = [_d, _d, _c, _vp]
+    L.sw_id_delays.argtypes = [_vp, _c, _vp, _c, _d, _vp, ctypes.POINTER(_c)]
+    L.sw_ddplan_parse_block.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(_c), _vp, _c,
+                                        ctypes.POINTER(_c)]
+    L.sw_idisp_create.argtypes = [ctypes.POINTER(IdispConfig), _pvp]
+    L.sw_idisp_destroy.argtypes = [_vp]
+    L.sw_idisp_info.argtypes = [_vp, ctypes.POINTER(_c), _vp, ctypes.POINTER(ctypes.c_float)]
+    L.sw_idisp_dedisperse.argtypes = [_vp, _vp, _sz, _c, _c, _vp, _vp]
 
 
 def last_error() -> str:
@@ -354,6 +368,74 @@ def generate_weights(delays, weights, chan_freqs, nantennas, nbeams, nchans, cur
                                     float(current_epoch), float(delay_epoch), _stream(stream)))
 
 
+def channel_frequencies(centre_frequency, bandwidth, nchans):
+    """PipelineConfig::channel_frequencies (Hz, FP64)."""
+    import numpy as np
+    out = np.zeros(nchans, np.float64)
+    check(lib().sw_channel_frequencies(float(centre_frequency), float(bandwidth), nchans, out.ctypes.data))
+    return out
+
+
+def id_delays(dms, chan_freqs, tsamp):
+    """DMSampleDelay table [ndms][nchans] and its maximum."""
+    import numpy as np
+    dms = np.ascontiguousarray(dms, np.float32)
+    cf = np.ascontiguousarray(chan_freqs, np.float64)
+    out = np.zeros((len(dms), len(cf)), np.int32)
+    mx = ctypes.c_int()
+    check(lib().sw_id_delays(dms.ctypes.data, len(dms), cf.ctypes.data, len(cf), float(tsamp), out.ctypes.data,
+                             ctypes.byref(mx)))
+    return out, mx.value
+
+
+def ddplan_parse_block(spec):
+    """DedispersionPlan::add_block(std::string) -> (coherent_dm, tscrunch, incoherent DMs float32)."""
+    import numpy as np
+    cdm, ts, n = ctypes.c_float(), ctypes.c_int(), ctypes.c_int()
+    check(lib().sw_ddplan_parse_block(spec.encode(), ctypes.byref(cdm), ctypes.byref(ts), None, 0, ctypes.byref(n)))
+    dms = np.zeros(n.value, np.float32)
+    check(lib().sw_ddplan_parse_block(spec.encode(), ctypes.byref(cdm), ctypes.byref(ts), dms.ctypes.data, n.value,
+                                      ctypes.byref(n)))
+    return cdm.value, ts.value, dms
+
+
+class IncoherentDedisperser:
+    """IncoherentDedisperser(config, dms, tscrunch) over TFB device buffers (sw_idisp_* in the C ABI)."""
+
+    def __init__(self, nchans, centre_frequency, bandwidth, cb_tscrunch, dms, tscrunch=1, chan_freqs=None):
+        import numpy as np
+        self._dms = np.ascontiguousarray(dms, np.float32)
+        self._cf = None if chan_freqs is None else np.ascontiguousarray(chan_freqs, np.float64)
+        cfg = IdispConfig(nchans, float(centre_frequency), float(bandwidth),
+                          None if self._cf is None else self._cf.ctypes.data, cb_tscrunch, self._dms.ctypes.data,
+                          len(self._dms), tscrunch)
+        h = ctypes.c_void_p()
+        check(lib().sw_idisp_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.nchans, self.ndms, self.tscrunch = nchans, len(self._dms), tscrunch
+        md, sc = ctypes.c_int(), ctypes.c_float()
+        self.delays = np.zeros((self.ndms, nchans), np.int32)
+        check(lib().sw_idisp_info(h, ctypes.byref(md), self.delays.ctypes.data, ctypes.byref(sc)))
+        self.max_delay, self.scale = md.value, sc.value
+
+    def nout(self, nsamples):
+        return (nsamples - self.max_delay) // self.tscrunch
+
+    def dedisperse(self, tfb, nsamples, nbeams, nvec, tdb, stream=None):
+        check(lib().sw_idisp_dedisperse(self._h, _ptr(tfb), nsamples, nbeams, nvec, _ptr(tdb), _stream(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sw_idisp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DelayModelFile:
     """DelayManager over a delay-model file (sw_delay_model_* in the C ABI)."""
 
@@ -452,11 +534,14 @@ class PipelineConfigC(ctypes.Structure):
                 ("centre_frequency", ctypes.c_double), ("bandwidth", ctypes.c_double),
                 ("total_nchans", ctypes.c_int), ("ndms", ctypes.c_int),
                 ("coherent_dms", ctypes.POINTER(ctypes.c_float)), ("fft_len", ctypes.POINTER(ctypes.c_int)),
-                ("overlap", ctypes.POINTER(ctypes.c_int)), ("cb_form", ctypes.c_int), ("batched_dms", ctypes.c_int)]
+                ("overlap", ctypes.POINTER(ctypes.c_int)), ("cb_form", ctypes.c_int), ("batched_dms", ctypes.c_int),
+                ("id_enable", ctypes.c_int), ("id_tscrunch", ctypes.c_void_p), ("id_dms", ctypes.c_void_p),
+                ("id_ndms", ctypes.c_void_p), ("id_dispatch_size", ctypes.c_int)]
 
 
 class ObsHeaderC(ctypes.Structure):
-    _fields_ = [("nantennas", ctypes.c_int), ("utc_start", ctypes.c_double), ("sample_clock", ctypes.c_double)]
+    _fields_ = [("nantennas", ctypes.c_int), ("utc_start", ctypes.c_double), ("sample_clock", ctypes.c_double),
+                ("utc_offset", ctypes.c_double)]
 
 
 class BlockC(ctypes.Structure):
@@ -479,12 +564,15 @@ def _declare_pipeline(L):
     L.sw_pipeline_set_beamsets.argtypes = [_vp, _c, _vp, _vp]
     L.sw_pipeline_init.argtypes = [_vp, ctypes.POINTER(ObsHeaderC)]
     L.sw_pipeline_process.argtypes = [_vp, _vp, _sz, _c]
-    L.sw_pipeline_stage_times.argtypes = [_vp, ctypes.POINTER(ctypes.c_float)]
+    L.sw_pipeline_stage_times.argtypes = [_vp, ctypes.POINTER(ctypes.c_float), _c]
+    L.sw_pipeline_set_delay_file.argtypes = [_vp, ctypes.c_char_p]
+    L.sw_pipeline_set_tdb_sink.argtypes = [_vp, SINK_FN, _vp]
     L.sw_pipeline_stream.argtypes = [_vp]
     L.sw_pipeline_stream.restype = _vp
 
 
-STAGES = ("transpose", "statistics", "dedispersion", "incoherent_beam", "coherent_beam", "other")
+STAGES = ("transpose", "statistics", "dedispersion", "incoherent_beam", "coherent_beam", "incoherent_dedispersion",
+          "weights", "other")
 
 
 class BeamformerPipeline:
@@ -492,7 +580,8 @@ class BeamformerPipeline:
 
     def __init__(self, nantennas, nchans, nbeams, nsamples, tscrunch, dms, centre_frequency, bandwidth,
                  stokes="I", ib_subtract=True, output_level=24.0, fft_len=None, overlap=None, form="mfma",
-                 batched=True, total_nchans=4096):
+                 batched=True, total_nchans=4096, id_plan=None, id_dispatch_size=0):
+        """id_plan: None (no incoherent dedispersion) or, per coherent DM, (tscrunch, incoherent DMs)."""
         import numpy as np
         L = lib()
         if not hasattr(L, "_pipeline_declared"):
@@ -509,10 +598,22 @@ class BeamformerPipeline:
                               self._fft.ctypes.data_as(ip) if self._fft is not None else None,
                               self._ov.ctypes.data_as(ip) if self._ov is not None else None,
                               FORMS[form], 1 if batched else 0)
+        if id_plan is not None:
+            assert len(id_plan) == n
+            self._id_ts = np.array([b[0] for b in id_plan], np.int32)
+            self._id_dms = [np.ascontiguousarray(b[1], np.float32) for b in id_plan]
+            self._id_ptrs = (ctypes.c_void_p * n)(*[d.ctypes.data for d in self._id_dms])
+            self._id_nd = np.array([len(d) for d in self._id_dms], np.int32)
+            cfg.id_enable = 1
+            cfg.id_tscrunch = self._id_ts.ctypes.data
+            cfg.id_dms = ctypes.cast(self._id_ptrs, ctypes.c_void_p)
+            cfg.id_ndms = self._id_nd.ctypes.data
+            cfg.id_dispatch_size = id_dispatch_size
         h = ctypes.c_void_p()
         check(L.sw_pipeline_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value
         self._sinks = []
+        self._tdb_sink = SINK_FN()
 
     def set_weights(self, weights_np):
         import numpy as np
@@ -538,8 +639,19 @@ class BeamformerPipeline:
         check(lib().sw_pipeline_set_sinks(self.handle, self._sinks[0], None, self._sinks[1], None,
                                           self._sinks[2], None))
 
-    def init(self, nantennas, utc_start=1708082229.0, sample_clock=1712e6):
-        hdr = ObsHeaderC(nantennas, utc_start, sample_clock)
+    def set_delay_file(self, path):
+        """Weights from a delay-model file, regenerated on the device every gulp."""
+        check(lib().sw_pipeline_set_delay_file(self.handle, str(path).encode()))
+
+    def set_tdb_sink(self, fn):
+        def _f(user, blk):
+            fn(blk.contents)
+            return 0
+        self._tdb_sink = SINK_FN(_f)
+        check(lib().sw_pipeline_set_tdb_sink(self.handle, self._tdb_sink, None))
+
+    def init(self, nantennas, utc_start=1708082229.0, sample_clock=1712e6, utc_offset=0.0):
+        hdr = ObsHeaderC(nantennas, utc_start, sample_clock, utc_offset)
         check(lib().sw_pipeline_init(self.handle, ctypes.byref(hdr)))
 
     def __call__(self, taftp, nbytes=None):
@@ -553,8 +665,8 @@ class BeamformerPipeline:
         return False
 
     def stage_times(self):
-        buf = (ctypes.c_float * 6)()
-        check(lib().sw_pipeline_stage_times(self.handle, buf))
+        buf = (ctypes.c_float * len(STAGES))()
+        check(lib().sw_pipeline_stage_times(self.handle, buf, len(STAGES)))
         return dict(zip(STAGES, list(buf)))
 
     @property
@@ -580,3 +692,76 @@ def download_block(blk):
     check(lib().sw_memcpy_dtoh(out.ctypes.data, blk.data, blk.bytes, None))
     check(lib().sw_stream_synchronize(None))
     return out
+
+
+class IdPipelineConfigC(ctypes.Structure):
+    _fields_ = [("nchans", ctypes.c_int), ("centre_frequency", ctypes.c_double), ("bandwidth", ctypes.c_double),
+                ("chan_freqs", ctypes.c_void_p), ("cb_tscrunch", ctypes.c_int), ("nbeams", ctypes.c_int),
+                ("nvec", ctypes.c_int), ("nblocks", ctypes.c_int), ("coherent_dms", ctypes.c_void_p),
+                ("tscrunch", ctypes.c_void_p), ("dms", ctypes.c_void_p), ("ndms", ctypes.c_void_p),
+                ("dispatch_size", ctypes.c_int)]
+
+
+class IncoherentDedispersionPipeline:
+    """IncoherentDedispersionPipeline over device TFB blocks (sw_idpipe_* in the C ABI).
+
+    blocks: list of (coherent_dm, tscrunch, incoherent_dms) -- one per plan block."""
+
+    def __init__(self, nchans, centre_frequency, bandwidth, cb_tscrunch, nbeams, nvec, blocks, dispatch_size=0,
+                 chan_freqs=None):
+        import numpy as np
+        L = lib()
+        _c, _vp, _sz, _d = ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double
+        L.sw_idpipe_create.argtypes = [ctypes.POINTER(IdPipelineConfigC), ctypes.POINTER(ctypes.c_void_p)]
+        L.sw_idpipe_destroy.argtypes = [_vp]
+        L.sw_idpipe_set_sink.argtypes = [_vp, SINK_FN, _vp]
+        L.sw_idpipe_info.argtypes = [_vp, _c, ctypes.POINTER(_c), ctypes.POINTER(_sz), ctypes.POINTER(_sz)]
+        L.sw_idpipe_push.argtypes = [_vp, _c, _vp, _sz, _c, _d, _d, _vp]
+        self._cdms = np.array([b[0] for b in blocks], np.float32)
+        self._ts = np.array([b[1] for b in blocks], np.int32)
+        self._dms = [np.ascontiguousarray(b[2], np.float32) for b in blocks]
+        self._ptrs = (ctypes.c_void_p * len(blocks))(*[d.ctypes.data for d in self._dms])
+        self._nd = np.array([len(d) for d in self._dms], np.int32)
+        self._cf = None if chan_freqs is None else np.ascontiguousarray(chan_freqs, np.float64)
+        cfg = IdPipelineConfigC(nchans, float(centre_frequency), float(bandwidth),
+                                None if self._cf is None else self._cf.ctypes.data, cb_tscrunch, nbeams, nvec,
+                                len(blocks), self._cdms.ctypes.data, self._ts.ctypes.data,
+                                ctypes.cast(self._ptrs, ctypes.c_void_p), self._nd.ctypes.data, dispatch_size)
+        h = ctypes.c_void_p()
+        check(L.sw_idpipe_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self._sink = SINK_FN()
+
+    def info(self, block):
+        md, ds, fl = ctypes.c_int(), ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().sw_idpipe_info(self.handle, block, ctypes.byref(md), ctypes.byref(ds), ctypes.byref(fl)))
+        return {"max_delay": md.value, "dispatch_size": ds.value, "filled": fl.value}
+
+    def set_sink(self, fn):
+        """fn(block: BlockC) -> None, invoked synchronously for every TDB block."""
+        def _f(user, blk):
+            fn(blk.contents)
+            return 0
+        self._sink = SINK_FN(_f)
+        check(lib().sw_idpipe_set_sink(self.handle, self._sink, None))
+
+    def push(self, block, tfb, nsamples, utc_offset, tsamp, stream=None):
+        import numpy as np
+        if isinstance(tfb, np.ndarray):
+            arr = np.ascontiguousarray(tfb)
+            check(lib().sw_idpipe_push(self.handle, block, arr.ctypes.data, nsamples, 0, float(utc_offset),
+                                       float(tsamp), _stream(stream)))
+        else:
+            check(lib().sw_idpipe_push(self.handle, block, _ptr(tfb), nsamples, 1, float(utc_offset), float(tsamp),
+                                       _stream(stream)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().sw_idpipe_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

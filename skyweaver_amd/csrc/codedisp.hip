@@ -145,6 +145,130 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
     }
 }
 
+// N = 16384 held in registers: 512 threads x 32 points, one column per
+// workgroup.  With n = 512 n1 + 16 n2 + n3 and k = k1 + 32 k2 + 1024 k3
+// (n1, n2, k1, k2 < 32; n3, k3 < 16) the forward transform is
+//   DFT32 over n1 (thread t = 16 n2 + n3), x W^(t k1)
+//   -> exchange -> DFT32 over n2 (thread (k1, n3)), x W^(32 n3 k2)
+//   -> exchange -> DFT16 over n3 (thread (k1, k2), two per thread),
+// which leaves X[k] in registers in the order the chirp is read; the
+// inverse runs the same steps backwards with conjugate twiddles and ends
+// with each thread holding output samples 512 n1 + t.  Four LDS exchanges
+// per DM instead of a read-write round trip per radix pass.
+constexpr int k16 = 16384;
+constexpr int kEx2Stride = 17;  // padded row of 16 points (conflict-free reads)
+constexpr int k16Lds = 1024 * kEx2Stride;
+
+// v[k] *= w1^k for k = 1..R-1 (R = 16 or 32), powers by the product tree of
+// twiddle_powers, built 16 at a time to bound register use
+template <int R>
+__device__ __forceinline__ void apply_powers(float2* v, float2 w1)
+{
+    float2 w[16];
+    twiddle_powers<16>(w1, w);
+#pragma unroll
+    for(int k = 1; k < 16; ++k) v[k] = cmul(v[k], w[k]);
+    if constexpr(R == 32) {
+        const float2 w16 = cmul(w[8], w[8]);
+        v[16] = cmul(v[16], w16);
+#pragma unroll
+        for(int k = 1; k < 16; ++k) v[16 + k] = cmul(v[16 + k], cmul(w16, w[k]));
+    }
+}
+
+// One DM per launch (d): forward FFT, chirp, inverse FFT of one column.
+__global__ void __launch_bounds__(512) codedisp_16k_kernel(CdParams p, int d)
+{
+    __shared__ float2 lds[k16Lds + TwiddleLds<k16>::SIZE];
+    const Twiddles tw = load_twiddles<k16, 512>(lds + k16Lds, p.tw);
+    const int col = blockIdx.x % p.PA;
+    const int fs  = blockIdx.x / p.PA;
+    const int f = fs / p.nseg, s = fs % p.nseg;
+    const int t = threadIdx.x;
+    const float inv_n = 1.0f / (float)k16;
+    float2 v[32];
+
+    // ---- forward, step 1: thread t = (n2, n3) holds n1 = 0..31
+#pragma unroll
+    for(int n1 = 0; n1 < 32; ++n1) v[n1] = input_sample(p, f, col, s * p.hop - p.O + 512 * n1 + t);
+    __syncthreads();  // twiddle tables
+    dft32<false>(v);
+    apply_powers<32>(v, tw(t));
+#pragma unroll
+    for(int k1 = 0; k1 < 32; ++k1) lds[k1 * 512 + t] = v[k1];
+    __syncthreads();
+    // ---- step 2: thread (k1, n3) holds n2 = 0..31
+    {
+        const int k1 = t >> 4, n3 = t & 15;
+#pragma unroll
+        for(int n2 = 0; n2 < 32; ++n2) v[n2] = lds[k1 * 512 + n2 * 16 + n3];
+        __syncthreads();
+        dft32<false>(v);
+        apply_powers<32>(v, tw(32 * n3));
+#pragma unroll
+        for(int k2 = 0; k2 < 32; ++k2) lds[(k1 * 32 + k2) * kEx2Stride + n3] = v[k2];
+    }
+    __syncthreads();
+    // ---- step 3: pairs q = t, t + 512 (q = 32 k2 + k1): DFT16 over n3 -> k3,
+    // chirp at k = q + 1024 k3, then the inverse step 3': IDFT16 -> n3,
+    // x W^(-32 k2 n3)
+    const float2* H = p.chirp[d] + (size_t)f * k16;
+#pragma unroll
+    for(int h = 0; h < 2; ++h) {
+        const int q = t + 512 * h;
+        const int k1 = q & 31, k2 = q >> 5;
+        float2* x = v + 16 * h;
+#pragma unroll
+        for(int n3 = 0; n3 < 16; ++n3) x[n3] = lds[(k1 * 32 + k2) * kEx2Stride + n3];
+        dft16<false>(x);
+#pragma unroll
+        for(int k3 = 0; k3 < 16; ++k3) x[k3] = cmul(x[k3], H[q + 1024 * k3]);
+        dft16<true>(x);
+        apply_powers<16>(x, cconj(tw(32 * k2)));
+    }
+    __syncthreads();  // all step-3 reads done before the exchange buffer is rewritten
+#pragma unroll
+    for(int h = 0; h < 2; ++h) {
+        const int q = t + 512 * h;
+        const int k1 = q & 31, k2 = q >> 5;
+#pragma unroll
+        for(int n3 = 0; n3 < 16; ++n3) lds[(k1 * 32 + k2) * kEx2Stride + n3] = v[16 * h + n3];
+    }
+    __syncthreads();
+    // ---- step 2': thread (k1, n3) holds k2 = 0..31; IDFT32 -> n2, x W^(-k1 (16 n2 + n3))
+    {
+        const int k1 = t >> 4, n3 = t & 15;
+#pragma unroll
+        for(int k2 = 0; k2 < 32; ++k2) v[k2] = lds[(k1 * 32 + k2) * kEx2Stride + n3];
+        __syncthreads();
+        dft32<true>(v);
+        const float2 b = cconj(tw(k1 * n3));
+        apply_powers<32>(v, cconj(tw(16 * k1)));
+#pragma unroll
+        for(int n2 = 0; n2 < 32; ++n2) v[n2] = cmul(v[n2], b);
+#pragma unroll
+        for(int n2 = 0; n2 < 32; ++n2) lds[k1 * 512 + n2 * 16 + n3] = v[n2];
+    }
+    __syncthreads();
+    // ---- step 1': thread t = (n2, n3) holds k1 = 0..31; IDFT32 -> n1
+#pragma unroll
+    for(int k1 = 0; k1 < 32; ++k1) v[k1] = lds[k1 * 512 + t];
+    dft32<true>(v);
+    int8_t* out = p.out[d];
+    float2* outf = p.outf[d];
+    const int half = p.O / 2;
+#pragma unroll
+    for(int n1 = 0; n1 < 32; ++n1) {
+        const int pos = 512 * n1 + t;
+        const int j = s * p.hop + pos - half;
+        if(pos >= half && pos < k16 - half && j < p.T) {
+            const float2 y = cscale(v[n1], inv_n);
+            store_c8(out + (((size_t)f * p.PA + col) * p.T + j) * 2, y);
+            if(outf) outf[((size_t)f * p.T + j) * p.PA + col] = y;
+        }
+    }
+}
+
 // Decimation-in-frequency first stage for N = RP * L:
 //   Z_r[n] = exp(-2 pi i n r / N) sum_q x[n + q L] exp(-2 pi i q r / RP)
 template <int RP>
@@ -261,7 +385,14 @@ int launch_lds(int L, const CdParams& p, int blocks, hipStream_t st)
     switch(L) {
         SW_CD(64, 128, 2, 512) SW_CD(128, 64, 2, 512) SW_CD(256, 32, 2, 512)
         SW_CD(512, 16, 2, 512) SW_CD(1024, 8, 2, 512) SW_CD(2048, 4, 2, 512)
-        SW_CD(4096, 2, 2, 512) SW_CD(8192, 1, 2, 512) SW_CD(16384, 1, 1, 512)
+        SW_CD(4096, 2, 2, 512) SW_CD(8192, 1, 2, 512)
+    case 16384:
+        if constexpr(F32)
+            hipLaunchKernelGGL((codedisp_tm_kernel<16384, 1, 1, 512, true>), dim3(blocks), dim3(512), 0, st, p);
+        else
+            for(int d = 0; d < p.ndm; ++d)
+                hipLaunchKernelGGL(codedisp_16k_kernel, dim3(blocks), dim3(512), 0, st, p, d);
+        break;
     default: set_error("codedisp: unsupported transform length"); return SW_ERR_UNSUPPORTED;
     }
 #undef SW_CD

@@ -115,6 +115,56 @@ __device__ __forceinline__ void dft16(float2* v)
     for(int k = 0; k < 16; ++k) v[k] = o[k];
 }
 
+// cos, sin of 2 pi k / 32, k < 16
+__device__ constexpr float kW32[16][2] = {
+    {1.00000000000000000000f, 0.00000000000000000000f},
+    {0.98078528040323043058f, 0.19509032201612824808f},
+    {0.92387953251128673848f, 0.38268343236508978178f},
+    {0.83146961230254523567f, 0.55557023301960217765f},
+    {0.70710678118654757274f, 0.70710678118654746172f},
+    {0.55557023301960228867f, 0.83146961230254523567f},
+    {0.38268343236508983729f, 0.92387953251128673848f},
+    {0.19509032201612833135f, 0.98078528040323043058f},
+    {0.00000000000000006123f, 1.00000000000000000000f},
+    {-0.19509032201612819257f, 0.98078528040323043058f},
+    {-0.38268343236508972627f, 0.92387953251128673848f},
+    {-0.55557023301960195560f, 0.83146961230254545772f},
+    {-0.70710678118654746172f, 0.70710678118654757274f},
+    {-0.83146961230254534669f, 0.55557023301960217765f},
+    {-0.92387953251128673848f, 0.38268343236508989280f},
+    {-0.98078528040323043058f, 0.19509032201612860891f}};
+
+// natural order in and out: radix-2 split into two DFT16 of the even and odd
+// samples, X[k] = E[k] + W32^k O[k], X[k + 16] = E[k] - W32^k O[k]
+template <bool INV>
+__device__ __forceinline__ void dft32(float2* v)
+{
+    float2 e[16], o[16];
+#pragma unroll
+    for(int i = 0; i < 16; ++i) {
+        e[i] = v[2 * i];
+        o[i] = v[2 * i + 1];
+    }
+    dft16<INV>(e);
+    dft16<INV>(o);
+#pragma unroll
+    for(int k = 0; k < 16; ++k) {
+        float2 t;
+        if(k == 0)
+            t = o[0];
+        else if(k == 8)
+            t = rot90<INV>(o[8]);
+        else if(k == 4)
+            t = rot45<INV, 1>(o[4]);
+        else if(k == 12)
+            t = rot45<INV, 3>(o[12]);
+        else
+            t = cmul(o[k], make_float2(kW32[k][0], INV ? kW32[k][1] : -kW32[k][1]));
+        v[k]      = cadd(e[k], t);
+        v[k + 16] = csub(e[k], t);
+    }
+}
+
 template <int R, bool INV>
 __device__ __forceinline__ void dft(float2* v)
 {
@@ -124,9 +174,11 @@ __device__ __forceinline__ void dft(float2* v)
         dft4<INV>(v[0], v[1], v[2], v[3]);
     } else if constexpr(R == 8) {
         dft8<INV>(v);
-    } else {
-        static_assert(R == 16, "radix");
+    } else if constexpr(R == 16) {
         dft16<INV>(v);
+    } else {
+        static_assert(R == 32, "radix");
+        dft32<INV>(v);
     }
 }
 
@@ -183,6 +235,11 @@ __device__ __forceinline__ void twiddle_powers(float2 w1, float2* w)
         w[8] = cmul(w[4], w[4]);
 #pragma unroll
         for(int k = 1; k < 8; ++k) w[8 + k] = cmul(w[8], w[k]);
+    }
+    if constexpr(R > 16) {
+        w[16] = cmul(w[8], w[8]);
+#pragma unroll
+        for(int k = 1; k < 16; ++k) w[16 + k] = cmul(w[16], w[k]);
     }
 }
 

@@ -1,0 +1,412 @@
+// idisp.hip -- incoherent (post-detection) dedispersion of TFB powers on the
+// GPU, and the DedispersionPlan block parser.
+//
+// Reference: IncoherentDedisperser::prepare / dedisperse,
+// cpp/skyweaver/src/IncoherentDedisperser.cu:36-125, with DMSampleDelay
+// (cpp/skyweaver/dedispersion_utils.cuh:19-40):
+//   delta[d][f] = (int)(K dm_d (1/f_0^2 - 1/f^2) / tsamp + 0.5), f_0 = lowest channel,
+//   tsamp = cb_tscrunch / (bandwidth / nchans)
+//   out[t'][d][b] = (int8)clamp(sum_{s<ts} sum_f in[D + t' ts + s - delta[d][f]][f][b]
+//                               / (float)sqrt(F ts))
+// with D = max delta and t' < (T - D) / ts.  The sums are exact integers, so
+// the summation order is free; the division and the truncating clamp are the
+// reference's.
+//
+// Kernel design: a workgroup owns 64 words (256 bytes) of a (t, f) row,
+// TT output samples and a tile of up to 4 DPW consecutive DMs (one wave per
+// DM residue).  For every channel it stages the union of the rows its DMs
+// read into LDS, biased by XOR 0x80 so every byte is an unsigned x + 128; each
+// lane then adds its word into two registers of 16-bit lanes (even and odd
+// bytes), i.e. four beam sums per two integer adds.  The 16-bit lanes hold
+// at most 256 biased bytes; longer sums (nchans * ts > 256) are flushed into
+// 32-bit accumulators.
+#include "sw_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+struct sw_idisp {
+    int nchans = 0, ndms = 0, tscrunch = 1;
+    int max_delay = 0;
+    float scale = 1.0f;
+    std::vector<float> dms;
+    std::vector<int> delays;          // [D][F]
+    // device tiling
+    int dpw = 2, tt = 16, rows_cap = 0;
+    bool wide = false;
+    std::vector<int2> tiles;          // (first DM, count)
+    int* d_delays = nullptr;
+    int2* d_span = nullptr;           // [ntiles][F] (delta_min, delta_max)
+    int2* d_tiles = nullptr;
+};
+
+namespace sw {
+namespace {
+
+constexpr double kDmConstHzS = 4.148806423e15;  // Hz^2 pc^-1 cm^3 s
+
+// one step of 16-bit-lane accumulation of a biased word
+__device__ __forceinline__ void id_add(uint32_t w, uint32_t& e, uint32_t& o)
+{
+    e += w & 0x00FF00FFu;
+    o += (w >> 8) & 0x00FF00FFu;
+}
+
+__device__ __forceinline__ int8_t id_quant(int32_t sum, float scale)
+{
+    // reference: clamp<int8_t>(float_sum / scale_factor), truncating cast
+    const float v = (float)sum / scale;
+    return (int8_t)fmaxf(-128.0f, fminf(127.0f, v));
+}
+
+template <int DPW, int TT, bool WIDE>
+__global__ void __launch_bounds__(256)
+idisp_kernel(const uint32_t* __restrict__ tfb, uint32_t* __restrict__ tdb,
+             const int* __restrict__ delays, const int2* __restrict__ span,
+             const int2* __restrict__ tiles, int nchans, int nwords, int ndms, int ts,
+             int max_delay, long nsamples, int nout, int tb0, float scale)
+{
+    extern __shared__ uint4 stage[];  // [rows][16] uint4 = [rows][64] words
+    const uint32_t* rows = reinterpret_cast<const uint32_t*>(stage);
+    const int tile  = blockIdx.x;
+    const int chunk = blockIdx.y;
+    const int tb    = tb0 + blockIdx.z;
+    const int wave  = threadIdx.x >> 6;
+    const int lane  = threadIdx.x & 63;
+    const int2 td   = tiles[tile];
+    const int t0    = tb * TT;
+    const long in0  = (long)max_delay + (long)t0 * ts;
+
+    uint32_t e[DPW][TT], o[DPW][TT];
+    uint32_t w32[WIDE ? DPW : 1][WIDE ? TT : 1][4];
+#pragma unroll
+    for(int j = 0; j < DPW; ++j)
+#pragma unroll
+        for(int i = 0; i < TT; ++i) {
+            e[j][i] = 0;
+            o[j][i] = 0;
+            if constexpr(WIDE) {
+#pragma unroll
+                for(int k = 0; k < 4; ++k) w32[j][i][k] = 0;
+            }
+        }
+    int pending = 0;  // biased bytes in the 16-bit lanes (WIDE only)
+
+    for(int f = 0; f < nchans; ++f) {
+        const int2 sp   = span[(size_t)tile * nchans + f];
+        const long lo   = in0 - sp.y;
+        const int nrows = TT * ts + sp.y - sp.x;
+        __syncthreads();
+        for(int i = threadIdx.x; i < nrows * 16; i += 256) {
+            const int r = i >> 4, q = i & 15;
+            const int w4 = chunk * 64 + q * 4;
+            uint4 v = make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
+            if(w4 < nwords && lo + r < nsamples) {
+                v = *reinterpret_cast<const uint4*>(tfb + ((size_t)(lo + r) * nchans + f) * nwords + w4);
+                v.x ^= 0x80808080u;
+                v.y ^= 0x80808080u;
+                v.z ^= 0x80808080u;
+                v.w ^= 0x80808080u;
+            }
+            stage[r * 16 + q] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for(int j = 0; j < DPW; ++j) {
+            const int dl = wave + 4 * j;
+            if(dl < td.y) {
+                const int delta = delays[(size_t)(td.x + dl) * nchans + f];
+                const uint32_t* src = rows + (sp.y - delta) * 64 + lane;
+#pragma unroll
+                for(int i = 0; i < TT; ++i)
+                    for(int s = 0; s < ts; ++s) id_add(src[(i * ts + s) * 64], e[j][i], o[j][i]);
+            }
+        }
+        if constexpr(WIDE) {
+            pending += ts;
+            if(pending + ts > 256 || f == nchans - 1) {
+#pragma unroll
+                for(int j = 0; j < DPW; ++j)
+#pragma unroll
+                    for(int i = 0; i < TT; ++i) {
+                        w32[j][i][0] += e[j][i] & 0xFFFFu;
+                        w32[j][i][1] += o[j][i] & 0xFFFFu;
+                        w32[j][i][2] += e[j][i] >> 16;
+                        w32[j][i][3] += o[j][i] >> 16;
+                        e[j][i] = 0;
+                        o[j][i] = 0;
+                    }
+                pending = 0;
+            }
+        }
+    }
+
+    const int word = chunk * 64 + lane;
+    if(word >= nwords) return;
+    const int32_t bias = 128 * nchans * ts;
+#pragma unroll
+    for(int j = 0; j < DPW; ++j) {
+        const int dl = wave + 4 * j;
+        if(dl >= td.y) continue;
+#pragma unroll
+        for(int i = 0; i < TT; ++i) {
+            if(t0 + i >= nout) continue;
+            int32_t s[4];
+            if constexpr(WIDE) {
+#pragma unroll
+                for(int k = 0; k < 4; ++k) s[k] = (int32_t)w32[j][i][k] - bias;
+            } else {
+                s[0] = (int32_t)(e[j][i] & 0xFFFFu) - bias;
+                s[1] = (int32_t)(o[j][i] & 0xFFFFu) - bias;
+                s[2] = (int32_t)(e[j][i] >> 16) - bias;
+                s[3] = (int32_t)(o[j][i] >> 16) - bias;
+            }
+            const uint32_t packed = (uint32_t)(uint8_t)id_quant(s[0], scale) |
+                                    ((uint32_t)(uint8_t)id_quant(s[1], scale) << 8) |
+                                    ((uint32_t)(uint8_t)id_quant(s[2], scale) << 16) |
+                                    ((uint32_t)(uint8_t)id_quant(s[3], scale) << 24);
+            tdb[((size_t)(t0 + i) * ndms + td.x + dl) * nwords + word] = packed;
+        }
+    }
+}
+
+constexpr int kRowsCap = 256;  // staged rows per channel: 64 KB of LDS
+
+void free_plan_device(sw_idisp* p)
+{
+    (void)hipFree(p->d_delays);
+    (void)hipFree(p->d_span);
+    (void)hipFree(p->d_tiles);
+    p->d_delays = nullptr;
+    p->d_span = nullptr;
+    p->d_tiles = nullptr;
+}
+
+} // namespace
+} // namespace sw
+
+extern "C" int sw_id_delays(const float* dms, int ndms, const double* chan_freqs, int nchans,
+                            double tsamp, int* h_delays, int* max_delay)
+{
+    SW_CHECK_ARG(dms && chan_freqs && h_delays && ndms > 0 && nchans > 0 && tsamp > 0, "bad argument");
+    // DMSampleDelay(dm, f_ref = channel_frequencies().front(), tsamp)(f)
+    const double f_ref = chan_freqs[0];
+    int mx = 0;
+    for(int d = 0; d < ndms; ++d)
+        for(int c = 0; c < nchans; ++c) {
+            const double f = chan_freqs[c];
+            const double delay = sw::kDmConstHzS * (1 / (f_ref * f_ref) - 1 / (f * f)) * (double)dms[d];
+            const int v = (int)(delay / tsamp + 0.5);
+            h_delays[(size_t)d * nchans + c] = v;
+            mx = std::max(mx, v);
+        }
+    if(max_delay) *max_delay = mx;
+    return SW_OK;
+}
+
+extern "C" int sw_channel_frequencies(double centre_frequency, double bandwidth, int nchans,
+                                      double* h_freqs)
+{
+    // PipelineConfig::calculate_channel_frequencies,
+    // cpp/skyweaver/src/PipelineConfig.cpp:188-203
+    SW_CHECK_ARG(h_freqs && nchans > 0, "bad argument");
+    const double chbw = bandwidth / nchans;
+    const double fbottom = centre_frequency - bandwidth / 2.0;
+    for(int c = 0; c < nchans; ++c) h_freqs[c] = fbottom + chbw / 2.0 + (chbw * c);
+    return SW_OK;
+}
+
+extern "C" int sw_ddplan_parse_block(const char* spec, float* coherent_dm, int* tscrunch,
+                                     float* h_dms, int max_dms, int* ndms)
+{
+    // DedispersionPlan::add_block(std::string), cpp/skyweaver/src/DedispersionPlan.cpp:17-78:
+    // "cdm", "cdm:tscrunch" or "cdm:begin:end:step:tscrunch"; the incoherent
+    // DMs of the last form are accumulated in float (dm += step while dm < end),
+    // and an empty range yields {begin}.
+    SW_CHECK_ARG(spec && coherent_dm && tscrunch && ndms, "null argument");
+    std::vector<std::string> parts;
+    {
+        std::stringstream ss(spec);
+        std::string item;
+        while(std::getline(ss, item, ':')) parts.push_back(item);
+    }
+    std::vector<float> dms;
+    try {
+        if(parts.size() == 1 || parts.size() == 2) {
+            *coherent_dm = std::stof(parts[0]);
+            *tscrunch = parts.size() == 2 ? std::stoi(parts[1]) : 1;
+            dms.push_back(*coherent_dm);
+        } else if(parts.size() == 5) {
+            *coherent_dm = std::stof(parts[0]);
+            const float b = std::stof(parts[1]), e = std::stof(parts[2]), st = std::stof(parts[3]);
+            *tscrunch = std::stoi(parts[4]);
+            for(float dm = b; dm < e; dm += st) {
+                dms.push_back(dm);
+                if(dms.size() > (size_t)1 << 24) break;
+            }
+            if(dms.empty()) dms.push_back(b);
+        } else {
+            sw::set_error(std::string("Block defition string of unexpected size: ") + spec);
+            return SW_ERR_INVALID;
+        }
+    } catch(std::invalid_argument& e) {
+        sw::set_error(std::string("Error parsing block definition string: ") + e.what());
+        return SW_ERR_INVALID;
+    } catch(std::out_of_range&) {
+        sw::set_error("Block definition value out of range");
+        return SW_ERR_INVALID;
+    }
+    *ndms = (int)dms.size();
+    if(h_dms) {
+        SW_CHECK_ARG(max_dms >= (int)dms.size(), "DM buffer too small");
+        std::memcpy(h_dms, dms.data(), sizeof(float) * dms.size());
+    }
+    return SW_OK;
+}
+
+extern "C" int sw_idisp_create(const sw_idisp_config* cfg, sw_idisp** out)
+{
+    SW_CHECK_ARG(cfg && out, "null argument");
+    *out = nullptr;
+    SW_CHECK_ARG(cfg->nchans > 0 && cfg->ndms > 0 && cfg->dms, "empty plan");
+    SW_CHECK_ARG(cfg->tscrunch > 0 && cfg->cb_tscrunch > 0, "tscrunch must be positive");
+    SW_CHECK_ARG(cfg->bandwidth != 0.0, "bandwidth must be non-zero");
+    auto p = std::make_unique<sw_idisp>();
+    p->nchans = cfg->nchans;
+    p->ndms = cfg->ndms;
+    p->tscrunch = cfg->tscrunch;
+    p->dms.assign(cfg->dms, cfg->dms + cfg->ndms);
+    std::vector<double> freqs(cfg->nchans);
+    if(cfg->chan_freqs)
+        freqs.assign(cfg->chan_freqs, cfg->chan_freqs + cfg->nchans);
+    else
+        sw_channel_frequencies(cfg->centre_frequency, cfg->bandwidth, cfg->nchans, freqs.data());
+    const double chbw = cfg->bandwidth / cfg->nchans;
+    const double tsamp = cfg->cb_tscrunch / chbw;
+    p->delays.resize((size_t)cfg->ndms * cfg->nchans);
+    sw_id_delays(p->dms.data(), p->ndms, freqs.data(), p->nchans, tsamp, p->delays.data(), &p->max_delay);
+    for(int v: p->delays)
+        SW_CHECK_ARG(v >= 0, "channel frequencies must not lie below the first channel's");
+    p->scale = (float)std::sqrt((double)((size_t)p->nchans * p->tscrunch));
+
+    // tiling: TT output samples per workgroup, tiles of consecutive DMs whose
+    // union of read rows fits the LDS stage
+    p->wide = (long)p->nchans * p->tscrunch > 256;
+    p->tt = p->wide ? 8 : 16;
+    if(p->tt * p->tscrunch > sw::kRowsCap / 2) p->tt = 4;
+    if(p->tt * p->tscrunch > sw::kRowsCap) {
+        sw::set_error("incoherent dedispersion tscrunch too large (at most 64)");
+        return SW_ERR_UNSUPPORTED;
+    }
+    p->dpw = 2;
+    const int dmax_tile = 4 * p->dpw;
+    const int F = p->nchans;
+    std::vector<int2> span;
+    for(int d = 0; d < p->ndms;) {
+        int n = 1;
+        while(n < dmax_tile && d + n < p->ndms) {
+            int worst = 0;
+            for(int f = 0; f < F; ++f) {
+                int lo = INT32_MAX, hi = INT32_MIN;
+                for(int k = 0; k <= n; ++k) {
+                    lo = std::min(lo, p->delays[(size_t)(d + k) * F + f]);
+                    hi = std::max(hi, p->delays[(size_t)(d + k) * F + f]);
+                }
+                worst = std::max(worst, hi - lo);
+            }
+            if(worst + p->tt * p->tscrunch > sw::kRowsCap) break;
+            ++n;
+        }
+        p->tiles.push_back(make_int2(d, n));
+        int rows = 0;
+        for(int f = 0; f < F; ++f) {
+            int lo = INT32_MAX, hi = INT32_MIN;
+            for(int k = 0; k < n; ++k) {
+                lo = std::min(lo, p->delays[(size_t)(d + k) * F + f]);
+                hi = std::max(hi, p->delays[(size_t)(d + k) * F + f]);
+            }
+            span.push_back(make_int2(lo, hi));
+            rows = std::max(rows, hi - lo + p->tt * p->tscrunch);
+        }
+        p->rows_cap = std::max(p->rows_cap, rows);
+        d += n;
+    }
+    SW_HIP_TRY(hipMalloc(&p->d_delays, sizeof(int) * p->delays.size()));
+    SW_HIP_TRY(hipMalloc(&p->d_span, sizeof(int2) * span.size()));
+    SW_HIP_TRY(hipMalloc(&p->d_tiles, sizeof(int2) * p->tiles.size()));
+    SW_HIP_TRY(hipMemcpy(p->d_delays, p->delays.data(), sizeof(int) * p->delays.size(), hipMemcpyHostToDevice));
+    SW_HIP_TRY(hipMemcpy(p->d_span, span.data(), sizeof(int2) * span.size(), hipMemcpyHostToDevice));
+    SW_HIP_TRY(hipMemcpy(p->d_tiles, p->tiles.data(), sizeof(int2) * p->tiles.size(), hipMemcpyHostToDevice));
+    *out = p.release();
+    return SW_OK;
+}
+
+extern "C" int sw_idisp_destroy(sw_idisp* p)
+{
+    if(p) {
+        sw::free_plan_device(p);
+        delete p;
+    }
+    return SW_OK;
+}
+
+extern "C" int sw_idisp_info(const sw_idisp* p, int* max_delay, int* h_delays, float* scale)
+{
+    SW_CHECK_ARG(p, "null plan");
+    if(max_delay) *max_delay = p->max_delay;
+    if(h_delays) std::memcpy(h_delays, p->delays.data(), sizeof(int) * p->delays.size());
+    if(scale) *scale = p->scale;
+    return SW_OK;
+}
+
+extern "C" int sw_idisp_dedisperse(sw_idisp* p, const void* d_tfb, size_t nsamples, int nbeams, int nvec,
+                                   void* d_tdb, void* stream)
+{
+    SW_CHECK_ARG(p && d_tfb && d_tdb, "null argument");
+    SW_CHECK_ARG(nbeams > 0 && (nvec == 1 || nvec == 4), "nvec must be 1 (int8) or 4 (char4)");
+    if(nsamples <= (size_t)p->max_delay) {
+        sw::set_error("Fewer than max_sample_delay samples passed to dedisperse method");
+        return SW_ERR_INVALID;
+    }
+    if((nsamples - p->max_delay) % p->tscrunch != 0) {
+        sw::set_error("(nsamples - max_sample_delay) must be a multiple of tscrunch;");
+        return SW_ERR_INVALID;
+    }
+    const long nbv = (long)nbeams * nvec;
+    SW_CHECK_ARG(nbv % 16 == 0, "nbeams * nvec must be a multiple of 16");
+    SW_CHECK_ARG(((uintptr_t)d_tfb & 15) == 0 && ((uintptr_t)d_tdb & 3) == 0, "misaligned buffer");
+    const int nwords = (int)(nbv / 4);
+    const int nout = (int)((nsamples - p->max_delay) / p->tscrunch);
+    const int nchunks = (nwords + 63) / 64;
+    const int ntb = (nout + p->tt - 1) / p->tt;
+    const size_t lds = (size_t)p->rows_cap * 256;
+    hipStream_t st = sw::as_stream(stream);
+    for(int tb0 = 0; tb0 < ntb; tb0 += 65535) {
+        const dim3 grid((unsigned)p->tiles.size(), (unsigned)nchunks, (unsigned)std::min(65535, ntb - tb0));
+#define SW_ID_LAUNCH(TT, WIDE)                                                                             \
+    hipLaunchKernelGGL((sw::idisp_kernel<2, TT, WIDE>), grid, dim3(256), lds, st,                         \
+                       static_cast<const uint32_t*>(d_tfb), static_cast<uint32_t*>(d_tdb), p->d_delays,   \
+                       p->d_span, p->d_tiles, p->nchans, nwords, p->ndms, p->tscrunch, p->max_delay,      \
+                       (long)nsamples, nout, tb0, p->scale)
+        if(p->wide) {
+            if(p->tt == 8)
+                SW_ID_LAUNCH(8, true);
+            else
+                SW_ID_LAUNCH(4, true);
+        } else {
+            if(p->tt == 16)
+                SW_ID_LAUNCH(16, false);
+            else
+                SW_ID_LAUNCH(4, false);
+        }
+#undef SW_ID_LAUNCH
+        SW_LAUNCH_CHECK();
+    }
+    return SW_OK;
+}

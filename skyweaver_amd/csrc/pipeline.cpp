@@ -5,12 +5,19 @@
 //   (H2D) -> transpose -> statistics -> scalings (first gulp only) ->
 //   coherent dedispersion of all DMs -> per DM: incoherent beam, coherent
 //   beam, CB handler, IB handler -> stats handler.
+// With a delay-model file, the delays valid at the gulp's unix time are
+// fetched and the weights regenerated on the device before the transpose
+// (BeamformerPipeline.cu:133-147).  With incoherent dedispersion enabled,
+// every coherent DM's TFB block is also pushed into the incoherent
+// dedispersion pipeline (the reference installs it as the CB handler,
+// cpp/skyweaver/detail/IncoherentDedispersionPipeline.cu:101-113).
 // Differences by design: every buffer is allocated once at creation, the
 // stages are enqueued back to back on one stream with no intermediate host
 // synchronisation (only when a sink must read a block, and at the end of
 // the gulp), and the scalings are indexed per beamset (beamset*F + f).
 #include "sw_common.hpp"
 
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -50,14 +57,25 @@ struct sw_pipeline {
     // sinks
     sw_sink_fn cb_sink = nullptr, ib_sink = nullptr, st_sink = nullptr;
     void *cb_user = nullptr, *ib_user = nullptr, *st_user = nullptr;
+    // delay-model path
+    sw_delay_model* delays = nullptr;
+    float* d_delays        = nullptr;  // [B][A] float3
+    double* d_freqs        = nullptr;  // [F]
+    std::vector<float> h_delays;
+    double loaded_epoch    = NAN;
+    // incoherent dedispersion of the coherent beams
+    sw_idpipe* idp         = nullptr;
+    std::vector<int> id_ts;
+    std::vector<std::vector<float>> id_dms;
     // state
     sw_obs_header header{};
     bool inited        = false;
     size_t call_count  = 0;
     double tick_per_block = 0;
     hipEvent_t ev[6]{};
-    std::vector<hipEvent_t> ev_dm;  // per DM: ib start, cb start, cb end
-    float stage_ms[6]{};
+    hipEvent_t ev_w[2]{};            // weights generation
+    std::vector<hipEvent_t> ev_dm;  // per DM: ib start, cb start, cb end, id end
+    float stage_ms[SW_NSTAGES]{};
 
     size_t ftpa_bytes() const { return (size_t)cfg.nchans * cfg.nsamples * 2 * cfg.nantennas * 2; }
     int nv() const { return cfg.stokes == SW_STOKES_IQUV ? 4 : 1; }
@@ -67,10 +85,14 @@ struct sw_pipeline {
     {
         if(stream) (void)hipStreamSynchronize(stream);
         if(cd) sw_codedisp_destroy(cd);
-        void* bufs[] = {taftp_d, ftpa, stats, stats_ws, ib_raw, ib_q, cb_out, weights, packed, scl, bmap, antw};
+        if(idp) sw_idpipe_destroy(idp);
+        if(delays) sw_delay_model_close(delays);
+        void* bufs[] = {taftp_d, ftpa, stats, stats_ws, ib_raw, ib_q, cb_out, weights, packed, scl, bmap, antw,
+                        d_delays, d_freqs};
         for(void* b : bufs) (void)hipFree(b);
         for(auto* b : ftpa_dd) (void)hipFree(b);
         for(auto e : ev) if(e) (void)hipEventDestroy(e);
+        for(auto e : ev_w) if(e) (void)hipEventDestroy(e);
         for(auto e : ev_dm) if(e) (void)hipEventDestroy(e);
         if(stream) (void)hipStreamDestroy(stream);
     }
@@ -165,7 +187,8 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
     if(p->cfg.total_nchans <= 0) p->cfg.total_nchans = 4096;
     SW_HIP_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     for(auto& e : p->ev) SW_HIP_TRY(hipEventCreate(&e));
-    p->ev_dm.resize(3 * c.ndms);
+    for(auto& e : p->ev_w) SW_HIP_TRY(hipEventCreate(&e));
+    p->ev_dm.resize(4 * c.ndms);
     for(auto& e : p->ev_dm) SW_HIP_TRY(hipEventCreate(&e));
 
     // coherent dedispersion plan: tsamp = F / bw, band bottom = centre - bw/2
@@ -209,6 +232,38 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
     if(rc) return rc;
     sw_bf_dims d = bf_dims(p.get());
     p->use_mfma = c.cb_form == SW_CB_MFMA && sw::cb_mfma_supported(d);
+
+    if(c.id_enable) {
+        // one incoherent dedispersion block per coherent DM
+        p->id_ts.resize(c.ndms);
+        p->id_dms.resize(c.ndms);
+        std::vector<const float*> dm_ptrs(c.ndms);
+        std::vector<int> nd(c.ndms);
+        for(int i = 0; i < c.ndms; ++i) {
+            p->id_ts[i] = c.id_tscrunch ? c.id_tscrunch[i] : 1;
+            if(c.id_dms && c.id_ndms && c.id_dms[i] && c.id_ndms[i] > 0)
+                p->id_dms[i].assign(c.id_dms[i], c.id_dms[i] + c.id_ndms[i]);
+            else
+                p->id_dms[i].assign(1, p->dms[i]);
+            dm_ptrs[i] = p->id_dms[i].data();
+            nd[i] = (int)p->id_dms[i].size();
+        }
+        sw_idpipe_config ic{};
+        ic.nchans = c.nchans;
+        ic.centre_frequency = c.centre_frequency;
+        ic.bandwidth = c.bandwidth;
+        ic.cb_tscrunch = c.tscrunch;
+        ic.nbeams = c.nbeams;
+        ic.nvec = nv;
+        ic.nblocks = c.ndms;
+        ic.coherent_dms = p->dms.data();
+        ic.tscrunch = p->id_ts.data();
+        ic.dms = dm_ptrs.data();
+        ic.ndms = nd.data();
+        ic.dispatch_size = c.id_dispatch_size;
+        rc = sw_idpipe_create(&ic, &p->idp);
+        if(rc) return rc;
+    }
     SW_HIP_TRY(hipStreamSynchronize(p->stream));
     *out = p.release();
     return SW_OK;
@@ -235,6 +290,45 @@ extern "C" int sw_pipeline_set_sinks(sw_pipeline* p, sw_sink_fn cb, void* cb_use
     return SW_OK;
 }
 
+extern "C" int sw_pipeline_set_tdb_sink(sw_pipeline* p, sw_sink_fn tdb, void* tdb_user)
+{
+    SW_CHECK_ARG(p, "null pipeline");
+    SW_CHECK_ARG(p->idp, "incoherent dedispersion is not enabled in this pipeline");
+    return sw_idpipe_set_sink(p->idp, tdb, tdb_user);
+}
+
+extern "C" int sw_pipeline_set_delay_file(sw_pipeline* p, const char* path)
+{
+    SW_CHECK_ARG(p && path, "null argument");
+    const auto& c = p->cfg;
+    sw_delay_model* m = nullptr;
+    int rc = sw_delay_model_open(path, c.nbeams, c.nantennas, &m);
+    if(rc) return rc;
+    int ns = 0;
+    sw_delay_model_beamsets(m, &ns, nullptr, nullptr, 0);
+    if(ns > 16) {
+        sw_delay_model_close(m);
+        sw::set_error("delay model has more than 16 beamsets");
+        return SW_ERR_UNSUPPORTED;
+    }
+    std::vector<int> map(c.nbeams);
+    std::vector<float> bw((size_t)ns * c.nantennas);
+    sw_delay_model_beamsets(m, &ns, map.data(), bw.data(), ns);
+    if(p->delays) sw_delay_model_close(p->delays);
+    p->delays = m;
+    p->loaded_epoch = NAN;
+    p->h_delays.assign((size_t)c.nbeams * c.nantennas * 3, 0.0f);
+    if(!p->d_delays) SW_HIP_TRY(hipMalloc(&p->d_delays, sizeof(float) * p->h_delays.size()));
+    if(!p->d_freqs) {
+        std::vector<double> f(c.nchans);
+        sw_channel_frequencies(c.centre_frequency, c.bandwidth, c.nchans, f.data());
+        SW_HIP_TRY(hipMalloc(&p->d_freqs, sizeof(double) * c.nchans));
+        SW_HIP_TRY(hipMemcpy(p->d_freqs, f.data(), sizeof(double) * c.nchans, hipMemcpyHostToDevice));
+    }
+    p->weights_set = true;  // generated per gulp
+    return sw_pipeline_set_beamsets(p, ns, map.data(), bw.data());
+}
+
 extern "C" int sw_pipeline_set_weights(sw_pipeline* p, const void* h_weights)
 {
     SW_CHECK_ARG(p && h_weights, "null argument");
@@ -245,6 +339,10 @@ extern "C" int sw_pipeline_set_weights(sw_pipeline* p, const void* h_weights)
     if(rc) return rc;
     SW_HIP_TRY(hipStreamSynchronize(p->stream));
     p->weights_set = true;
+    if(p->delays) {  // static weights replace the delay-model path
+        sw_delay_model_close(p->delays);
+        p->delays = nullptr;
+    }
     return SW_OK;
 }
 
@@ -273,10 +371,10 @@ extern "C" int sw_pipeline_init(sw_pipeline* p, const sw_obs_header* header)
     return SW_OK;
 }
 
-extern "C" int sw_pipeline_stage_times(const sw_pipeline* p, float* ms6)
+extern "C" int sw_pipeline_stage_times(const sw_pipeline* p, float* ms, int n)
 {
-    SW_CHECK_ARG(p && ms6, "null argument");
-    for(int i = 0; i < 6; ++i) ms6[i] = p->stage_ms[i];
+    SW_CHECK_ARG(p && ms && n >= 0, "null argument");
+    for(int i = 0; i < n && i < SW_NSTAGES; ++i) ms[i] = p->stage_ms[i];
     return SW_OK;
 }
 
@@ -301,8 +399,33 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
     }
     const sw_bf_dims d = bf_dims(p);
     const int F = c.nchans, A = c.nantennas, S = p->nbeamsets;
-    const double utc_offset_gulp = (double)p->call_count * p->tick_per_block / p->header.sample_clock;
+    const double utc_offset_gulp =
+        p->header.utc_offset + (double)p->call_count * p->tick_per_block / p->header.sample_clock;
     int rc;
+    SW_HIP_TRY(hipEventRecord(p->ev_w[0], st));
+    if(p->delays) {
+        // DelayManager::delays + WeightsManager::weights at the gulp's unix time
+        // (BeamformerPipeline.cu:292-295: utc_start + utc_offset + n ticks / clock)
+        const long double unix_ts = (long double)p->header.utc_start + (long double)p->header.utc_offset +
+                                    (long double)(p->call_count * (unsigned long long)p->tick_per_block) /
+                                        (long double)p->header.sample_clock;
+        double epoch = 0;
+        rc = sw_delay_model_get(p->delays, (double)unix_ts, p->h_delays.data(), &epoch);
+        if(rc) return rc;
+        if(!(epoch == p->loaded_epoch)) {
+            SW_HIP_TRY(hipMemcpyAsync(p->d_delays, p->h_delays.data(), sizeof(float) * p->h_delays.size(),
+                                      hipMemcpyHostToDevice, st));
+            SW_HIP_TRY(hipStreamSynchronize(st));  // the host copy is refilled on the next model
+            p->loaded_epoch = epoch;
+        }
+        rc = sw_generate_weights(p->d_delays, p->weights, p->d_freqs, A, c.nbeams, F, (double)unix_ts, epoch, st);
+        if(rc) return rc;
+        if(p->use_mfma) {
+            rc = sw_cb_pack_weights(p->weights, p->packed, F, c.nbeams, A, st);
+            if(rc) return rc;
+        }
+    }
+    SW_HIP_TRY(hipEventRecord(p->ev_w[1], st));
     SW_HIP_TRY(hipEventRecord(p->ev[0], st));
     // the gulp is held time-major (FPAT) up to the dedisperser
     rc = sw_transpose_fpat(in, p->ftpa, c.nsamples / 256, nant_in, A, F, st);
@@ -341,10 +464,10 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
         int fl = 0, ov = 0;
         sw_codedisp_info(p->cd, dm, &fl, &ov);
         const double tsamp = c.nchans / c.bandwidth;
-        SW_HIP_TRY(hipEventRecord(p->ev_dm[3 * dm], st));
+        SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm], st));
         rc = sw_ib_beamform(p->ftpa_dd[dm], p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, &d, st);
         if(rc) return rc;
-        SW_HIP_TRY(hipEventRecord(p->ev_dm[3 * dm + 1], st));
+        SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm + 1], st));
         if(p->use_mfma)
             rc = sw_cb_beamform_packed(p->ftpa_dd[dm], p->packed, cb_scaling, cb_offsets, p->bmap,
                                        p->ib_raw, p->cb_out, &d, st);
@@ -352,7 +475,7 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
             rc = sw_cb_beamform(p->ftpa_dd[dm], p->weights, cb_scaling, cb_offsets, p->bmap, p->ib_raw,
                                 p->cb_out, &d, SW_CB_PLAIN, nullptr, st);
         if(rc) return rc;
-        SW_HIP_TRY(hipEventRecord(p->ev_dm[3 * dm + 2], st));
+        SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm + 2], st));
         // the dedispersed block is delayed by the filter delay tsamp*O/2
         const double utc = utc_offset_gulp + tsamp * ov / 2.0;
         sw_block blk;
@@ -370,6 +493,11 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
         blk.utc_offset = utc;
         rc = call_sink(p, p->ib_sink, p->ib_user, blk);
         if(rc) return rc;
+        if(p->idp) {
+            rc = sw_idpipe_push(p->idp, dm, p->cb_out, T2, 1, utc, tsamp * c.tscrunch, st);
+            if(rc) return rc;
+        }
+        SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm + 3], st));
     }
     SW_HIP_TRY(hipEventRecord(p->ev[4], st));
     {
@@ -384,7 +512,7 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
     SW_HIP_TRY(hipStreamSynchronize(st));
     // per-stage device times of this gulp
     float t = 0;
-    for(int i = 0; i < 6; ++i) p->stage_ms[i] = 0;
+    for(int i = 0; i < SW_NSTAGES; ++i) p->stage_ms[i] = 0;
     SW_HIP_TRY(hipEventElapsedTime(&t, p->ev[0], p->ev[1]));
     p->stage_ms[0] = t;
     SW_HIP_TRY(hipEventElapsedTime(&t, p->ev[1], p->ev[2]));
@@ -392,13 +520,20 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
     SW_HIP_TRY(hipEventElapsedTime(&t, p->ev[2], p->ev[3]));
     p->stage_ms[2] = t;
     for(int dm = 0; dm < c.ndms; ++dm) {
-        SW_HIP_TRY(hipEventElapsedTime(&t, p->ev_dm[3 * dm], p->ev_dm[3 * dm + 1]));
+        SW_HIP_TRY(hipEventElapsedTime(&t, p->ev_dm[4 * dm], p->ev_dm[4 * dm + 1]));
         p->stage_ms[3] += t;
-        SW_HIP_TRY(hipEventElapsedTime(&t, p->ev_dm[3 * dm + 1], p->ev_dm[3 * dm + 2]));
+        SW_HIP_TRY(hipEventElapsedTime(&t, p->ev_dm[4 * dm + 1], p->ev_dm[4 * dm + 2]));
         p->stage_ms[4] += t;
+        if(p->idp) {
+            SW_HIP_TRY(hipEventElapsedTime(&t, p->ev_dm[4 * dm + 2], p->ev_dm[4 * dm + 3]));
+            p->stage_ms[5] += t;
+        }
     }
+    SW_HIP_TRY(hipEventElapsedTime(&t, p->ev_w[0], p->ev_w[1]));
+    p->stage_ms[6] = t;
     SW_HIP_TRY(hipEventElapsedTime(&t, p->ev[0], p->ev[4]));
-    p->stage_ms[5] = t - (p->stage_ms[0] + p->stage_ms[1] + p->stage_ms[2] + p->stage_ms[3] + p->stage_ms[4]);
+    p->stage_ms[7] = t - (p->stage_ms[0] + p->stage_ms[1] + p->stage_ms[2] + p->stage_ms[3] + p->stage_ms[4] +
+                          p->stage_ms[5]);
     ++p->call_count;
     return SW_OK;
 }

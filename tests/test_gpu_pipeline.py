@@ -110,3 +110,89 @@ def test_pipeline_rejects_wrong_gulp_size(sw, dev):
     p.init(A)
     with pytest.raises(sw.SkyweaverError, match="Unexpected buffer size"):
         p(np.zeros(100, np.int8))
+
+
+def _read_models(path):
+    hdr = np.dtype([("nbeams", "<u4"), ("nantennas", "<u4"), ("start", "<f8"), ("end", "<f8")])
+    raw, out, off = open(path, "rb").read(), [], 0
+    while off < len(raw):
+        h = np.frombuffer(raw, hdr, 1, off)[0]
+        off += hdr.itemsize
+        n = int(h["nbeams"]) * int(h["nantennas"]) * 3
+        out.append((float(h["start"]), float(h["end"]),
+                    np.frombuffer(raw, "<f4", n, off).reshape(h["nbeams"], h["nantennas"], 3)))
+        off += 4 * n
+    return out
+
+
+def test_pipeline_delay_file(sw, orc, dev, golden_dir):
+    """Weights and beamsets from the reference's delay-model fixture: per gulp
+    the model valid at utc_start + n * 2 * total_nchans * T / sample_clock is
+    turned into weights on the device; the beams equal the oracle chain fed
+    with the oracle's weights for that time (DM 0: bit-exact)."""
+    import os
+    from skyweaver_amd import DeviceBuffer as DB
+    Bd = 72                      # >= the 67 beams of the fixture
+    H = T // 256
+    utc_start, clock, total = 1708082300.0, 1712e6, 4096
+    p = sw.BeamformerPipeline(A, F, Bd, T, TS, [0.0], CFREQ, BW, fft_len=[1024], total_nchans=total)
+    p.set_delay_file(os.path.join(golden_dir, "test_delays.bin"))
+    p.init(57, utc_start=utc_start, sample_clock=clock)
+    got, stats = [], []
+    p.set_sinks(cb=lambda blk: got.append(sw.download_block(blk)),
+                stats=lambda blk: stats.append(sw.download_block(blk).view(np.float32).reshape(F, 2, A, 4)))
+    gulps = [sw.synth_voltages((H, 57, F, 256, 2, 2), seed=0x5EED0009 + i, sigma=16.0) for i in range(2)]
+    for g in gulps:
+        p(DB.from_host(g))
+    p.close()
+    models = _read_models(os.path.join(golden_dir, "test_delays.bin"))
+    freqs = sw.channel_frequencies(CFREQ, BW, F)
+    nsets, bmap, bw = orc.parse_beamsets(models[0][2], Bd, A)
+    assert nsets == 2
+    scal = None
+    for gi, g in enumerate(gulps):
+        t_unix = utc_start + gi * 2.0 * total * T / clock
+        k = next(i for i, m in enumerate(models) if m[0] <= t_unix <= m[1])
+        delays = np.zeros((Bd, A, 3), np.float32)
+        delays[:67, :57] = models[k][2]
+        w = orc.weights(delays, freqs, t_unix, models[k][0])
+        ftpa = orc.transpose(g, A)
+        if scal is None:  # from the GPU statistics of the first gulp, as the pipeline does
+            scal = orc.update_scalings(stats[0][..., 1], bw, nsets, TS, TS, 24.0)
+        raw, _ = orc.ib_beamform(ftpa, TS, scal["ib_scaling"], scal["ib_offsets"], bw, nsets, "I")
+        cb = orc.cb_beamform(ftpa, w, TS, scal["cb_scaling"], scal["cb_offsets"], bw, bmap, nsets, "I", True)
+        g_cb = got[gi].view(np.int8).reshape(cb.shape)
+        np.testing.assert_array_equal(g_cb, cb)
+
+
+def test_pipeline_incoherent_dedispersion(sw, orc, dev):
+    """The coherent beams of every gulp feed the incoherent dedisperser; each
+    TDB block equals the oracle's dedispersion of the concatenated TFB stream."""
+    from skyweaver_amd import DeviceBuffer as DB
+    H = T // 256
+    weights = sw.synth_weights((F, B, A, 2), seed=0x5EED100A)
+    idms = np.arange(0, 40, 2.5, dtype=np.float32)
+    p = sw.BeamformerPipeline(A, F, B, T, TS, [0.0], CFREQ, BW, fft_len=[1024], id_plan=[(2, idms)],
+                              id_dispatch_size=300)
+    p.set_weights(weights)
+    p.init(A)
+    tfb, tdb = [], []
+    p.set_sinks(cb=lambda blk: tfb.append(sw.download_block(blk).view(np.int8).reshape(blk.dims[0], blk.dims[1],
+                                                                                         blk.dims[2])))
+    p.set_tdb_sink(lambda blk: tdb.append((sw.download_block(blk).view(np.int8).reshape(
+        blk.dims[0], blk.dims[1], blk.dims[2]), blk.utc_offset, blk.tsamp)))
+    for i in range(3):
+        p(DB.from_host(sw.synth_voltages((H, A, F, 256, 2, 2), seed=0x5EED0100 + i, sigma=16.0)))
+    times = p.stage_times()
+    p.close()
+    stream = np.concatenate(tfb)                       # 3 x 256 TFB samples
+    tsamp_cb = TSAMP * TS
+    d_o, mx = orc.id_delays(idms, sw.channel_frequencies(CFREQ, BW, F), tsamp_cb)
+    assert mx > 0
+    nblk = (len(stream) - mx) // 300
+    assert len(tdb) == nblk >= 1
+    for k, (a, utc, tsb) in enumerate(tdb):
+        np.testing.assert_array_equal(a, orc.incoherent_dedisperse(stream[k * 300:k * 300 + 300 + mx], d_o, mx, 2))
+        assert utc == pytest.approx((k * 300 + mx) * tsamp_cb, rel=1e-9)
+        assert tsb == pytest.approx(tsamp_cb * 2, rel=1e-12)
+    assert times["incoherent_dedispersion"] >= 0.0

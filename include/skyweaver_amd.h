@@ -227,7 +227,7 @@ int sw_idisp_info(const sw_idisp* plan, int* max_delay, int* h_delays, float* sc
 /* d_tfb: [nsamples][nchans][nbeams * nvec] int8 (nvec = 4: char4 IQUV);
  * d_tdb: [(nsamples - max_delay) / tscrunch][ndms][nbeams * nvec] int8.
  * nsamples must exceed max_delay and (nsamples - max_delay) be a multiple
- * of tscrunch (the reference's errors); nbeams * nvec % 16 == 0. */
+ * of tscrunch (the reference's errors); nbeams * nvec % 4 == 0. */
 int sw_idisp_dedisperse(sw_idisp* plan, const void* d_tfb, size_t nsamples, int nbeams,
                         int nvec, void* d_tdb, void* stream);
 

@@ -65,10 +65,23 @@ inline std::string last_error()
     return buf;
 }
 
+// Message of an exception a handler threw inside a library callback; the
+// callback returns an error code through the C ABI and check() rethrows it.
+inline std::string& handler_error()
+{
+    static thread_local std::string msg;
+    return msg;
+}
+
 // Convert a C ABI status into the reference's exception types.
 inline void check(int rc, double epoch = 0.0)
 {
     if(rc == SW_OK) return;
+    if(!handler_error().empty()) {
+        std::string m = "handler failed: " + handler_error();
+        handler_error().clear();
+        throw std::runtime_error(m);
+    }
     if(rc == SW_ERR_DELAY_EPOCH) throw InvalidDelayEpoch(epoch);
     throw std::runtime_error(last_error());
 }
@@ -293,7 +306,11 @@ int cb_trampoline(void* user, sw_block const* b)
     try {
         (*static_cast<H*>(user))(BlockView(*b), static_cast<std::size_t>(b->dm_idx));
         return 0;
+    } catch(std::exception const& e) {
+        handler_error() = e.what();
+        return 1;
     } catch(...) {
+        handler_error() = "unknown exception";
         return 1;
     }
 }
@@ -303,7 +320,11 @@ int stats_trampoline(void* user, sw_block const* b)
     try {
         (*static_cast<H*>(user))(BlockView(*b));
         return 0;
+    } catch(std::exception const& e) {
+        handler_error() = e.what();
+        return 1;
     } catch(...) {
+        handler_error() = "unknown exception";
         return 1;
     }
 }

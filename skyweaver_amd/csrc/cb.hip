@@ -23,6 +23,8 @@
 //    weight update into [F][B][2][2A] rows ([wr | ~wi], [wi | wr]).
 #include "sw_common.hpp"
 
+#include <type_traits>
+
 namespace sw {
 namespace {
 
@@ -84,6 +86,35 @@ struct Epi {
                 r[3] = ib_subtract(power[3], ib.w, sc);
             } else {
                 const float off = offset[so];
+                r[0] = rescale<SW_STOKES_I>(power[0], off, sc);
+                r[1] = rescale<SW_STOKES_Q>(power[1], off, sc);
+                r[2] = rescale<SW_STOKES_U>(power[2], off, sc);
+                r[3] = rescale<SW_STOKES_V>(power[3], off, sc);
+            }
+            char4 q;
+            q.x = clamp_i8(r[0]);
+            q.y = clamp_i8(r[1]);
+            q.z = clamp_i8(r[2]);
+            q.w = clamp_i8(r[3]);
+            reinterpret_cast<char4*>(out)[oi] = q;
+        }
+    }
+    // As store(), with the beamset's scale / offset and the incoherent-beam
+    // powers already loaded.
+    __device__ __forceinline__ static void store_pre(const float* power, int8_t* out, const float* ib,
+                                                     float sc, float off, int tout, int f, int b,
+                                                     int nchans, int nbeams, int ib_sub)
+    {
+        const size_t oi = ((size_t)tout * nchans + f) * nbeams + b;
+        if constexpr(NV == 1) {
+            const float r = ib_sub ? ib_subtract(power[0], ib[0], sc) : rescale<S>(power[0], off, sc);
+            out[oi] = clamp_i8(r);
+        } else {
+            float r[4];
+            if(ib_sub) {
+#pragma unroll
+                for(int k = 0; k < 4; ++k) r[k] = ib_subtract(power[k], ib[k], sc);
+            } else {
                 r[0] = rescale<SW_STOKES_I>(power[0], off, sc);
                 r[1] = rescale<SW_STOKES_Q>(power[1], off, sc);
                 r[2] = rescale<SW_STOKES_U>(power[2], off, sc);
@@ -205,6 +236,18 @@ __global__ void cb_pack_kernel(const uint8_t* __restrict__ w, uint8_t* __restric
     o[3 * qa + g] = r;
 }
 
+// Per-output epilogue parameters, loaded before the beam tile's MFMAs so
+// that their latency hides behind them: beamset scale / offset of the lane's
+// beam and (for few outputs per lane) the incoherent-beam powers.
+template <int S, int NOUT>
+struct EpiParams {
+    static constexpr int NV = nstokes(S);
+    static constexpr bool kPreIb = NOUT * NV <= 4;
+    float sc = 0, off = 0;
+    float ib[kPreIb ? NOUT : 1][NV];
+    int s = 0;
+};
+
 template <int S, int TS, int NSTEPS>
 __global__ void __launch_bounds__(kMfmaWaves * 64)
 cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ packed,
@@ -216,6 +259,13 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     constexpr int NV = nstokes(S);
     constexpr int A  = NSTEPS * 16;  // antennas (K' = 2A = 32 * NSTEPS)
     constexpr int KP = 2 * A;
+    // weight tile: 32 beams x 2 rows x KP bytes, rows padded by 16 bytes so
+    // that the 32 lanes of a half-wave read distinct banks
+    constexpr int WPITCH = 2 * KP + 16;          // bytes per beam in LDS
+    constexpr int WTILE  = 32 * 2 * KP / 16;     // uint4 per tile in global memory
+    constexpr int WPER   = (WTILE + kMfmaWaves * 64 - 1) / (kMfmaWaves * 64);
+    constexpr int NOUT   = TS <= 16 ? 16 / TS : 1;  // outputs per lane per tile
+    __shared__ uint4 wlds[2][32 * WPITCH / 16];
     // XCD-aware mapping: consecutive workgroups go round-robin over the 8
     // XCDs; keep all time chunks of a channel on one XCD so its packed
     // weights stay resident in that XCD's L2.
@@ -231,18 +281,27 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
             tc = bid % ntchunks;
         }
     }
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int tid  = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     const int t0   = (tc * kMfmaWaves + wave) * 32;  // first time row of this wave
-    if(t0 >= nsamples) return;
+    // waves past the end of the gulp still take part in the weight staging
+    // and its barriers, but compute on row 0 and store nothing
+    const bool active = t0 < nsamples;
+    const int tl   = active ? t0 : 0;
     const int row = lane & 31;  // A-operand row / output column index
     const int grp = lane >> 5;  // k group (16 bytes)
+    // A-operand row i = 8a + 4g + b carries time sample 16g + 4a + b, so that
+    // accumulator register r of a lane in half g holds time 16g + r: each
+    // lane sees 16 consecutive samples of its beam and the time scrunch needs
+    // no exchange between lanes.
+    const int trow = 16 * ((row >> 2) & 1) + 4 * (row >> 3) + (row & 3);
 
     // ---- A operands: 32 time rows x K' for both polarisations
     v4i va[2][NSTEPS];
 #pragma unroll
     for(int p = 0; p < 2; ++p) {
-        const uint8_t* vrow = ftpa + (((size_t)f * nsamples + t0 + row) * 2 + p) * A * 2;
+        const uint8_t* vrow = ftpa + (((size_t)f * nsamples + tl + trow) * 2 + p) * A * 2;
 #pragma unroll
         for(int s = 0; s < NSTEPS; ++s) {
             const int k  = 32 * s + 16 * grp;
@@ -278,43 +337,83 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
 
     const int nsamps_out = nsamples / TS;
     const int nbt = (nbeams + 31) / 32;
+    const uint4* wsrc = reinterpret_cast<const uint4*>(packed + (size_t)f * nbeams * 2 * KP);
+    const size_t wtotal = (size_t)nbeams * 2 * KP / 16;  // uint4 of this channel
+    // global -> registers for tile bt (beams past the end read as zero)
+    auto fetch = [&](int bt, uint4* r) {
+#pragma unroll
+        for(int i = 0; i < WPER; ++i) {
+            const int idx = tid + i * kMfmaWaves * 64;
+            const size_t g = (size_t)bt * WTILE + idx;
+            r[i] = (idx < WTILE && g < wtotal) ? wsrc[g] : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto stash = [&](int buf, const uint4* r) {
+#pragma unroll
+        for(int i = 0; i < WPER; ++i) {
+            const int idx = tid + i * kMfmaWaves * 64;
+            if(idx < WTILE) {
+                const int beam = idx / (2 * KP / 16), c16 = idx % (2 * KP / 16);
+                wlds[buf][beam * (WPITCH / 16) + c16] = r[i];
+            }
+        }
+    };
+    uint4 wreg[WPER];
+    fetch(0, wreg);
+    stash(0, wreg);
+    __syncthreads();
+
     for(int bt = 0; bt < nbt; ++bt) {
+        const int buf = bt & 1;
+        if(bt + 1 < nbt) fetch(bt + 1, wreg);  // in flight during this tile
         const int b = bt * 32 + row;
         const bool bvalid = b < nbeams;
-        const uint8_t* wrow = packed + ((size_t)f * nbeams + (bvalid ? b : 0)) * 2 * KP + 16 * grp;
+        // epilogue parameters of this lane's beam
+        EpiParams<S, NOUT> ep;
+        if(active && bvalid) {
+            ep.s = bmap[b];
+            const int so = ep.s * nchans + f;
+            ep.sc = scale[so];
+            if(!ib_sub) ep.off = offset[so];
+            if constexpr(EpiParams<S, NOUT>::kPreIb) {
+                if(ib_sub) {
+#pragma unroll
+                    for(int o = 0; o < NOUT; ++o) {
+                        const int tout = TS <= 16 ? (t0 + 16 * grp + o * TS) / TS : t0 / 32;
+                        const size_t ii = ((size_t)ep.s * nsamps_out + tout) * nchans + f;
+                        if constexpr(NV == 1) {
+                            ep.ib[o][0] = ib_raw[ii];
+                        } else {
+                            const float4 v = reinterpret_cast<const float4*>(ib_raw)[ii];
+                            ep.ib[o][0] = v.x;
+                            ep.ib[o][1] = v.y;
+                            ep.ib[o][2] = v.z;
+                            ep.ib[o][3] = v.w;
+                        }
+                    }
+                }
+            }
+        }
+        const uint8_t* wl = reinterpret_cast<const uint8_t*>(wlds[buf]) + row * WPITCH + 16 * grp;
         v16i acc_re[2] = {corr[0], corr[1]};
         v16i acc_im[2] = {v16i{0}, v16i{0}};
 #pragma unroll
         for(int s = 0; s < NSTEPS; ++s) {
-            v4i wre = *reinterpret_cast<const v4i*>(wrow + 32 * s);
-            v4i wim = *reinterpret_cast<const v4i*>(wrow + KP + 32 * s);
-            if(!bvalid) { wre = v4i{0, 0, 0, 0}; wim = v4i{0, 0, 0, 0}; }
+            const v4i wre = *reinterpret_cast<const v4i*>(wl + 32 * s);
+            const v4i wim = *reinterpret_cast<const v4i*>(wl + KP + 32 * s);
 #pragma unroll
             for(int p = 0; p < 2; ++p) {
                 acc_re[p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(va[p][s], wre, acc_re[p], 0, 0, 0);
                 acc_im[p] = __builtin_amdgcn_mfma_i32_32x32x32_i8(va[p][s], wim, acc_im[p], 0, 0, 0);
             }
         }
-        // ---- detect: register r of this lane holds time row
-        //      8*(r/4) + 4*grp + (r%4) of column (beam) `row`
+        // ---- detect: register r of this lane holds time sample 16*grp + r
+        //      of beam `b`
         float st[16][NV];
 #pragma unroll
         for(int r = 0; r < 16; ++r)
             Epi<S>::detect((float)acc_re[0][r], (float)acc_im[0][r], (float)acc_re[1][r],
                            (float)acc_im[1][r], st[r]);
-        // ---- exchange halves so each lane holds 16 consecutive rows:
-        //      lanes 0-31 rows 0-15, lanes 32-63 rows 16-31, in the order
-        //      r = 0-3, 8-11, 4-7, 12-15.
-#pragma unroll
-        for(int r = 0; r < 8; ++r) {
-#pragma unroll
-            for(int k = 0; k < NV; ++k) {
-                const float send = grp ? st[r][k] : st[8 + r][k];
-                const float recv = __shfl_xor(send, 32);
-                if(grp) st[r][k] = recv; else st[8 + r][k] = recv;
-            }
-        }
-        constexpr int seq[16] = {0, 1, 2, 3, 8, 9, 10, 11, 4, 5, 6, 7, 12, 13, 14, 15};
         if constexpr(TS <= 16) {
 #pragma unroll
             for(int o = 0; o < 16 / TS; ++o) {
@@ -324,11 +423,16 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
 #pragma unroll
                 for(int q = 0; q < TS; ++q)
 #pragma unroll
-                    for(int k = 0; k < NV; ++k) power[k] += st[seq[o * TS + q]][k];
-                if(bvalid)
-                    Epi<S>::store(power, out, ib_raw, scale, offset, bmap,
-                                  (t0 + 16 * grp + o * TS) / TS, f, b, nchans, nbeams,
-                                  nsamps_out, ib_sub);
+                    for(int k = 0; k < NV; ++k) power[k] += st[o * TS + q][k];
+                if(active && bvalid) {
+                    const int tout = (t0 + 16 * grp + o * TS) / TS;
+                    if constexpr(EpiParams<S, NOUT>::kPreIb)
+                        Epi<S>::store_pre(power, out, ep.ib[o], ep.sc, ep.off, tout, f, b, nchans, nbeams,
+                                          ib_sub);
+                    else
+                        Epi<S>::store(power, out, ib_raw, scale, offset, bmap, tout, f, b, nchans, nbeams,
+                                      nsamps_out, ib_sub);
+                }
             }
         } else {  // TS == 32: the upper half continues the lower half's sum
             float power[NV];
@@ -337,7 +441,7 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
 #pragma unroll
             for(int q = 0; q < 16; ++q)
 #pragma unroll
-                for(int k = 0; k < NV; ++k) power[k] += st[seq[q]][k];
+                for(int k = 0; k < NV; ++k) power[k] += st[q][k];
             float carried[NV];
 #pragma unroll
             for(int k = 0; k < NV; ++k) carried[k] = __shfl_xor(power[k], 32);
@@ -347,12 +451,14 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
 #pragma unroll
                 for(int q = 0; q < 16; ++q)
 #pragma unroll
-                    for(int k = 0; k < NV; ++k) power[k] += st[seq[q]][k];
-                if(bvalid)
-                    Epi<S>::store(power, out, ib_raw, scale, offset, bmap, t0 / 32, f, b,
-                                  nchans, nbeams, nsamps_out, ib_sub);
+                    for(int k = 0; k < NV; ++k) power[k] += st[q][k];
+                if(active && bvalid)
+                    Epi<S>::store_pre(power, out, ep.ib[0], ep.sc, ep.off, t0 / 32, f, b, nchans, nbeams,
+                                      ib_sub);
             }
         }
+        if(bt + 1 < nbt) stash(buf ^ 1, wreg);
+        __syncthreads();
     }
 }
 

@@ -103,18 +103,29 @@ idisp_kernel(const uint32_t* __restrict__ tfb, uint32_t* __restrict__ tdb,
         const long lo   = in0 - sp.y;
         const int nrows = TT * ts + sp.y - sp.x;
         __syncthreads();
-        for(int i = threadIdx.x; i < nrows * 16; i += 256) {
-            const int r = i >> 4, q = i & 15;
-            const int w4 = chunk * 64 + q * 4;
-            uint4 v = make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
-            if(w4 < nwords && lo + r < nsamples) {
-                v = *reinterpret_cast<const uint4*>(tfb + ((size_t)(lo + r) * nchans + f) * nwords + w4);
-                v.x ^= 0x80808080u;
-                v.y ^= 0x80808080u;
-                v.z ^= 0x80808080u;
-                v.w ^= 0x80808080u;
+        if((nwords & 3) == 0) {  // rows are 16-byte aligned
+            for(int i = threadIdx.x; i < nrows * 16; i += 256) {
+                const int r = i >> 4, q = i & 15;
+                const int w4 = chunk * 64 + q * 4;
+                uint4 v = make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
+                if(w4 < nwords && lo + r < nsamples) {
+                    v = *reinterpret_cast<const uint4*>(tfb + ((size_t)(lo + r) * nchans + f) * nwords + w4);
+                    v.x ^= 0x80808080u;
+                    v.y ^= 0x80808080u;
+                    v.z ^= 0x80808080u;
+                    v.w ^= 0x80808080u;
+                }
+                stage[r * 16 + q] = v;
             }
-            stage[r * 16 + q] = v;
+        } else {
+            uint32_t* st32 = reinterpret_cast<uint32_t*>(stage);
+            for(int i = threadIdx.x; i < nrows * 64; i += 256) {
+                const int r = i >> 6, q = i & 63;
+                const int w = chunk * 64 + q;
+                uint32_t v = 0x80808080u;
+                if(w < nwords && lo + r < nsamples) v = tfb[((size_t)(lo + r) * nchans + f) * nwords + w] ^ 0x80808080u;
+                st32[i] = v;
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -379,7 +390,7 @@ extern "C" int sw_idisp_dedisperse(sw_idisp* p, const void* d_tfb, size_t nsampl
         return SW_ERR_INVALID;
     }
     const long nbv = (long)nbeams * nvec;
-    SW_CHECK_ARG(nbv % 16 == 0, "nbeams * nvec must be a multiple of 16");
+    SW_CHECK_ARG(nbv % 4 == 0, "nbeams * nvec must be a multiple of 4");
     SW_CHECK_ARG(((uintptr_t)d_tfb & 15) == 0 && ((uintptr_t)d_tdb & 3) == 0, "misaligned buffer");
     const int nwords = (int)(nbv / 4);
     const int nout = (int)((nsamples - p->max_delay) / p->tscrunch);

@@ -51,6 +51,8 @@ CASES = [
     (128, 32, 1, 4, np.arange(0, 100, 2.5, dtype=np.float32), 20, (1284e6, 26.75e6, 16)),   # 512 adds/sum
     (300, 16, 1, 1, np.arange(0, 50, 1.0, dtype=np.float32), 19, (1284e6, 62.6953125e6, 8)),
     (256, 32, 1, 1, np.arange(1024, dtype=np.float32) * np.float32(0.9765625), 24, (1284e6, 53.5e6, 8)),
+    (64, 72, 1, 2, np.arange(0, 40, 2.5, dtype=np.float32), 37, (1284e6, 13.375e6, 16)),      # 18-word rows
+    (32, 300, 1, 1, np.arange(0, 80, 4.0, dtype=np.float32), 21, (1284e6, 6.6875e6, 16)),     # 75 words, 2 chunks
 ]
 
 
@@ -93,7 +95,7 @@ def test_errors(sw, dev):
     with pytest.raises(sw.SkyweaverError, match="multiple of tscrunch"):
         idd.dedisperse(buf, idd.max_delay + 6, 64, 1, out)
     with pytest.raises(sw.SkyweaverError):
-        idd.dedisperse(buf, idd.max_delay + 8, 12, 1, out)  # 12 beams: not a multiple of 16 bytes
+        idd.dedisperse(buf, idd.max_delay + 8, 6, 1, out)  # 6 beams: rows not whole 32-bit words
 
 
 def test_pipeline_aggregation(sw, orc, dev):

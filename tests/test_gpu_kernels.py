@@ -187,6 +187,24 @@ def test_cb_parity_shapes(sw, orc, dev, form, A, B, ts, ib_sub):
 
 
 @pytest.mark.parametrize("form", ["plain", "mfma"])
+@pytest.mark.parametrize("stokes,ts,ib_sub", [("I", 1, True), ("IQUV", 1, True), ("IQUV", 2, False),
+                                               ("I", 32, True), ("IQUV", 32, True)])
+def test_cb_parity_partial_chunk(sw, orc, dev, form, stokes, ts, ib_sub):
+    # T = 160 is not a multiple of the MFMA form's 128-sample workgroup chunk:
+    # the last workgroup has idle waves that still take part in weight staging
+    print(f"cb partial chunk form={form} stokes={stokes} ts={ts} ib_sub={ib_sub}", flush=True)
+    F, T, A, B, S = 2, 160, 48, 50, 2
+    v = _rand_c8((F, T, 2, A), 500 + ts)
+    w = _phase_weights((F, B, A), 501 + ts)
+    bmap = (np.arange(B) % S).astype(np.int32)
+    antw = np.ones((S, A), np.float32)
+    antw[1, ::3] = 0
+    out, (cbs, cbo) = _run_cb(sw, dev, v, w, bmap, antw, ts, stokes, ib_sub, form)
+    exp = orc.cb_beamform(v, w, ts, cbs, cbo, antw, bmap, S, stokes, ib_sub)
+    np.testing.assert_array_equal(out, exp)
+
+
+@pytest.mark.parametrize("form", ["plain", "mfma"])
 def test_cb_parity_extreme_int8(sw, orc, dev, form):
     # -128 in weights and voltages (the MFMA form's ~x + 1 negation path) and
     # full-scale values whose powers round in float
@@ -214,3 +232,13 @@ def test_cb_plain_equals_mfma_at_scale(sw, dev):
     o2, _ = _run_cb(sw, dev, v, w, bmap, antw, ts, "I", True, "mfma")
     np.testing.assert_array_equal(o1, o2)
     assert np.unique(o1).size > 50  # non-degenerate output
+
+
+@pytest.mark.parametrize("F,R,C", [(2, 128, 4096), (3, 4096, 128), (2, 130, 66), (1, 7, 9), (2, 64, 68)])
+def test_corner_turn(sw, dev, F, R, C):
+    # [F][R][C] -> [F][C][R] of 16-bit (complex int8) elements, both kernel paths
+    x = _rng(F * R + C).integers(-128, 128, (F, R, C, 2)).astype(np.int8)
+    d_x = _to_dev(x, dev)
+    d_y = _empty((F, C, R, 2), np.int8, dev)
+    sw.corner_turn(d_x, d_y, F, R, C)
+    np.testing.assert_array_equal(_to_host(d_y), x.transpose(0, 2, 1, 3))

@@ -138,18 +138,19 @@ def bench_idisp(sw, steps):
         sw.check(sw.lib().sw_memcpy_dtod(d_in.ptr + off, d_tile.ptr, n, None))
     d_out = DeviceBuffer(nout * len(ID_DMS) * ID_B)
     st = sw.Stream()
-    idd.dedisperse(d_in, nsamp, ID_B, 1, d_out, stream=st)   # warm-up
+    # channel-major input, as held by the aggregation ring
+    idd.dedisperse(d_in, nsamp, ID_B, 1, d_out, stream=st, layout="ftb")   # warm-up
     e0, e1 = sw.Event(), sw.Event()
     e0.record(st)
     for _ in range(steps):
-        idd.dedisperse(d_in, nsamp, ID_B, 1, d_out, stream=st)
+        idd.dedisperse(d_in, nsamp, ID_B, 1, d_out, stream=st, layout="ftb")
     e1.record(st)
     st.synchronize()
     ms = e0.elapsed_ms(e1) / steps
     adds = float(nout) * len(ID_DMS) * ID_B * ID_F
     nbytes = d_in.nbytes + d_out.nbytes
     tsamp_cb = ID_CBTS * ID_F / ID_BW
-    return {"kernel": "idisp_kernel", "ms_per_dispatch": round(ms, 3), "steps": steps,
+    return {"kernel": "idisp_kernel", "layout": "FTB (channel-major aggregation ring)", "ms_per_dispatch": round(ms, 3), "steps": steps,
             "dispatch_samples": disp, "max_delay": idd.max_delay, "ndms": len(ID_DMS), "nchans": ID_F,
             "nbeams": ID_B, "adds_per_dispatch": adds,
             "adds_per_s": adds / (ms * 1e-3), "valu_add16_peak": VALU_ADD16_PEAK,

@@ -230,17 +230,25 @@ int sw_idisp_info(const sw_idisp* plan, int* max_delay, int* h_delays, float* sc
  * of tscrunch (the reference's errors); nbeams * nvec % 4 == 0. */
 int sw_idisp_dedisperse(sw_idisp* plan, const void* d_tfb, size_t nsamples, int nbeams,
                         int nvec, void* d_tdb, void* stream);
+/* The same on channel-major input d_ftb: [nchans][nsamples][nbeams * nvec]
+ * int8 (what the aggregation ring holds; a channel's rows are contiguous). */
+int sw_idisp_dedisperse_ftb(sw_idisp* plan, const void* d_ftb, size_t nsamples, int nbeams,
+                            int nvec, void* d_tdb, void* stream);
 
 /* ------------------------------------------------------ CoherentDedisperser
  * create_coherent_dedisperser_config + CoherentDedisperser::dedisperse,
  * cpp/skyweaver/CoherentDedisperser.cuh:55-92, cpp/skyweaver/src/CoherentDedisperser.cu:19-353.
  * A plan holds, per coherent DM, the FFT length N, the overlap O (even),
- * the FP64-generated chirp tables and the overlap-save history (the previous
- * gulp's last O samples of every channel; zeros before the first gulp).
- * Output sample j of a gulp is the dedispersed input sample j - O/2 (the
- * reference's filter delay tsamp*O/2).  fft_len / overlap entries <= 0 (or
- * NULL arrays) select O = even-ceil of the intra-channel smear at the
- * bottom of the band and N = the smallest power of two >= max(1024, 16 O). */
+ * the FP64-generated chirp tables and the overlap-save history (zeros
+ * before the first gulp).  The overlap-save segments (hop N - O) lie on one
+ * grid over the whole stream of gulps, so no transform is wasted on a
+ * partial segment at a gulp's end.  Output sample j of the stream is the
+ * dedispersed input sample j - O/2 (the reference's filter delay
+ * tsamp*O/2); a gulp emits stream samples [gT - L, (g+1)T - L), where the
+ * latency L = 0 when the hop divides the gulp length T and L = N - O
+ * otherwise (sw_codedisp_info).  fft_len / overlap entries <= 0 (or NULL
+ * arrays) select O = even-ceil of the intra-channel smear at the bottom of
+ * the band and N = the smallest power of two >= max(1024, 16 O). */
 typedef struct sw_codedisp_config {
     int nchans;        /* F */
     int nantennas;     /* A (FTPA padded antenna count) */
@@ -258,19 +266,24 @@ typedef struct sw_codedisp sw_codedisp;
 
 int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, sw_codedisp** plan);
 int sw_codedisp_destroy(sw_codedisp* plan);
-int sw_codedisp_info(const sw_codedisp* plan, int dm_idx, int* fft_len, int* overlap);
+int sw_codedisp_info(const sw_codedisp* plan, int dm_idx, int* fft_len, int* overlap,
+                     int* latency);
 /* Dedisperse the gulp d_ftpa (FTPA [F][T][2][A]) for the DMs
  * [dm_begin, dm_begin + dm_count): d_out[d] receives FTPA int8 for DM d;
  * d_out_f32 (NULL or per-DM NULL allowed) receives the unquantised y/N as
  * float (re, im) pairs.  batched = 1 shares each forward FFT between up to 8
  * DMs of equal (N, O) in one launch; batched = 0 launches one DM at a time.
- * Both give identical results.  The history is not advanced. */
+ * Both give identical results.  Every DM of a gulp is run before
+ * sw_codedisp_push_history, which ends the gulp. */
 int sw_codedisp_run(sw_codedisp* plan, const void* d_ftpa, void* const* d_out,
                     void* const* d_out_f32, int dm_begin, int dm_count, int batched,
                     void* stream);
-/* Advance the overlap-save history to the end of the gulp d_ftpa. */
+/* End the gulp d_ftpa: advance the overlap-save history and the segment
+ * grid, and carry outputs computed ahead of this gulp's window. */
 int sw_codedisp_push_history(sw_codedisp* plan, const void* d_ftpa, void* stream);
-/* Replace the history ([F][Omax][2][A] complex int8, Omax = largest overlap). */
+/* Replace the history ([F][H][2][A] complex int8; H = the largest overlap
+ * when every DM's hop divides the gulp length, else the largest FFT length
+ * among DMs with latency). */
 int sw_codedisp_set_history(sw_codedisp* plan, const void* d_hist, void* stream);
 
 /* ----------------------------------------------------- time-major layout

@@ -97,7 +97,7 @@ def _declare(L):
     _pvp = ctypes.POINTER(ctypes.c_void_p)
     L.sw_codedisp_create.argtypes = [ctypes.POINTER(CodedispConfig), _vp, _pvp]
     L.sw_codedisp_destroy.argtypes = [_vp]
-    L.sw_codedisp_info.argtypes = [_vp, _c, ctypes.POINTER(_c), ctypes.POINTER(_c)]
+    L.sw_codedisp_info.argtypes = [_vp, _c, ctypes.POINTER(_c), ctypes.POINTER(_c), ctypes.POINTER(_c)]
     L.sw_codedisp_run.argtypes = [_vp, _vp, _pvp, _pvp, _c, _c, _c, _vp]
     L.sw_codedisp_push_history.argtypes = [_vp, _vp, _vp]
     L.sw_codedisp_set_history.argtypes = [_vp, _vp, _vp]
@@ -131,6 +131,7 @@ def _declare(L):
     L.sw_idisp_destroy.argtypes = [_vp]
     L.sw_idisp_info.argtypes = [_vp, ctypes.POINTER(_c), _vp, ctypes.POINTER(ctypes.c_float)]
     L.sw_idisp_dedisperse.argtypes = [_vp, _vp, _sz, _c, _c, _vp, _vp]
+    L.sw_idisp_dedisperse_ftb.argtypes = [_vp, _vp, _sz, _c, _c, _vp, _vp]
 
 
 def last_error() -> str:
@@ -421,8 +422,10 @@ class IncoherentDedisperser:
     def nout(self, nsamples):
         return (nsamples - self.max_delay) // self.tscrunch
 
-    def dedisperse(self, tfb, nsamples, nbeams, nvec, tdb, stream=None):
-        check(lib().sw_idisp_dedisperse(self._h, _ptr(tfb), nsamples, nbeams, nvec, _ptr(tdb), _stream(stream)))
+    def dedisperse(self, tfb, nsamples, nbeams, nvec, tdb, stream=None, layout="tfb"):
+        """layout "tfb": [T][F][B]; "ftb": channel-major [F][T][B]."""
+        fn = lib().sw_idisp_dedisperse if layout == "tfb" else lib().sw_idisp_dedisperse_ftb
+        check(fn(self._h, _ptr(tfb), nsamples, nbeams, nvec, _ptr(tdb), _stream(stream)))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -495,9 +498,16 @@ class CoherentDedisperser:
         self.ndms = n
 
     def info(self, d):
-        N, O = ctypes.c_int(), ctypes.c_int()
-        check(lib().sw_codedisp_info(self.handle, d, ctypes.byref(N), ctypes.byref(O)))
+        """(FFT length, overlap) of DM d."""
+        N, O, L = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().sw_codedisp_info(self.handle, d, ctypes.byref(N), ctypes.byref(O), ctypes.byref(L)))
         return N.value, O.value
+
+    def latency(self, d):
+        """Output latency of DM d in samples: a gulp g emits stream samples [gT - L, (g+1)T - L)."""
+        L = ctypes.c_int()
+        check(lib().sw_codedisp_info(self.handle, d, None, None, ctypes.byref(L)))
+        return L.value
 
     def run(self, ftpa, outs, outs_f32=None, dm_begin=0, dm_count=None, batched=True, stream=None):
         n = self.ndms

@@ -33,7 +33,8 @@
 #include <vector>
 
 namespace sw {
-int corner_turn(const void* src, void* dst, int nchans, int R, int C, hipStream_t st);
+int corner_turn(const void* src, void* dst, int nchans, int R, int C, hipStream_t st, int src_ld = 0);
+int corner_turn_f64(const void* src, void* dst, int nchans, int R, int C, hipStream_t st, int src_ld = 0);
 
 namespace {
 
@@ -48,10 +49,13 @@ struct CdParams {
     const int8_t* hist;  // [F][PA][Oh]
     float2* z;           // [F*nseg][RP][PA][L] (N > kMaxL)
     int T, O, Oh, hop, nseg, PA, F, RP, N;
+    long in0;   // input index (relative to the gulp start) of segment 0's first sample
+    long y0;    // output-buffer index of segment 0's first kept sample
+    int ylen;   // output-buffer samples per column
     int ndm;
     const float2* chirp[kMaxDm];  // [F][RP][L] (RP == 1: natural order)
-    int8_t* out[kMaxDm];          // [F][PA][T] complex int8
-    float2* outf[kMaxDm];         // optional FTPA-ordered [F][T][PA] y/N
+    int8_t* out[kMaxDm];          // [F][PA][ylen] complex int8
+    float2* outf[kMaxDm];         // optional [F][PA][ylen] y/N
     float2* v[kMaxDm];            // [F*nseg][RP][PA][L] (N > kMaxL)
     const float2* tw;             // exp(-2 pi i m / L)
     const float2* twN;            // exp(-2 pi i m / N)
@@ -78,6 +82,19 @@ __device__ __forceinline__ float2 input_sample(const CdParams& p, int f, int col
     return make_float2(0.0f, 0.0f);
 }
 
+// raw complex int8 sample (re | im << 8), zero outside history + gulp
+__device__ __forceinline__ uint32_t input_raw(const CdParams& p, int f, int col, int tau)
+{
+    if(tau < 0) return *reinterpret_cast<const uint16_t*>(p.hist + (((size_t)f * p.PA + col) * p.Oh + p.Oh + tau) * 2);
+    if(tau < p.T) return *reinterpret_cast<const uint16_t*>(p.cur + (((size_t)f * p.PA + col) * p.T + tau) * 2);
+    return 0;
+}
+
+__device__ __forceinline__ float2 unpack_c8(uint32_t v)
+{
+    return make_float2((float)(int8_t)(v & 0xff), (float)(int8_t)((v >> 8) & 0xff));
+}
+
 template <int L, int C, int NBUF, int NT, bool F32>
 __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
 {
@@ -101,7 +118,7 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
             if constexpr(F32)
                 v = p.z[(((size_t)fs * p.RP + r) * p.PA + col) * L + n];
             else
-                v = input_sample(p, f, col, s * p.hop - p.O + n);
+                v = input_sample(p, f, col, (int)(p.in0 + (long)s * p.hop + n));
             X[c * padded(L) + pad(n)] = v;
         }
         __syncthreads();
@@ -132,12 +149,13 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
             const int half = p.O / 2;
             for(int i = tid; i < C * p.hop; i += NT) {
                 const int c = i / p.hop, m = i % p.hop;
-                const int j = s * p.hop + m;
-                if(j < p.T) {
+                const long j = p.y0 + (long)s * p.hop + m;
+                if(j < p.ylen) {
                     const int col = g * C + c;
                     const float2 y = cscale(W[c * padded(L) + pad(half + m)], inv_n);
-                    store_c8(out + (((size_t)f * p.PA + col) * p.T + j) * 2, y);
-                    if(outf) outf[((size_t)f * p.T + j) * p.PA + col] = y;
+                    const size_t o = ((size_t)f * p.PA + col) * p.ylen + j;
+                    store_c8(out + o * 2, y);
+                    if(outf) outf[o] = y;
                 }
             }
         }
@@ -176,22 +194,41 @@ __device__ __forceinline__ void apply_powers(float2* v, float2 w1)
     }
 }
 
-// One DM per launch (d): forward FFT, chirp, inverse FFT of one column.
-__global__ void __launch_bounds__(512) codedisp_16k_kernel(CdParams p, int d)
+// One DM per launch (d): forward FFT, chirp, inverse FFT of one column per
+// unit u = (channel, segment, column).  Persistent: a workgroup (one per CU,
+// the exchange buffer fills the LDS) walks units u = blockIdx.x + k gridDim.x
+// and loads the next unit's 32 input samples per thread while transforming
+// the current one.
+__global__ void __launch_bounds__(512) codedisp_16k_kernel(CdParams p, int d, int nunits)
 {
     __shared__ float2 lds[k16Lds + TwiddleLds<k16>::SIZE];
     const Twiddles tw = load_twiddles<k16, 512>(lds + k16Lds, p.tw);
-    const int col = blockIdx.x % p.PA;
-    const int fs  = blockIdx.x / p.PA;
-    const int f = fs / p.nseg, s = fs % p.nseg;
     const int t = threadIdx.x;
     const float inv_n = 1.0f / (float)k16;
+    uint32_t nxt[16];  // two raw samples per register
+    auto fetch = [&](int u) {
+        const int col = u % p.PA, fs = u / p.PA;
+        const int f = fs / p.nseg, s = fs % p.nseg;
+#pragma unroll
+        for(int i = 0; i < 16; ++i)
+            nxt[i] = input_raw(p, f, col, (int)(p.in0 + (long)s * p.hop + 512 * (2 * i) + t)) |
+                     (input_raw(p, f, col, (int)(p.in0 + (long)s * p.hop + 512 * (2 * i + 1) + t)) << 16);
+    };
+    if(blockIdx.x < nunits) fetch(blockIdx.x);
+    for(int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const int col = u % p.PA;
+    const int fs  = u / p.PA;
+    const int f = fs / p.nseg, s = fs % p.nseg;
     float2 v[32];
 
     // ---- forward, step 1: thread t = (n2, n3) holds n1 = 0..31
 #pragma unroll
-    for(int n1 = 0; n1 < 32; ++n1) v[n1] = input_sample(p, f, col, s * p.hop - p.O + 512 * n1 + t);
-    __syncthreads();  // twiddle tables
+    for(int i = 0; i < 16; ++i) {
+        v[2 * i]     = unpack_c8(nxt[i]);
+        v[2 * i + 1] = unpack_c8(nxt[i] >> 16);
+    }
+    if(u + (int)gridDim.x < nunits) fetch(u + gridDim.x);  // in flight during the transforms
+    __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
     dft32<false>(v);
     apply_powers<32>(v, tw(t));
 #pragma unroll
@@ -260,12 +297,14 @@ __global__ void __launch_bounds__(512) codedisp_16k_kernel(CdParams p, int d)
 #pragma unroll
     for(int n1 = 0; n1 < 32; ++n1) {
         const int pos = 512 * n1 + t;
-        const int j = s * p.hop + pos - half;
-        if(pos >= half && pos < k16 - half && j < p.T) {
+        const long j = p.y0 + (long)s * p.hop + pos - half;
+        if(pos >= half && pos < k16 - half && j < p.ylen) {
             const float2 y = cscale(v[n1], inv_n);
-            store_c8(out + (((size_t)f * p.PA + col) * p.T + j) * 2, y);
-            if(outf) outf[((size_t)f * p.T + j) * p.PA + col] = y;
+            const size_t o = ((size_t)f * p.PA + col) * p.ylen + j;
+            store_c8(out + o * 2, y);
+            if(outf) outf[o] = y;
         }
+    }
     }
 }
 
@@ -284,7 +323,7 @@ __global__ void __launch_bounds__(256) codedisp_dif_kernel(CdParams p, int L)
     const int f = fs / p.nseg, s = fs % p.nseg;
     float2 v[RP];
 #pragma unroll
-    for(int q = 0; q < RP; ++q) v[q] = input_sample(p, f, col, s * p.hop - p.O + n + q * L);
+    for(int q = 0; q < RP; ++q) v[q] = input_sample(p, f, col, (int)(p.in0 + (long)s * p.hop + n + q * L));
     dft<RP, false>(v);
 #pragma unroll
     for(int rr = 0; rr < RP; ++rr)
@@ -314,11 +353,12 @@ __global__ void __launch_bounds__(256) codedisp_dit_kernel(CdParams p, int L, in
     for(int q = 0; q < RP; ++q) {
         const int pos = n + q * L;
         if(pos >= half && pos < p.N - half) {
-            const int j = s * p.hop + pos - half;
-            if(j < p.T) {
+            const long j = p.y0 + (long)s * p.hop + pos - half;
+            if(j < p.ylen) {
                 const float2 y = cscale(v[q], inv_n);
-                store_c8(p.out[d] + (((size_t)f * p.PA + col) * p.T + j) * 2, y);
-                if(p.outf[d]) p.outf[d][((size_t)f * p.T + j) * p.PA + col] = y;
+                const size_t o = ((size_t)f * p.PA + col) * p.ylen + j;
+                store_c8(p.out[d] + o * 2, y);
+                if(p.outf[d]) p.outf[d][o] = y;
             }
         }
     }
@@ -352,15 +392,46 @@ __global__ void chirp_kernel(float2* out, int nchans, int N, int RP, double low_
     out[((size_t)chan * RP + r) * L + m] = make_float2((float)c, (float)s);
 }
 
-// history[f][col][i] = gulp[f][col][T - Oh + i]
-__global__ void history_kernel(int8_t* hist, const int8_t* cur, int ncols, int T, int Oh)
+// new history = the last Oh samples of (old history ++ gulp), per column
+__global__ void history_kernel(int8_t* dst, const int8_t* old, const int8_t* cur, int ncols, int T, int Oh)
 {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(idx >= (size_t)ncols * Oh) return;
     const int i = (int)(idx % Oh);
     const size_t col = idx / Oh;
-    *reinterpret_cast<uint16_t*>(hist + idx * 2) =
-        *reinterpret_cast<const uint16_t*>(cur + (col * T + T - Oh + i) * 2);
+    const long q = (long)i + T;  // position in old ++ gulp
+    const uint16_t v = q < Oh ? *reinterpret_cast<const uint16_t*>(old + (col * Oh + q) * 2)
+                              : *reinterpret_cast<const uint16_t*>(cur + (col * T + (q - Oh)) * 2);
+    *reinterpret_cast<uint16_t*>(dst + idx * 2) = v;
+}
+
+// dst[col][i] = src[col][off + i] for i < n (row pitches sld, dld): carries
+// outputs computed ahead of the emitted window to the front of the next
+// gulp's buffer; when the regions overlap (latency > T) the copy goes
+// through a scratch buffer in two steps
+template <typename E>
+__global__ void carry_kernel(E* dst, const E* src, int ncols, int dld, int sld, int off, int n)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(idx >= (size_t)ncols * n) return;
+    const int i = (int)(idx % n);
+    const size_t col = idx / n;
+    dst[col * dld + i] = src[col * sld + off + i];
+}
+
+template <typename E>
+int carry(E* y, E* scratch, int ncols, int ylen, int T, int lat, hipStream_t st)
+{
+    const size_t n = (size_t)ncols * lat;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if(lat <= T) {
+        hipLaunchKernelGGL(carry_kernel<E>, grid, dim3(256), 0, st, y, y, ncols, ylen, ylen, T, lat);
+    } else {
+        hipLaunchKernelGGL(carry_kernel<E>, grid, dim3(256), 0, st, scratch, y, ncols, lat, ylen, T, lat);
+        hipLaunchKernelGGL(carry_kernel<E>, grid, dim3(256), 0, st, y, scratch, ncols, ylen, lat, 0, lat);
+    }
+    SW_LAUNCH_CHECK();
+    return SW_OK;
 }
 
 // ------------------------------------------------------------------ host
@@ -372,6 +443,16 @@ LdsShape lds_shape(int L)
 {
     if(L >= 16384) return {1, 1, 512};
     return {8192 / L, 2, 512};
+}
+
+// one persistent workgroup per CU of the current device
+int persistent_blocks()
+{
+    int dev = 0, ncu = 0;
+    if(hipGetDevice(&dev) != hipSuccess ||
+       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+        return 256;
+    return ncu;
 }
 
 template <bool F32>
@@ -391,7 +472,8 @@ int launch_lds(int L, const CdParams& p, int blocks, hipStream_t st)
             hipLaunchKernelGGL((codedisp_tm_kernel<16384, 1, 1, 512, true>), dim3(blocks), dim3(512), 0, st, p);
         else
             for(int d = 0; d < p.ndm; ++d)
-                hipLaunchKernelGGL(codedisp_16k_kernel, dim3(blocks), dim3(512), 0, st, p, d);
+                hipLaunchKernelGGL(codedisp_16k_kernel, dim3(std::min(blocks, persistent_blocks())), dim3(512), 0,
+                                   st, p, d, blocks);
         break;
     default: set_error("codedisp: unsupported transform length"); return SW_ERR_UNSUPPORTED;
     }
@@ -403,8 +485,10 @@ int launch_lds(int L, const CdParams& p, int blocks, hipStream_t st)
 } // namespace
 
 struct CodedispGroup {
-    int N, O, L, RP, hop, nseg;
-    std::vector<int> dms;  // indices into the plan's DM list
+    int N, O, L, RP, hop, nseg;  // nseg: most segments one gulp can need
+    int lat = 0;                 // output latency (samples): 0 if hop divides T, else hop
+    long long mdone = 0;         // segments of the stream transformed so far
+    std::vector<int> dms;        // indices into the plan's DM list
 };
 
 } // namespace sw
@@ -419,18 +503,28 @@ struct sw_codedisp {
     std::vector<sw::CodedispGroup> groups;
     std::vector<float2*> chirp;          // per DM
     std::map<int, float2*> twiddles;     // per length
-    int8_t* hist = nullptr;              // [F][PA][Omax] complex int8
+    int Oh = 0;                          // history samples per column
+    int8_t* hist = nullptr;              // [F][PA][Oh] complex int8
+    int8_t* hist2 = nullptr;             // the other history buffer (ping-pong)
     int8_t* xtm  = nullptr;              // time-major copy of an FTPA input
-    std::vector<int8_t*> ytm;            // time-major outputs, one per DM slot
+    std::vector<int8_t*> ytm;            // per DM: [F][PA][T + lat] time-major outputs
+    std::vector<float2*> ytf;            // per DM, on demand: float outputs, same layout
+    std::vector<int> ylen;               // per DM: T + lat
+    std::vector<int> lat;                // per DM output latency
+    long long gulps = 0;                 // gulps completed (push_history calls)
     float2* zbuf = nullptr;              // DIF stage output (N > kMaxL)
     std::vector<float2*> vbuf;           // per DM slot (N > kMaxL)
+    void* carry_tmp = nullptr;           // carries with latency > T
     ~sw_codedisp()
     {
+        (void)hipFree(carry_tmp);
         for(auto* c : chirp) (void)hipFree(c);
         for(auto& kv : twiddles) (void)hipFree(kv.second);
         (void)hipFree(hist);
+        (void)hipFree(hist2);
         (void)hipFree(xtm);
         for(auto* y : ytm) (void)hipFree(y);
+        for(auto* y : ytf) (void)hipFree(y);
         (void)hipFree(zbuf);
         for(auto* v : vbuf) (void)hipFree(v);
     }
@@ -516,6 +610,10 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
             g.L = std::min(g.N, sw::kMaxL);
             g.RP = g.N / g.L;
             g.hop = g.N - g.O;
+            // segments sit on one grid over the whole stream; when hop does
+            // not divide the gulp, a gulp's last segment reaches into the next
+            // gulp, so its outputs are emitted one hop later (latency = hop)
+            g.lat = (p->T % g.hop == 0) ? 0 : g.hop;
             g.nseg = (p->T + g.hop - 1) / g.hop;
             SW_CHECK_ARG(p->PA % sw::lds_shape(g.L).C == 0,
                          "2*nantennas must be a multiple of the transform's column tile");
@@ -523,6 +621,16 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
             p->groups.push_back(g);
         }
         p->groups[gid[key]].dms.push_back(d);
+    }
+    p->lat.assign(p->dms.size(), 0);
+    p->ylen.assign(p->dms.size(), p->T);
+    for(auto& g : p->groups) {
+        // history: the input a gulp's first segment can reach back to
+        p->Oh = std::max(p->Oh, g.lat > 0 ? g.N : g.O);
+        for(int d : g.dms) {
+            p->lat[d] = g.lat;
+            p->ylen[d] = (p->T + g.lat + 3) / 4 * 4;  // 8-byte aligned rows for the corner turn
+        }
     }
     size_t zmax = 0, slots = 1, vslots = 0;
     for(auto& g : p->groups) {
@@ -549,14 +657,26 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
         SW_LAUNCH_CHECK();
     }
     const size_t vol = (size_t)p->F * p->PA * p->T * 2;
-    if(p->Omax > 0) {
-        const size_t hb = (size_t)p->F * p->PA * p->Omax * 2;
+    if(p->Oh > 0) {
+        const size_t hb = (size_t)p->F * p->PA * p->Oh * 2;
         SW_HIP_TRY(hipMalloc(&p->hist, hb));
+        SW_HIP_TRY(hipMalloc(&p->hist2, hb));
         SW_HIP_TRY(hipMemsetAsync(p->hist, 0, hb, st));
     }
     if(!p->input_fpat) SW_HIP_TRY(hipMalloc(&p->xtm, vol));
-    p->ytm.resize(slots, nullptr);
-    for(auto& y : p->ytm) SW_HIP_TRY(hipMalloc(&y, vol));
+    p->ytm.resize(p->dms.size(), nullptr);
+    p->ytf.resize(p->dms.size(), nullptr);
+    for(size_t d = 0; d < p->dms.size(); ++d) {
+        const size_t yb = (size_t)p->F * p->PA * p->ylen[d] * 2;
+        SW_HIP_TRY(hipMalloc(&p->ytm[d], yb));
+        SW_HIP_TRY(hipMemsetAsync(p->ytm[d], 0, yb, st));  // outputs before the stream start are zero
+    }
+    (void)slots;
+    int latmax_over_T = 0;
+    for(int l: p->lat)
+        if(l > p->T) latmax_over_T = std::max(latmax_over_T, l);
+    if(latmax_over_T > 0)
+        SW_HIP_TRY(hipMalloc(&p->carry_tmp, (size_t)p->F * p->PA * latmax_over_T * sizeof(float2)));
     if(zmax) {
         SW_HIP_TRY(hipMalloc(&p->zbuf, zmax));
         p->vbuf.resize(vslots, nullptr);
@@ -576,26 +696,28 @@ extern "C" int sw_codedisp_destroy(sw_codedisp* plan)
     return SW_OK;
 }
 
-extern "C" int sw_codedisp_info(const sw_codedisp* plan, int dm_idx, int* fft_len, int* overlap)
+extern "C" int sw_codedisp_info(const sw_codedisp* plan, int dm_idx, int* fft_len, int* overlap,
+                                int* latency)
 {
     SW_CHECK_ARG(plan && dm_idx >= 0 && dm_idx < (int)plan->dms.size(), "bad DM index");
     if(fft_len) *fft_len = plan->N[dm_idx];
     if(overlap) *overlap = plan->O[dm_idx];
+    if(latency) *latency = plan->lat[dm_idx];
     return SW_OK;
 }
 
 extern "C" int sw_codedisp_set_history(sw_codedisp* plan, const void* d_hist, void* stream)
 {
     SW_CHECK_ARG(plan && d_hist, "null argument");
-    if(plan->Omax == 0) return SW_OK;
+    if(plan->Oh == 0) return SW_OK;
     hipStream_t st = sw::as_stream(stream);
     if(plan->input_fpat) {
-        SW_HIP_TRY(hipMemcpyAsync(plan->hist, d_hist, (size_t)plan->F * plan->PA * plan->Omax * 2,
+        SW_HIP_TRY(hipMemcpyAsync(plan->hist, d_hist, (size_t)plan->F * plan->PA * plan->Oh * 2,
                                   hipMemcpyDeviceToDevice, st));
         return SW_OK;
     }
-    // FTPA [F][Omax][PA] -> [F][PA][Omax]
-    return sw::corner_turn(d_hist, plan->hist, plan->F, plan->Omax, plan->PA, st);
+    // FTPA [F][Oh][PA] -> [F][PA][Oh]
+    return sw::corner_turn(d_hist, plan->hist, plan->F, plan->Oh, plan->PA, st);
 }
 
 static const int8_t* time_major_input(sw_codedisp* plan, const void* d_in, hipStream_t st, int& rc)
@@ -609,15 +731,31 @@ static const int8_t* time_major_input(sw_codedisp* plan, const void* d_in, hipSt
 extern "C" int sw_codedisp_push_history(sw_codedisp* plan, const void* d_in, void* stream)
 {
     SW_CHECK_ARG(plan && d_in, "null argument");
-    if(plan->Omax == 0) return SW_OK;
-    SW_CHECK_ARG(plan->T >= plan->Omax, "gulp shorter than the overlap");
     hipStream_t st = sw::as_stream(stream);
+    const int ncols = plan->F * plan->PA;
+    // outputs computed ahead of this gulp's window move to the front
+    for(size_t d = 0; d < plan->dms.size(); ++d) {
+        const int lat = plan->lat[d];
+        if(lat == 0) continue;
+        int rc = sw::carry(reinterpret_cast<uint16_t*>(plan->ytm[d]), static_cast<uint16_t*>(plan->carry_tmp),
+                           ncols, plan->ylen[d], plan->T, lat, st);
+        if(rc) return rc;
+        if(plan->ytf[d]) {
+            rc = sw::carry(plan->ytf[d], static_cast<float2*>(plan->carry_tmp), ncols, plan->ylen[d], plan->T, lat,
+                           st);
+            if(rc) return rc;
+        }
+    }
+    for(auto& g : plan->groups) g.mdone = ((long long)(plan->gulps + 1) * plan->T) / g.hop;
+    ++plan->gulps;
+    if(plan->Oh == 0) return SW_OK;
     // an FTPA input was turned into xtm by the run of this gulp
     const int8_t* cur = plan->input_fpat ? static_cast<const int8_t*>(d_in) : plan->xtm;
-    const size_t n = (size_t)plan->F * plan->PA * plan->Omax;
+    const size_t n = (size_t)ncols * plan->Oh;
     hipLaunchKernelGGL(sw::history_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                       plan->hist, cur, plan->F * plan->PA, plan->T, plan->Omax);
+                       plan->hist2, plan->hist, cur, ncols, plan->T, plan->Oh);
     SW_LAUNCH_CHECK();
+    std::swap(plan->hist, plan->hist2);
     return SW_OK;
 }
 
@@ -636,6 +774,14 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const*
         std::vector<int> todo;
         for(int d : g.dms)
             if(d >= dm_begin && d < dm_begin + dm_count) todo.push_back(d);
+        // segments of this gulp: all grid segments whose input ends within it
+        const long long gT = plan->gulps * (long long)plan->T;
+        const long long m0 = g.mdone;
+        const long long m1 = (gT + plan->T) / g.hop;
+        const int nseg = (int)(m1 - m0);
+        if(nseg <= 0 && !todo.empty()) {
+            // nothing new to transform (possible only when a gulp is shorter than hop)
+        }
         const int per_launch = batched ? sw::kMaxDm : 1;
         for(size_t b0 = 0; b0 < todo.size(); b0 += per_launch) {
             const int nb = (int)std::min<size_t>(per_launch, todo.size() - b0);
@@ -645,9 +791,12 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const*
             p.z    = plan->zbuf;
             p.T = plan->T;
             p.O = g.O;
-            p.Oh = plan->Omax;
+            p.Oh = plan->Oh;
             p.hop = g.hop;
-            p.nseg = g.nseg;
+            p.nseg = std::max(nseg, 0);
+            p.in0 = (long)(m0 * g.hop - g.O - gT);
+            p.y0 = (long)(m0 * g.hop - gT + g.lat);
+            p.ylen = plan->ylen[g.dms[0]];
             p.PA = plan->PA;
             p.F = plan->F;
             p.RP = g.RP;
@@ -659,16 +808,24 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const*
                 const int d = todo[b0 + k];
                 SW_CHECK_ARG(d_out[d], "null output buffer");
                 p.chirp[k] = plan->chirp[d];
-                p.out[k]   = plan->ytm[k];
-                p.outf[k]  = d_out_f32 ? static_cast<float2*>(d_out_f32[d]) : nullptr;
+                p.out[k]   = plan->ytm[d];
+                if(d_out_f32 && d_out_f32[d] && !plan->ytf[d]) {
+                    // float outputs (a diagnostic path): buffer allocated on first use
+                    const size_t yb = (size_t)plan->F * plan->PA * plan->ylen[d] * sizeof(float2);
+                    SW_HIP_TRY(hipMalloc(&plan->ytf[d], yb));
+                    SW_HIP_TRY(hipMemsetAsync(plan->ytf[d], 0, yb, st));
+                }
+                p.outf[k]  = (d_out_f32 && d_out_f32[d]) ? plan->ytf[d] : nullptr;
                 if(g.RP > 1) p.v[k] = plan->vbuf[k];
             }
             const int G = plan->PA / sw::lds_shape(g.L).C;
-            if(g.RP == 1) {
-                rc = sw::launch_lds<false>(g.L, p, plan->F * g.nseg * G, st);
+            if(p.nseg == 0) {
+                // no segment completes in this gulp
+            } else if(g.RP == 1) {
+                rc = sw::launch_lds<false>(g.L, p, plan->F * p.nseg * G, st);
                 if(rc) return rc;
             } else {
-                const size_t items = (size_t)plan->F * g.nseg * plan->PA * g.L;
+                const size_t items = (size_t)plan->F * p.nseg * plan->PA * g.L;
                 const unsigned eb = (unsigned)((items + 255) / 256);
                 switch(g.RP) {
                 case 2: hipLaunchKernelGGL(sw::codedisp_dif_kernel<2>, dim3(eb), dim3(256), 0, st, p, g.L); break;
@@ -678,7 +835,7 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const*
                 default: sw::set_error("codedisp: FFT length too large"); return SW_ERR_UNSUPPORTED;
                 }
                 SW_LAUNCH_CHECK();
-                rc = sw::launch_lds<true>(g.L, p, plan->F * g.nseg * g.RP * G, st);
+                rc = sw::launch_lds<true>(g.L, p, plan->F * p.nseg * g.RP * G, st);
                 if(rc) return rc;
                 for(int k = 0; k < nb; ++k) {
                     switch(g.RP) {
@@ -690,10 +847,16 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const*
                     SW_LAUNCH_CHECK();
                 }
             }
-            // time-major result -> FTPA for the beamformers
+            // emitted window [0, T) of the time-major result -> FTPA for the beamformers
             for(int k = 0; k < nb; ++k) {
-                rc = sw::corner_turn(plan->ytm[k], d_out[todo[b0 + k]], plan->F, plan->PA, plan->T, st);
+                const int d = todo[b0 + k];
+                rc = sw::corner_turn(plan->ytm[d], d_out[d], plan->F, plan->PA, plan->T, st, plan->ylen[d]);
                 if(rc) return rc;
+                if(p.outf[k]) {
+                    rc = sw::corner_turn_f64(plan->ytf[d], d_out_f32[d], plan->F, plan->PA, plan->T, st,
+                                             plan->ylen[d]);
+                    if(rc) return rc;
+                }
             }
         }
     }

@@ -14,14 +14,22 @@
 // oracle's order (time ascending, antenna ascending, one rounding for the
 // product and one for the sum), so raw powers are bit-identical to the
 // oracle for any weights, not only when the sums happen to be exact.  The
-// arithmetic is tiny next to the HBM read of the FTPA block; the rows are
-// staged through LDS so that the HBM reads are coalesced.
+// rows are staged through LDS so that the HBM reads are coalesced.
+//
+// Exact fast path (Stokes I, antenna weights all 0 or 1): every term
+// w_a (re^2 + im^2) is a non-negative integer, so while the total stays
+// below 2^24 every partial sum of the oracle's float chain is exact and the
+// float result equals the integer sum.  The kernel then sums with
+// v_dot4_i32_i8 (two antennas of one polarisation per word, masked by the
+// beamset's 0/1 weights) in int32 and converts once; a wave whose total
+// reaches 2^24 on any lane recomputes its outputs with the float chain.
 #include "sw_common.hpp"
 
 namespace sw {
 namespace {
 
-constexpr int kOut = 64;  // output samples per workgroup (one wave)
+constexpr int kOut = 64;        // output samples per workgroup (one wave)
+constexpr int kMaskWords = 64;  // 0/1 weight masks for up to 128 antennas
 
 // One wave per (64 consecutive output samples, channel).  For each step tt of
 // the scrunch window the 64 rows t'*ts + tt (one per output) are staged in
@@ -50,6 +58,64 @@ ib_kernel(const uint8_t* __restrict__ ftpa, float* __restrict__ raw, int8_t* __r
 #pragma unroll
         for(int k = 0; k < NV; ++k) power[s][k] = 0.0f;
     const uint4* chan = reinterpret_cast<const uint4*>(ftpa + (size_t)f * nsamples * 4 * nant);
+
+    // ---- exact integer path
+    __shared__ uint32_t wmask[SMAX][kMaskWords];  // per word: 0xFFFF per antenna of weight 1
+    bool exact = false;
+    if constexpr(S == SW_STOKES_I) {
+        if(nant <= 2 * kMaskWords) {
+            bool binary = true;
+            for(int i = lane; i < nbeamsets * nant / 2; i += kOut) {
+                const int sb = i / (nant / 2), j = i % (nant / 2);
+                const float w0 = antw[sb * nant + 2 * j], w1 = antw[sb * nant + 2 * j + 1];
+                binary = binary && (w0 == 0.0f || w0 == 1.0f) && (w1 == 0.0f || w1 == 1.0f);
+                wmask[sb][j] = (w0 != 0.0f ? 0x0000FFFFu : 0u) | (w1 != 0.0f ? 0xFFFF0000u : 0u);
+            }
+            exact = __syncthreads_and(binary);
+        }
+    }
+    if(exact) {
+        int isum[SMAX];
+#pragma unroll
+        for(int s = 0; s < SMAX; ++s) isum[s] = 0;
+        const int hw = nant / 2;  // words per polarisation
+        for(int tt = 0; tt < tscrunch; ++tt) {
+            __syncthreads();
+            for(int i = lane; i < nrows * rowq; i += kOut) {
+                const int r = i / rowq, q = i % rowq;
+                rows[r * ldsq + q] = chan[((size_t)(tp0 + r) * tscrunch + tt) * rowq + q];
+            }
+            __syncthreads();
+            if(lane < nrows) {
+                const uint32_t* rw = reinterpret_cast<const uint32_t*>(rows + lane * ldsq);
+#pragma unroll
+                for(int s = 0; s < SMAX; ++s) {
+                    if(s < nbeamsets) {
+                        int acc = 0;
+                        for(int j = 0; j < hw; ++j) {
+                            const int m = (int)wmask[s][j];
+                            const int x0 = (int)rw[j], x1 = (int)rw[hw + j];
+                            acc = __builtin_amdgcn_sdot4(x0 & m, x0, acc, false);
+                            acc = __builtin_amdgcn_sdot4(x1 & m, x1, acc, false);
+                        }
+                        isum[s] += acc;
+                    }
+                }
+            }
+        }
+        bool big = false;
+#pragma unroll
+        for(int s = 0; s < SMAX; ++s) {
+            big = big || (isum[s] >= (1 << 24));
+            power[s][0] = (float)isum[s];
+        }
+        exact = !__any(big);  // wave-uniform
+    }
+    if(!exact) {
+#pragma unroll
+    for(int s = 0; s < SMAX; ++s)
+#pragma unroll
+        for(int k = 0; k < NV; ++k) power[s][k] = 0.0f;
     for(int tt = 0; tt < tscrunch; ++tt) {
         __syncthreads();
         for(int i = lane; i < nrows * rowq; i += kOut) {
@@ -91,6 +157,7 @@ ib_kernel(const uint8_t* __restrict__ ftpa, float* __restrict__ raw, int8_t* __r
                 }
             }
         }
+    }
     }
     if(tp >= nsamps_out) return;
 #pragma unroll

@@ -65,12 +65,19 @@ __device__ __forceinline__ int8_t id_quant(int32_t sum, float scale)
     return (int8_t)fmaxf(-128.0f, fminf(127.0f, v));
 }
 
-template <int DPW, int TT, bool WIDE>
+constexpr int kPfRows = 64;  // stage rows up to which the next channel is prefetched in registers
+
+// Input word (t, f, w) lives at t * stride_t + f * stride_f + w: TFB
+// (stride_t = F * nwords, stride_f = nwords) or channel-major FTB
+// (stride_t = nwords, stride_f = T * nwords), which keeps a channel's rows
+// together.  PF: the next channel's stage (<= kPfRows rows) is loaded into
+// registers while the current one is summed.
+template <int DPW, int TT, bool WIDE, bool PF>
 __global__ void __launch_bounds__(256)
 idisp_kernel(const uint32_t* __restrict__ tfb, uint32_t* __restrict__ tdb,
              const int* __restrict__ delays, const int2* __restrict__ span,
              const int2* __restrict__ tiles, int nchans, int nwords, int ndms, int ts,
-             int max_delay, long nsamples, int nout, int tb0, float scale)
+             int max_delay, long nsamples, int nout, int tb0, float scale, long stride_t, long stride_f)
 {
     extern __shared__ uint4 stage[];  // [rows][16] uint4 = [rows][64] words
     const uint32_t* rows = reinterpret_cast<const uint32_t*>(stage);
@@ -98,32 +105,61 @@ idisp_kernel(const uint32_t* __restrict__ tfb, uint32_t* __restrict__ tdb,
         }
     int pending = 0;  // biased bytes in the 16-bit lanes (WIDE only)
 
+    // 16-byte pieces of channel f's stage (rows 16-byte aligned: nwords % 4 == 0)
+    auto piece = [&](int f, long lo, int i) {
+        const int r = i >> 4, q = i & 15;
+        const int w4 = chunk * 64 + q * 4;
+        uint4 v = make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
+        if(w4 < nwords && lo + r < nsamples) {
+            v = *reinterpret_cast<const uint4*>(tfb + (lo + r) * stride_t + f * stride_f + w4);
+            v.x ^= 0x80808080u;
+            v.y ^= 0x80808080u;
+            v.z ^= 0x80808080u;
+            v.w ^= 0x80808080u;
+        }
+        return v;
+    };
+    constexpr int NPF = PF ? kPfRows * 16 / 256 : 1;
+    uint4 pre[NPF];
+    if constexpr(PF) {
+        const int2 sp = span[(size_t)tile * nchans];
+        const int nr = TT * ts + sp.y - sp.x;
+#pragma unroll
+        for(int k = 0; k < NPF; ++k) {
+            const int i = threadIdx.x + 256 * k;
+            if(i < nr * 16) pre[k] = piece(0, in0 - sp.y, i);
+        }
+    }
+
     for(int f = 0; f < nchans; ++f) {
         const int2 sp   = span[(size_t)tile * nchans + f];
         const long lo   = in0 - sp.y;
         const int nrows = TT * ts + sp.y - sp.x;
         __syncthreads();
-        if((nwords & 3) == 0) {  // rows are 16-byte aligned
-            for(int i = threadIdx.x; i < nrows * 16; i += 256) {
-                const int r = i >> 4, q = i & 15;
-                const int w4 = chunk * 64 + q * 4;
-                uint4 v = make_uint4(0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u);
-                if(w4 < nwords && lo + r < nsamples) {
-                    v = *reinterpret_cast<const uint4*>(tfb + ((size_t)(lo + r) * nchans + f) * nwords + w4);
-                    v.x ^= 0x80808080u;
-                    v.y ^= 0x80808080u;
-                    v.z ^= 0x80808080u;
-                    v.w ^= 0x80808080u;
-                }
-                stage[r * 16 + q] = v;
+        if constexpr(PF) {
+#pragma unroll
+            for(int k = 0; k < NPF; ++k) {
+                const int i = threadIdx.x + 256 * k;
+                if(i < nrows * 16) stage[i] = pre[k];
             }
+            if(f + 1 < nchans) {  // next channel's rows, in flight during the sums
+                const int2 sn = span[(size_t)tile * nchans + f + 1];
+                const int nr = TT * ts + sn.y - sn.x;
+#pragma unroll
+                for(int k = 0; k < NPF; ++k) {
+                    const int i = threadIdx.x + 256 * k;
+                    if(i < nr * 16) pre[k] = piece(f + 1, in0 - sn.y, i);
+                }
+            }
+        } else if((nwords & 3) == 0) {
+            for(int i = threadIdx.x; i < nrows * 16; i += 256) stage[i] = piece(f, lo, i);
         } else {
             uint32_t* st32 = reinterpret_cast<uint32_t*>(stage);
             for(int i = threadIdx.x; i < nrows * 64; i += 256) {
                 const int r = i >> 6, q = i & 63;
                 const int w = chunk * 64 + q;
                 uint32_t v = 0x80808080u;
-                if(w < nwords && lo + r < nsamples) v = tfb[((size_t)(lo + r) * nchans + f) * nwords + w] ^ 0x80808080u;
+                if(w < nwords && lo + r < nsamples) v = tfb[(lo + r) * stride_t + f * stride_f + w] ^ 0x80808080u;
                 st32[i] = v;
             }
         }
@@ -376,8 +412,25 @@ extern "C" int sw_idisp_info(const sw_idisp* p, int* max_delay, int* h_delays, f
     return SW_OK;
 }
 
+namespace sw {
+int idisp_launch(sw_idisp* p, const void* d_in, size_t nsamples, int nbeams, int nvec, void* d_tdb, bool ftb,
+                 hipStream_t st);
+}
+
 extern "C" int sw_idisp_dedisperse(sw_idisp* p, const void* d_tfb, size_t nsamples, int nbeams, int nvec,
                                    void* d_tdb, void* stream)
+{
+    return sw::idisp_launch(p, d_tfb, nsamples, nbeams, nvec, d_tdb, false, sw::as_stream(stream));
+}
+
+extern "C" int sw_idisp_dedisperse_ftb(sw_idisp* p, const void* d_ftb, size_t nsamples, int nbeams, int nvec,
+                                       void* d_tdb, void* stream)
+{
+    return sw::idisp_launch(p, d_ftb, nsamples, nbeams, nvec, d_tdb, true, sw::as_stream(stream));
+}
+
+int sw::idisp_launch(sw_idisp* p, const void* d_tfb, size_t nsamples, int nbeams, int nvec, void* d_tdb, bool ftb,
+                     hipStream_t st)
 {
     SW_CHECK_ARG(p && d_tfb && d_tdb, "null argument");
     SW_CHECK_ARG(nbeams > 0 && (nvec == 1 || nvec == 4), "nvec must be 1 (int8) or 4 (char4)");
@@ -397,24 +450,28 @@ extern "C" int sw_idisp_dedisperse(sw_idisp* p, const void* d_tfb, size_t nsampl
     const int nchunks = (nwords + 63) / 64;
     const int ntb = (nout + p->tt - 1) / p->tt;
     const size_t lds = (size_t)p->rows_cap * 256;
-    hipStream_t st = sw::as_stream(stream);
+    const long stride_t = ftb ? nwords : (long)p->nchans * nwords;
+    const long stride_f = ftb ? (long)nsamples * nwords : nwords;
+    const bool pf = p->rows_cap <= sw::kPfRows && (nwords & 3) == 0;
     for(int tb0 = 0; tb0 < ntb; tb0 += 65535) {
         const dim3 grid((unsigned)p->tiles.size(), (unsigned)nchunks, (unsigned)std::min(65535, ntb - tb0));
-#define SW_ID_LAUNCH(TT, WIDE)                                                                             \
-    hipLaunchKernelGGL((sw::idisp_kernel<2, TT, WIDE>), grid, dim3(256), lds, st,                         \
+#define SW_ID_LAUNCH(TT, WIDE, PF)                                                                         \
+    hipLaunchKernelGGL((sw::idisp_kernel<2, TT, WIDE, PF>), grid, dim3(256), lds, st,                     \
                        static_cast<const uint32_t*>(d_tfb), static_cast<uint32_t*>(d_tdb), p->d_delays,   \
                        p->d_span, p->d_tiles, p->nchans, nwords, p->ndms, p->tscrunch, p->max_delay,      \
-                       (long)nsamples, nout, tb0, p->scale)
+                       (long)nsamples, nout, tb0, p->scale, stride_t, stride_f)
         if(p->wide) {
             if(p->tt == 8)
-                SW_ID_LAUNCH(8, true);
+                SW_ID_LAUNCH(8, true, false);
             else
-                SW_ID_LAUNCH(4, true);
+                SW_ID_LAUNCH(4, true, false);
+        } else if(p->tt == 16) {
+            if(pf)
+                SW_ID_LAUNCH(16, false, true);
+            else
+                SW_ID_LAUNCH(16, false, false);
         } else {
-            if(p->tt == 16)
-                SW_ID_LAUNCH(16, false);
-            else
-                SW_ID_LAUNCH(4, false);
+            SW_ID_LAUNCH(4, false, false);
         }
 #undef SW_ID_LAUNCH
         SW_LAUNCH_CHECK();

@@ -11,7 +11,9 @@
 //              TDB block to the handler and move the last max_delay slots to
 //              the front.
 // Unlike the reference (host vectors, OpenMP), the ring and the TDB block
-// live in device memory and the dedispersion runs on the GPU.  The TDB
+// live in device memory and the dedispersion runs on the GPU.  The ring is
+// held channel-major ([F][slots][B], "FTB"), so that the dedisperser reads
+// each channel's rows from one contiguous region.  The TDB
 // block's utc_offset is that of its first output sample (ring start plus
 // max_delay samples); the reference stamps it from the most recently pushed
 // block instead.
@@ -46,6 +48,34 @@ struct sw_idpipe {
 
 namespace {
 
+// dst[f][d0 + i][w] = src[f][s0 + i][w] for f < F, i < k, w < nw (32-bit
+// words), with independent channel and row strides
+__global__ void __launch_bounds__(256)
+rows_copy_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int F, long k, int nw,
+                 long s_f, long s_t, long s0, long d_f, long d_t, long d0)
+{
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t total = (size_t)F * k * nw;
+    if(idx >= total) return;
+    const int w = (int)(idx % nw);
+    const size_t r = idx / nw;
+    const long i = (long)(r % k);
+    const int f = (int)(r / k);
+    dst[f * d_f + (d0 + i) * d_t + w] = src[f * s_f + (s0 + i) * s_t + w];
+}
+
+int rows_copy(const void* src, void* dst, int F, long k, int nw, long s_f, long s_t, long s0, long d_f, long d_t,
+              long d0, hipStream_t st)
+{
+    const size_t total = (size_t)F * k * nw;
+    if(total == 0) return SW_OK;
+    hipLaunchKernelGGL(rows_copy_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                       static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), F, k, nw, s_f, s_t, s0, d_f,
+                       d_t, d0);
+    SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
 void release(sw_idpipe* p)
 {
     for(auto& b: p->blocks) {
@@ -59,7 +89,7 @@ void release(sw_idpipe* p)
 int dispatch(sw_idpipe* p, int bi, hipStream_t st)
 {
     auto& b = p->blocks[bi];
-    int rc = sw_idisp_dedisperse(b.plan, b.ring, b.slots, p->nbeams, p->nvec, b.tdb, st);
+    int rc = sw_idisp_dedisperse_ftb(b.plan, b.ring, b.slots, p->nbeams, p->nvec, b.tdb, st);
     if(rc) return rc;
     SW_HIP_TRY(hipStreamSynchronize(st));
     const size_t nout = b.dispatch / b.tscrunch;
@@ -87,11 +117,15 @@ int dispatch(sw_idpipe* p, int bi, hipStream_t st)
         }
     }
     ++b.ndispatched;
-    // keep the last max_delay slots (AggregationBuffer::dispatch)
+    // keep the last max_delay slots (AggregationBuffer::dispatch); the
+    // source [dispatch, dispatch + max_delay) and destination [0, max_delay)
+    // of every channel do not overlap (dispatch >= max_delay)
     b.filled = 0;
     if(b.max_delay > 0) {
-        SW_HIP_TRY(hipMemcpyAsync(b.ring, b.ring + b.dispatch * p->slot_bytes, b.max_delay * p->slot_bytes,
-                                  hipMemcpyDeviceToDevice, st));
+        const int nw = (int)(p->slot_bytes / p->nchans / 4);
+        const long chan = (long)b.slots * nw;
+        rc = rows_copy(b.ring, b.ring, p->nchans, b.max_delay, nw, chan, nw, (long)b.dispatch, chan, nw, 0, st);
+        if(rc) return rc;
         b.filled = b.max_delay;
     }
     b.ring_utc += b.dispatch * b.tsamp;
@@ -107,6 +141,7 @@ extern "C" int sw_idpipe_create(const sw_idpipe_config* cfg, sw_idpipe** out)
     SW_CHECK_ARG(cfg->nblocks > 0 && cfg->coherent_dms && cfg->tscrunch && cfg->dms && cfg->ndms,
                  "empty dedispersion plan");
     SW_CHECK_ARG(cfg->nbeams > 0 && (cfg->nvec == 1 || cfg->nvec == 4), "bad beam layout");
+    SW_CHECK_ARG((cfg->nbeams * cfg->nvec) % 4 == 0, "nbeams * nvec must be a multiple of 4");
     auto p = std::make_unique<sw_idpipe>();
     p->nchans = cfg->nchans;
     p->nbeams = cfg->nbeams;
@@ -195,12 +230,23 @@ extern "C" int sw_idpipe_push(sw_idpipe* p, int block, const void* tfb, size_t n
     b.tsamp = tsamp;
     if(b.filled == 0 && b.ndispatched == 0) b.ring_utc = utc_offset;
     const auto* src = static_cast<const uint8_t*>(tfb);
+    const int F = p->nchans;
+    const size_t row = p->slot_bytes / F;  // bytes of one (t, f) row
+    const int nw = (int)(row / 4);
     size_t done = 0;
     while(done < nsamples) {
         const size_t k = std::min(nsamples - done, b.slots - b.filled);
-        SW_HIP_TRY(hipMemcpyAsync(b.ring + b.filled * p->slot_bytes, src + done * p->slot_bytes,
-                                  k * p->slot_bytes,
-                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+        // TFB [t][f][b] -> ring [f][slot][b]
+        if(on_device) {
+            int rc = rows_copy(src + done * p->slot_bytes, b.ring, F, (long)k, nw, nw, (long)F * nw, 0,
+                               (long)b.slots * nw, nw, (long)b.filled, st);
+            if(rc) return rc;
+        } else {
+            for(int f = 0; f < F; ++f)
+                SW_HIP_TRY(hipMemcpy2DAsync(b.ring + ((size_t)f * b.slots + b.filled) * row, row,
+                                            src + done * p->slot_bytes + f * row, p->slot_bytes, row, k,
+                                            hipMemcpyHostToDevice, st));
+        }
         b.filled += k;
         done += k;
         if(b.filled == b.slots) {

@@ -48,23 +48,25 @@ taftp_to_fpat_kernel(const uint32_t* __restrict__ in, uint8_t* __restrict__ out,
     *reinterpret_cast<uint2*>(o1) = make_uint2(p1lo, p1hi);
 }
 
-// [F][R][C] -> [F][C][R], 16-bit elements, 64 x 64 tiles
+// [F][R][C] -> [F][C][R] of E (16-bit complex int8 or 8-byte complex
+// float) elements, 64 x 64 tiles; source rows are ld >= C elements apart
+template <typename E>
 __global__ void __launch_bounds__(256)
-corner_turn_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, int R, int C,
+corner_turn_kernel(const E* __restrict__ src, E* __restrict__ dst, int R, int C, int ld,
                    int tiles_r, int tiles_c)
 {
-    __shared__ uint16_t tile[64][65];
+    __shared__ E tile[64][65];
     const int bid = blockIdx.x;
     const int f   = bid / (tiles_r * tiles_c);
     const int rem = bid % (tiles_r * tiles_c);
     const int tr = rem / tiles_c, tc = rem % tiles_c;
     const int r0 = tr * 64, c0 = tc * 64;
-    const uint16_t* s = src + (size_t)f * R * C;
-    uint16_t* d       = dst + (size_t)f * R * C;
+    const E* s = src + (size_t)f * R * ld;
+    E* d       = dst + (size_t)f * R * C;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     for(int i = ty; i < 64; i += 4) {
         const int r = r0 + i, c = c0 + tx;
-        if(r < R && c < C) tile[i][tx] = s[(size_t)r * C + c];
+        if(r < R && c < C) tile[i][tx] = s[(size_t)r * ld + c];
     }
     __syncthreads();
     for(int i = ty; i < 64; i += 4) {
@@ -77,7 +79,7 @@ corner_turn_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst,
 // C % 4 == 0): each thread loads 4 consecutive columns of a row and stores 4
 // consecutive rows of a column.
 __global__ void __launch_bounds__(256)
-corner_turn_x4_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, int R, int C,
+corner_turn_x4_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, int R, int C, int ld,
                       int tiles_r, int tiles_c)
 {
     __shared__ uint16_t tile[64][68];  // 136-byte rows: 8-byte aligned, banks staggered
@@ -86,14 +88,14 @@ corner_turn_x4_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ d
     const int rem = bid % (tiles_r * tiles_c);
     const int tr = rem / tiles_c, tc = rem % tiles_c;
     const int r0 = tr * 64, c0 = tc * 64;
-    const uint16_t* s = src + (size_t)f * R * C;
+    const uint16_t* s = src + (size_t)f * R * ld;
     uint16_t* d       = dst + (size_t)f * R * C;
     const int g = threadIdx.x & 15, q = threadIdx.x >> 4;
 #pragma unroll
     for(int k = 0; k < 4; ++k) {
         const int i = q + 16 * k, r = r0 + i, c = c0 + 4 * g;
         uint2 v = make_uint2(0, 0);
-        if(r < R && c < C) v = *reinterpret_cast<const uint2*>(s + (size_t)r * C + c);
+        if(r < R && c < C) v = *reinterpret_cast<const uint2*>(s + (size_t)r * ld + c);
         *reinterpret_cast<uint2*>(&tile[i][4 * g]) = v;
     }
     __syncthreads();
@@ -182,13 +184,25 @@ constexpr int kStatChunks = 8;
 
 } // namespace
 
-int corner_turn(const void* src, void* dst, int nchans, int R, int C, hipStream_t st)
+int corner_turn(const void* src, void* dst, int nchans, int R, int C, hipStream_t st, int src_ld)
 {
+    const int ld = src_ld > 0 ? src_ld : C;
     const int tr = (R + 63) / 64, tc = (C + 63) / 64;
-    const bool x4 = R % 4 == 0 && C % 4 == 0 && ((uintptr_t)src & 7) == 0 && ((uintptr_t)dst & 7) == 0;
-    hipLaunchKernelGGL(x4 ? corner_turn_x4_kernel : corner_turn_kernel, dim3((unsigned)((size_t)nchans * tr * tc)),
-                       dim3(256), 0, st, static_cast<const uint16_t*>(src), static_cast<uint16_t*>(dst), R, C, tr,
-                       tc);
+    const bool x4 = R % 4 == 0 && C % 4 == 0 && ld % 4 == 0 && ((uintptr_t)src & 7) == 0 &&
+                    ((uintptr_t)dst & 7) == 0;
+    hipLaunchKernelGGL(x4 ? corner_turn_x4_kernel : corner_turn_kernel<uint16_t>,
+                       dim3((unsigned)((size_t)nchans * tr * tc)), dim3(256), 0, st,
+                       static_cast<const uint16_t*>(src), static_cast<uint16_t*>(dst), R, C, ld, tr, tc);
+    SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
+int corner_turn_f64(const void* src, void* dst, int nchans, int R, int C, hipStream_t st, int src_ld)
+{
+    const int ld = src_ld > 0 ? src_ld : C;
+    const int tr = (R + 63) / 64, tc = (C + 63) / 64;
+    hipLaunchKernelGGL(corner_turn_kernel<float2>, dim3((unsigned)((size_t)nchans * tr * tc)), dim3(256), 0, st,
+                       static_cast<const float2*>(src), static_cast<float2*>(dst), R, C, ld, tr, tc);
     SW_LAUNCH_CHECK();
     return SW_OK;
 }
@@ -223,7 +237,7 @@ extern "C" int sw_corner_turn(const void* d_src, void* d_dst, int nchans, int ro
 {
     SW_CHECK_ARG(d_src && d_dst && d_src != d_dst, "null or aliased buffer");
     SW_CHECK_ARG(nchans > 0 && rows > 0 && cols > 0, "empty input");
-    return sw::corner_turn(d_src, d_dst, nchans, rows, cols, sw::as_stream(stream));
+    return sw::corner_turn(d_src, d_dst, nchans, rows, cols, sw::as_stream(stream), 0);
 }
 
 extern "C" int sw_statistics_fpat(const void* d_fpat, float* d_stats, int nchans, int nsamples,

@@ -461,8 +461,8 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
     const int nv = p->nv();
     const size_t T2 = p->tout();
     for(int dm = 0; dm < c.ndms; ++dm) {
-        int fl = 0, ov = 0;
-        sw_codedisp_info(p->cd, dm, &fl, &ov);
+        int fl = 0, ov = 0, lat = 0;
+        sw_codedisp_info(p->cd, dm, &fl, &ov, &lat);
         const double tsamp = c.nchans / c.bandwidth;
         SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm], st));
         rc = sw_ib_beamform(p->ftpa_dd[dm], p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, &d, st);
@@ -476,8 +476,9 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
                                 p->cb_out, &d, SW_CB_PLAIN, nullptr, st);
         if(rc) return rc;
         SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm + 2], st));
-        // the dedispersed block is delayed by the filter delay tsamp*O/2
-        const double utc = utc_offset_gulp + tsamp * ov / 2.0;
+        // the dedispersed block carries the filter delay tsamp*O/2 and, when
+        // the FFT hop does not divide the gulp, the dedisperser's latency
+        const double utc = utc_offset_gulp + tsamp * (ov / 2.0 - lat);
         sw_block blk;
         fill_block(blk, SW_BLOCK_CB_TFB, p->cb_out, T2 * F * c.nbeams * nv, "TFB", {T2, (size_t)F, (size_t)c.nbeams});
         blk.dm_idx = dm;

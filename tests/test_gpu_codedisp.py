@@ -1,6 +1,12 @@
 """GPU coherent dedispersion against the FP64 overlap-save oracle and the
 reference's golden vectors.
 
+The dedisperser's segments lie on one grid over the stream of gulps; a gulp
+g emits stream samples [gT - L, (g+1)T - L) with the plan's latency L (0
+when the hop N - O divides T).  The oracle transforms the whole stream with
+the same grid, and each gulp is compared with its window of the oracle's
+output (zeros before the stream start).
+
 Tolerances: dedispersed voltages (y/N before quantisation) within 1e-5
 relative (L2 norm over each DM's output) of the FP64 oracle; int8 outputs
 within +-1 (modulo the int8 wrap) of the oracle, with mismatches confined to
@@ -35,6 +41,16 @@ def _check(got_i8, got_f, exp_i8, exp_f, label):
     assert np.abs(d).max() <= 1, label
     assert frac <= 1e-3, label
     return rel
+
+
+def _window(ei, ef, start, T):
+    """Samples [start, start + T) of stream outputs [F][Ts][...] (zeros before 0)."""
+    F = ei.shape[0]
+    pad = max(0, -start)
+    zi = np.zeros((F, pad) + ei.shape[2:], ei.dtype)
+    zf = np.zeros((F, pad) + ef.shape[2:], ef.dtype)
+    a = max(0, start)
+    return (np.concatenate([zi, ei[:, a:start + T]], 1), np.concatenate([zf, ef[:, a:start + T]], 1))
 
 
 def _oracle_stream(orc, hist, cur, chirps, N, O, nthreads=16):
@@ -77,13 +93,16 @@ def test_codedisp_vs_fp64_oracle(sw, orc, dev, dm, N):
     cur = sw.synth_voltages((F, T, 2, A, 2), seed=0x5EED0002, sigma=16.0)
     plan = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, [dm], fft_len=[N])
     _, O = plan.info(0)
-    print(f"codedisp dm={dm} N={N} O={O}", flush=True)
+    lat = plan.latency(0)
+    assert lat == (0 if T % (N - O) == 0 else N - O)
+    print(f"codedisp dm={dm} N={N} O={O} latency={lat}", flush=True)
     d_cur = DB.from_host(cur)
     out = DB.empty(cur.shape, np.int8, fill=0x55)
     outf = DB.empty(cur.shape, np.float32)
     plan.run(d_cur, [out], [outf])
     chirp = orc.dm_chirp(dm, F_LOW, CHAN_BW, F, N)
     ei, ef = _oracle_stream(orc, np.zeros((F, O, 2, A, 2), np.int8), cur, chirp, N, O)
+    ei, ef = _window(ei, ef, -lat, T)
     _check(out.download(), outf.download(), ei, ef, f"DM {dm} N {N}")
 
 
@@ -95,10 +114,12 @@ def test_codedisp_dm0_is_identity(sw, dev):
                                   overlap=[0, 20])
     outs = [DB.empty(cur.shape, np.int8, fill=0x55) for _ in range(2)]
     plan.run(DB.from_host(cur), outs)
-    np.testing.assert_array_equal(outs[0].download(), cur)
-    # with O = 20 the output is delayed by O/2 (zero history before the first gulp)
-    exp = np.concatenate([np.zeros((F, 10, 2, A, 2), np.int8), cur[:, :T - 10]], 1)
-    np.testing.assert_array_equal(outs[1].download(), exp)
+    # stream sample j is input j - O/2, emitted with the plan's latency
+    for k, O in enumerate((0, 20)):
+        lat = plan.latency(k)
+        sh = lat + O // 2
+        exp = np.concatenate([np.zeros((F, min(sh, T), 2, A, 2), np.int8), cur[:, :max(0, T - sh)]], 1)
+        np.testing.assert_array_equal(outs[k].download(), exp)
 
 
 def test_codedisp_uneven_batch_and_history(sw, orc, dev):
@@ -112,6 +133,8 @@ def test_codedisp_uneven_batch_and_history(sw, orc, dev):
     plan_u = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, fft_len=Ns)
     Os = [plan_b.info(d)[1] for d in range(8)]
     assert Os == [0, 2, 4, 10, 18, 32, 46, 62]
+    lats = [plan_b.latency(d) for d in range(8)]
+    assert lats[0] == 0 and all(lats[d] == Ns[d] - Os[d] for d in range(1, 8))
     g1 = sw.synth_voltages((F, T, 2, A, 2), seed=0x5EED0003, sigma=16.0)
     g2 = sw.synth_voltages((F, T, 2, A, 2), seed=0x5EED0013, sigma=16.0)
     for gi, g in enumerate((g1, g2)):
@@ -129,5 +152,7 @@ def test_codedisp_uneven_batch_and_history(sw, orc, dev):
             np.testing.assert_array_equal(got, ou[d].download())
             if gi == 1 and d in (0, 3, 7):
                 chirp = orc.dm_chirp(dms[d], F_LOW, CHAN_BW, F, Ns[d])
-                ei, ef = _oracle_stream(orc, g1[:, T - Os[d]:], g2, chirp, Ns[d], Os[d])
+                ei, ef = _oracle_stream(orc, np.zeros((F, Os[d], 2, A, 2), np.int8), np.concatenate([g1, g2], 1),
+                                        chirp, Ns[d], Os[d])
+                ei, ef = _window(ei, ef, T - lats[d], T)
                 _check(got, obf[d].download(), ei, ef, f"gulp 2 DM {dms[d]} N {Ns[d]}")

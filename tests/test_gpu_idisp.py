@@ -8,14 +8,15 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _run(sw, idd, tfb, nvec):
+def _run(sw, idd, tfb, nvec, layout="tfb"):
     T, F = tfb.shape[:2]
     B = tfb.shape[2]
     nout = idd.nout(T)
-    d_in = sw.DeviceBuffer.from_host(np.ascontiguousarray(tfb))
+    x = tfb if layout == "tfb" else tfb.swapaxes(0, 1)
+    d_in = sw.DeviceBuffer.from_host(np.ascontiguousarray(x))
     shape = (nout, idd.ndms, B) + ((nvec,) if nvec > 1 else ())
     d_out = sw.DeviceBuffer.empty(shape, np.int8, fill=0x55)
-    idd.dedisperse(d_in, T, B, nvec, d_out)
+    idd.dedisperse(d_in, T, B, nvec, d_out, layout=layout)
     sw.synchronize()
     return d_out.download()
 
@@ -69,10 +70,11 @@ def test_vs_oracle(sw, orc, dev, case):
     tfb = _rand(shape, 100 + case, sigma=40.0)
     tfb.reshape(-1)[::997] = -128  # extremes
     tfb.reshape(-1)[1::991] = 127
-    out = _run(sw, idd, tfb, nvec)
     want = orc.incoherent_dedisperse(tfb, d_o, mx_o, ts)
-    assert out.shape == want.shape
-    np.testing.assert_array_equal(out, want)
+    for layout in ("tfb", "ftb"):
+        out = _run(sw, idd, tfb, nvec, layout)
+        assert out.shape == want.shape
+        np.testing.assert_array_equal(out, want, err_msg=layout)
 
 
 def test_saturating_sums(sw, orc, dev):

@@ -124,6 +124,26 @@ def test_ib_parity_shapes_and_float_weights(sw, orc, dev, A, ts):
     np.testing.assert_array_equal(out, eout)
 
 
+@pytest.mark.parametrize("A,ts,S", [(128, 32, 1), (64, 32, 3)])
+def test_ib_parity_exact_path_overflow(sw, orc, dev, A, ts, S):
+    # Stokes I with 0/1 weights takes the integer path; full-scale voltages
+    # push the totals past 2^24 on some outputs, where the float chain rounds:
+    # those waves must fall back to the oracle's float order
+    print(f"ib overflow A={A} ts={ts} S={S}", flush=True)
+    F, T = 2, 64 * 32 * 2
+    v = _rand_c8((F, T, 2, A), 7, sigma=8.0)    # totals ~1e6: exact path
+    v[0, : T // 2] = -128                       # channel 0, first half: totals ~1e8
+    w = np.ones((S, A), np.float32)
+    if S > 1:
+        w[1:] = (_rng(11).random((S - 1, A)) > 0.5)
+    scale, offset = _ib_scales(F, S, A, ts)
+    raw, out = _run_ib(sw, dev, v, w, scale, offset, ts, "I")
+    eraw, eout = orc.ib_beamform(v, ts, scale, offset, w, S, "I")
+    assert (eraw >= 2 ** 24).any() and (eraw < 2 ** 24).any()
+    np.testing.assert_array_equal(raw, eraw)
+    np.testing.assert_array_equal(out, eout)
+
+
 # --------------------------------------------------------- coherent beamformer
 def _cb_scales(F, S, A, ts, level=32.0):
     # CoherentBeamformerTester.cu:224-242

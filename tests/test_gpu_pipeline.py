@@ -50,21 +50,27 @@ def test_pipeline_end_to_end(sw, orc, dev):
     gpu_stats0 = got["stats"][0][1].view(np.float32).reshape(F, 2, A, 4)
     scal = orc.update_scalings(gpu_stats0[..., 1], antw, 1, TS, TS, 24.0)
     lowf = CFREQ - BW / 2
-    prev = None
+    ftpas = [orc.transpose(g, A) for g in gulps]
+    # DM 50 (N 2048, O 10): hop 2038 does not divide T, so the dedisperser
+    # runs on one segment grid over both gulps and emits each gulp's window
+    # one hop late; the oracle dedisperses the whole stream
+    O, lat = 10, fft_len[1] - 10
+    chirp = orc.dm_chirp(dms[1], lowf, BW / F, F, fft_len[1])
+    stream = np.concatenate(ftpas, 1)
+    zero = np.zeros((O, 2, A, 2), np.int8)
+    dstream = np.stack([orc.codedisp_channel(np.concatenate([zero, stream[f]]), chirp[f], fft_len[1], O,
+                                             stream.shape[1], want_float=False, nthreads=16)[1] for f in range(F)])
+    dstream = np.concatenate([np.zeros((F, lat, 2, A, 2), np.int8), dstream], 1)  # emitted = stream - lat
     exact = []
     for gi, g in enumerate(gulps):
-        ftpa = orc.transpose(g, A)
+        ftpa = ftpas[gi]
         st = got["stats"][gi][1].view(np.float32).reshape(F, 2, A, 4)
         np.testing.assert_allclose(st.astype(np.float64), orc.statistics(ftpa), rtol=1e-5, atol=1e-6)
         for d, dm in enumerate(dms):
             if dm == 0.0:
                 dd = ftpa
             else:
-                O = 10
-                chirp = orc.dm_chirp(dm, lowf, BW / F, F, fft_len[d])
-                hist = np.zeros((F, O, 2, A, 2), np.int8) if prev is None else prev[:, T - O:]
-                dd = np.stack([orc.codedisp_channel(np.concatenate([hist[f], ftpa[f]]), chirp[f], fft_len[d], O, T,
-                                                    want_float=False, nthreads=16)[1] for f in range(F)])
+                dd = np.ascontiguousarray(dstream[:, gi * T:(gi + 1) * T])
             _, ib_q = orc.ib_beamform(dd, TS, scal["ib_scaling"], scal["ib_offsets"], antw, 1, "I")
             cb = orc.cb_beamform(dd, weights, TS, scal["cb_scaling"], scal["cb_offsets"], antw, bmap, 1, "I", True)
             g_cb = got["cb"][2 * gi + d][1].view(np.int8).reshape(cb.shape)
@@ -78,10 +84,9 @@ def test_pipeline_end_to_end(sw, orc, dev):
                 assert np.abs(dcb).max() <= 1 and np.mean(dcb == 0) > 0.995
                 assert np.abs(g_ib.astype(int) - ib_q).max() <= 1
                 exact.append(np.mean(dcb == 0))
-        prev = ftpa
     print(f"DM 50 coherent beams exact fraction: {exact}", flush=True)
-    # metadata: the DM-50 blocks are delayed by the filter delay O/2 samples
-    assert got["cb"][1][2] == pytest.approx(TSAMP * 5, rel=1e-9)
+    # metadata: the DM-50 blocks carry the filter delay O/2 minus the latency
+    assert got["cb"][1][2] == pytest.approx(TSAMP * (5 - lat), rel=1e-9)
     assert got["cb"][1][3] == pytest.approx(TSAMP * TS, rel=1e-9)
 
 

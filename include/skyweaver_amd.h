@@ -217,10 +217,15 @@ typedef struct {
     const float* dms;
     int ndms;
     int tscrunch;
+    int kernel;   /* 0: matrix-core kernel where it applies (nchans % 4 == 0,
+                     16 tscrunch <= 128), else the 16-bit-lane VALU kernel;
+                     1: always the VALU kernel */
 } sw_idisp_config;
 typedef struct sw_idisp sw_idisp;
 int sw_idisp_create(const sw_idisp_config* config, sw_idisp** out);
 int sw_idisp_destroy(sw_idisp* plan);
+/* "idisp_mfma_kernel" or "idisp_kernel": the kernel the plan launches */
+const char* sw_idisp_kernel(const sw_idisp* plan);
 /* max_sample_delay(), delays() [ndms][nchans] and the scale sqrt(F ts);
  * any output may be NULL. */
 int sw_idisp_info(const sw_idisp* plan, int* max_delay, int* h_delays, float* scale);
@@ -261,6 +266,11 @@ typedef struct sw_codedisp_config {
     const int* fft_len;/* [ndms] or NULL */
     const int* overlap;/* [ndms] or NULL */
     int input_fpat;    /* 0: inputs and history are FTPA; 1: time-major FPAT */
+    int history_in_place; /* 1 (with input_fpat): the history is read from the
+                             previous gulp's buffer, which the caller keeps
+                             unchanged until the next sw_codedisp_run; no
+                             history copy.  Ignored when the history spans
+                             more than one gulp. */
 } sw_codedisp_config;
 typedef struct sw_codedisp sw_codedisp;
 

@@ -56,7 +56,7 @@ class BfDims(ctypes.Structure):
 class IdispConfig(ctypes.Structure):
     _fields_ = [("nchans", ctypes.c_int), ("centre_frequency", ctypes.c_double), ("bandwidth", ctypes.c_double),
                 ("chan_freqs", ctypes.c_void_p), ("cb_tscrunch", ctypes.c_int), ("dms", ctypes.c_void_p),
-                ("ndms", ctypes.c_int), ("tscrunch", ctypes.c_int)]
+                ("ndms", ctypes.c_int), ("tscrunch", ctypes.c_int), ("kernel", ctypes.c_int)]
 
 
 class CodedispConfig(ctypes.Structure):
@@ -64,7 +64,7 @@ class CodedispConfig(ctypes.Structure):
                 ("low_freq", ctypes.c_double), ("chan_bw", ctypes.c_double), ("tsamp", ctypes.c_double),
                 ("ndms", ctypes.c_int), ("dms", ctypes.POINTER(ctypes.c_float)),
                 ("fft_len", ctypes.POINTER(ctypes.c_int)), ("overlap", ctypes.POINTER(ctypes.c_int)),
-                ("input_fpat", ctypes.c_int)]
+                ("input_fpat", ctypes.c_int), ("history_in_place", ctypes.c_int)]
 
 
 _vp = ctypes.c_void_p
@@ -130,6 +130,8 @@ def _declare(L):
     L.sw_idisp_create.argtypes = [ctypes.POINTER(IdispConfig), _pvp]
     L.sw_idisp_destroy.argtypes = [_vp]
     L.sw_idisp_info.argtypes = [_vp, ctypes.POINTER(_c), _vp, ctypes.POINTER(ctypes.c_float)]
+    L.sw_idisp_kernel.argtypes = [_vp]
+    L.sw_idisp_kernel.restype = ctypes.c_char_p
     L.sw_idisp_dedisperse.argtypes = [_vp, _vp, _sz, _c, _c, _vp, _vp]
     L.sw_idisp_dedisperse_ftb.argtypes = [_vp, _vp, _sz, _c, _c, _vp, _vp]
 
@@ -403,15 +405,18 @@ def ddplan_parse_block(spec):
 class IncoherentDedisperser:
     """IncoherentDedisperser(config, dms, tscrunch) over TFB device buffers (sw_idisp_* in the C ABI)."""
 
-    def __init__(self, nchans, centre_frequency, bandwidth, cb_tscrunch, dms, tscrunch=1, chan_freqs=None):
+    def __init__(self, nchans, centre_frequency, bandwidth, cb_tscrunch, dms, tscrunch=1, chan_freqs=None,
+                 valu_kernel=False):
+        """valu_kernel: always use the 16-bit-lane VALU kernel instead of the matrix-core one."""
         import numpy as np
         self._dms = np.ascontiguousarray(dms, np.float32)
         self._cf = None if chan_freqs is None else np.ascontiguousarray(chan_freqs, np.float64)
         cfg = IdispConfig(nchans, float(centre_frequency), float(bandwidth),
                           None if self._cf is None else self._cf.ctypes.data, cb_tscrunch, self._dms.ctypes.data,
-                          len(self._dms), tscrunch)
+                          len(self._dms), tscrunch, 1 if valu_kernel else 0)
         h = ctypes.c_void_p()
         check(lib().sw_idisp_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.kernel = lib().sw_idisp_kernel(h).decode()
         self._h = h
         self.nchans, self.ndms, self.tscrunch = nchans, len(self._dms), tscrunch
         md, sc = ctypes.c_int(), ctypes.c_float()
@@ -482,7 +487,10 @@ class CoherentDedisperser:
     """Overlap-save coherent dedispersion plan (sw_codedisp_* in the C ABI)."""
 
     def __init__(self, nchans, nantennas, nsamples, low_freq, chan_bw, tsamp, dms, fft_len=None,
-                 overlap=None, stream=None, input_fpat=False):
+                 overlap=None, stream=None, input_fpat=False, history_in_place=False):
+        """history_in_place (with input_fpat): the history is read from the
+        gulp last given to push_history, which must stay unchanged until the
+        next run."""
         import numpy as np
         self._dms = np.ascontiguousarray(dms, np.float32)
         n = len(self._dms)
@@ -491,7 +499,8 @@ class CoherentDedisperser:
         cfg = CodedispConfig(nchans, nantennas, nsamples, low_freq, chan_bw, tsamp, n,
                              self._dms.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                              self._fft.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                             self._ov.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 1 if input_fpat else 0)
+                             self._ov.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 1 if input_fpat else 0,
+                             1 if history_in_place else 0)
         h = ctypes.c_void_p()
         check(lib().sw_codedisp_create(ctypes.byref(cfg), _stream(stream), ctypes.byref(h)))
         self.handle = h.value

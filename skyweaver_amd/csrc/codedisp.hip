@@ -33,8 +33,12 @@
 #include <vector>
 
 namespace sw {
-int corner_turn(const void* src, void* dst, int nchans, int R, int C, hipStream_t st, int src_ld = 0);
-int corner_turn_f64(const void* src, void* dst, int nchans, int R, int C, hipStream_t st, int src_ld = 0);
+// [F][R][C] -> [F][C][R]; source rows src_ld >= C elements apart, channels
+// R * src_ld apart; destination channels R * dst_chan_cols apart (>= C)
+int corner_turn(const void* src, void* dst, int nchans, int R, int C, hipStream_t st, int src_ld = 0,
+                int dst_chan_cols = 0);
+int corner_turn_f64(const void* src, void* dst, int nchans, int R, int C, hipStream_t st, int src_ld = 0,
+                    int dst_chan_cols = 0);
 
 namespace {
 
@@ -50,12 +54,13 @@ struct CdParams {
     float2* z;           // [F*nseg][RP][PA][L] (N > kMaxL)
     int T, O, Oh, hop, nseg, PA, F, RP, N;
     long in0;   // input index (relative to the gulp start) of segment 0's first sample
-    long y0;    // output-buffer index of segment 0's first kept sample
-    int ylen;   // output-buffer samples per column
+    long y0;    // ring index of segment 0's first kept sample (< ylen)
+    int ylen;   // output ring samples per column
+    const int8_t* prev;  // previous gulp ([F][PA][T]) holding the history, or null (hist used)
     int ndm;
     const float2* chirp[kMaxDm];  // [F][RP][L] (RP == 1: natural order)
-    int8_t* out[kMaxDm];          // [F][PA][ylen] complex int8
-    float2* outf[kMaxDm];         // optional [F][PA][ylen] y/N
+    int8_t* out[kMaxDm];          // [F][PA][ylen] complex int8 output ring
+    float2* outf[kMaxDm];         // optional [F][PA][ylen] y/N ring
     float2* v[kMaxDm];            // [F*nseg][RP][PA][L] (N > kMaxL)
     const float2* tw;             // exp(-2 pi i m / L)
     const float2* twN;            // exp(-2 pi i m / N)
@@ -77,6 +82,7 @@ __device__ __forceinline__ void store_c8(int8_t* p, float2 y)
 // input sample tau (relative to the gulp start) of column col of channel f
 __device__ __forceinline__ float2 input_sample(const CdParams& p, int f, int col, int tau)
 {
+    if(tau < 0 && p.prev) return load_c8(p.prev + (((size_t)f * p.PA + col) * p.T + p.T + tau) * 2);
     if(tau < 0) return load_c8(p.hist + (((size_t)f * p.PA + col) * p.Oh + p.Oh + tau) * 2);
     if(tau < p.T) return load_c8(p.cur + (((size_t)f * p.PA + col) * p.T + tau) * 2);
     return make_float2(0.0f, 0.0f);
@@ -85,6 +91,8 @@ __device__ __forceinline__ float2 input_sample(const CdParams& p, int f, int col
 // raw complex int8 sample (re | im << 8), zero outside history + gulp
 __device__ __forceinline__ uint32_t input_raw(const CdParams& p, int f, int col, int tau)
 {
+    if(tau < 0 && p.prev)
+        return *reinterpret_cast<const uint16_t*>(p.prev + (((size_t)f * p.PA + col) * p.T + p.T + tau) * 2);
     if(tau < 0) return *reinterpret_cast<const uint16_t*>(p.hist + (((size_t)f * p.PA + col) * p.Oh + p.Oh + tau) * 2);
     if(tau < p.T) return *reinterpret_cast<const uint16_t*>(p.cur + (((size_t)f * p.PA + col) * p.T + tau) * 2);
     return 0;
@@ -149,8 +157,9 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
             const int half = p.O / 2;
             for(int i = tid; i < C * p.hop; i += NT) {
                 const int c = i / p.hop, m = i % p.hop;
-                const long j = p.y0 + (long)s * p.hop + m;
-                if(j < p.ylen) {
+                long j = p.y0 + (long)s * p.hop + m;
+                if(j >= p.ylen) j -= p.ylen;  // y0 < ylen and s*hop + m < ylen
+                {
                     const int col = g * C + c;
                     const float2 y = cscale(W[c * padded(L) + pad(half + m)], inv_n);
                     const size_t o = ((size_t)f * p.PA + col) * p.ylen + j;
@@ -163,29 +172,38 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
     }
 }
 
-// N = 16384 held in registers: 512 threads x 32 points, one column per
-// workgroup.  With n = 512 n1 + 16 n2 + n3 and k = k1 + 32 k2 + 1024 k3
-// (n1, n2, k1, k2 < 32; n3, k3 < 16) the forward transform is
-//   DFT32 over n1 (thread t = 16 n2 + n3), x W^(t k1)
-//   -> exchange -> DFT32 over n2 (thread (k1, n3)), x W^(32 n3 k2)
-//   -> exchange -> DFT16 over n3 (thread (k1, k2), two per thread),
-// which leaves X[k] in registers in the order the chirp is read; the
-// inverse runs the same steps backwards with conjugate twiddles and ends
-// with each thread holding output samples 512 n1 + t.  Four LDS exchanges
-// per DM instead of a read-write round trip per radix pass.
-constexpr int k16 = 16384;
-constexpr int kEx2Stride = 17;  // padded row of 16 points (conflict-free reads)
-constexpr int k16Lds = 1024 * kEx2Stride;
+// N = 512 R1 (R1 = 8, 16, 32: N = 4096, 8192, 16384) held in registers:
+// 512 threads x 32 points per unit, a unit being C = 32 / R1 columns of one
+// (channel, segment).  With n = 512 n1 + 16 n2 + n3 and
+// k = k1 + R1 k2 + 32 R1 k3 (n1, k1 < R1; n2, k2 < 32; n3, k3 < 16) the
+// forward transform is
+//   DFT_R1 over n1 (thread t = 16 n2 + n3, each column), x W_N^(t k1)
+//   -> exchange -> DFT32 over n2 (thread (c, k1, n3)), x W_N^(R1 n3 k2)
+//   -> exchange -> DFT16 over n3 (thread (c, k1, k2), two per thread),
+// which leaves X[k] in registers; the chirp is stored permuted to the order
+// the threads read it (index 32 R1 k3 + 32 k1 + k2).  The inverse runs the
+// same steps backwards with conjugate twiddles and ends with each thread
+// holding output samples 512 n1 + t of its columns.  Four LDS exchanges per
+// DM instead of a read-write round trip per radix pass.  (Keeping the
+// spectrum for a second DM would not fit the registers: one DM per launch.)
+constexpr int kEx1Row = 528;   // exchange 1/4 row (c, k1): 512 points + 16 pad
+constexpr int kEx2Row = 560;   // exchange 2/3 block (c, k1): 32 rows of 17
+constexpr int kEx2Stride = 17; // padded row of 16 points
+constexpr int kRegLds = 32 * kEx2Row;
+// Lane groups of a wave that read or write one exchange row at a time start
+// 32 banks apart (528 and 560 float2 are 32 dwords mod 64), and the rows of
+// 17 points put 32 consecutive k2 on distinct banks.
 
-// v[k] *= w1^k for k = 1..R-1 (R = 16 or 32), powers by the product tree of
-// twiddle_powers, built 16 at a time to bound register use
+// v[k] *= w1^k for k = 1..R-1 (R = 8, 16 or 32), powers by the product tree
+// of twiddle_powers, built 16 at a time to bound register use
 template <int R>
 __device__ __forceinline__ void apply_powers(float2* v, float2 w1)
 {
-    float2 w[16];
-    twiddle_powers<16>(w1, w);
+    constexpr int R0 = R < 16 ? R : 16;
+    float2 w[R0];
+    twiddle_powers<R0>(w1, w);
 #pragma unroll
-    for(int k = 1; k < 16; ++k) v[k] = cmul(v[k], w[k]);
+    for(int k = 1; k < R0; ++k) v[k] = cmul(v[k], w[k]);
     if constexpr(R == 32) {
         const float2 w16 = cmul(w[8], w[8]);
         v[16] = cmul(v[16], w16);
@@ -194,117 +212,162 @@ __device__ __forceinline__ void apply_powers(float2* v, float2 w1)
     }
 }
 
-// One DM per launch (d): forward FFT, chirp, inverse FFT of one column per
-// unit u = (channel, segment, column).  Persistent: a workgroup (one per CU,
-// the exchange buffer fills the LDS) walks units u = blockIdx.x + k gridDim.x
-// and loads the next unit's 32 input samples per thread while transforming
-// the current one.
-__global__ void __launch_bounds__(512) codedisp_16k_kernel(CdParams p, int d, int nunits)
+// Persistent: a workgroup (one per CU, the exchange buffer fills the LDS)
+// walks units u = blockIdx.x + k gridDim.x, u = (channel, segment, column
+// group), and loads the next unit's input while transforming the current
+// one.  DM d of the launch's parameter block.
+template <int R1>
+__global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, int nunits)
 {
-    __shared__ float2 lds[k16Lds + TwiddleLds<k16>::SIZE];
-    const Twiddles tw = load_twiddles<k16, 512>(lds + k16Lds, p.tw);
+    constexpr int N = 512 * R1;
+    constexpr int C = 32 / R1;
+    __shared__ float2 lds[kRegLds + TwiddleLds<N>::SIZE];
+    const Twiddles tw = load_twiddles<N, 512>(lds + kRegLds, p.tw);
     const int t = threadIdx.x;
-    const float inv_n = 1.0f / (float)k16;
-    uint32_t nxt[16];  // two raw samples per register
+    const float inv_n = 1.0f / (float)N;
+    const int ngrp = p.PA / C;
+    uint32_t nxt[16];  // two raw samples per register: value i = c R1 + n1
     auto fetch = [&](int u) {
-        const int col = u % p.PA, fs = u / p.PA;
+        const int cg = u % ngrp, fs = u / ngrp;
         const int f = fs / p.nseg, s = fs % p.nseg;
+        const long b = p.in0 + (long)s * p.hop;  // first input sample, relative to the gulp start
 #pragma unroll
-        for(int i = 0; i < 16; ++i)
-            nxt[i] = input_raw(p, f, col, (int)(p.in0 + (long)s * p.hop + 512 * (2 * i) + t)) |
-                     (input_raw(p, f, col, (int)(p.in0 + (long)s * p.hop + 512 * (2 * i + 1) + t)) << 16);
+        for(int c = 0; c < C; ++c) {
+            const size_t cc = (size_t)f * p.PA + cg * C + c;
+            const uint16_t* cur = reinterpret_cast<const uint16_t*>(p.cur) + cc * p.T;
+            uint32_t* dst = nxt + c * (R1 / 2);
+            if(b >= 0 && b + N <= p.T) {  // inside this gulp (uniform branch)
+                const uint16_t* x = cur + b;
+#pragma unroll
+                for(int i = 0; i < R1 / 2; ++i)
+                    dst[i] = (uint32_t)x[1024 * i + t] | ((uint32_t)x[1024 * i + 512 + t] << 16);
+            } else if(b < 0 && b + N <= p.T) {  // history, then this gulp
+                // lo[tau] for tau < 0: the previous gulp's tail or the history buffer
+                const uint16_t* lo = p.prev ? reinterpret_cast<const uint16_t*>(p.prev) + cc * p.T + p.T
+                                            : reinterpret_cast<const uint16_t*>(p.hist) + cc * p.Oh + p.Oh;
+                const int b32 = (int)b;
+#pragma unroll
+                for(int i = 0; i < R1 / 2; ++i) {
+                    const int t0 = b32 + 1024 * i + t, t1 = t0 + 512;
+                    dst[i] = (uint32_t)(t0 < 0 ? lo : cur)[t0] | ((uint32_t)(t1 < 0 ? lo : cur)[t1] << 16);
+                }
+            } else {
+#pragma unroll
+                for(int i = 0; i < R1 / 2; ++i)
+                    dst[i] = input_raw(p, f, cg * C + c, (int)(b + 1024 * i + t)) |
+                             (input_raw(p, f, cg * C + c, (int)(b + 1024 * i + 512 + t)) << 16);
+            }
+        }
     };
+    // thread roles: step 2 (and 2') row j = c R1 + k1 with n3; step 3 pairs
+    // q = t + 512 h -> row q >> 5, k2 = q & 31
+    const int n3 = t & 15, j2 = t >> 4;
+    const int k1_2 = j2 % R1;
     if(blockIdx.x < nunits) fetch(blockIdx.x);
     for(int u = blockIdx.x; u < nunits; u += gridDim.x) {
-    const int col = u % p.PA;
-    const int fs  = u / p.PA;
-    const int f = fs / p.nseg, s = fs % p.nseg;
-    float2 v[32];
+        const int cg = u % ngrp, fs = u / ngrp;
+        const int f = fs / p.nseg, s = fs % p.nseg;
+        float2 v[32];
 
-    // ---- forward, step 1: thread t = (n2, n3) holds n1 = 0..31
+        // ---- forward, step 1: thread t = (n2, n3) holds n1 = 0..R1-1 of each column
 #pragma unroll
-    for(int i = 0; i < 16; ++i) {
-        v[2 * i]     = unpack_c8(nxt[i]);
-        v[2 * i + 1] = unpack_c8(nxt[i] >> 16);
-    }
-    if(u + (int)gridDim.x < nunits) fetch(u + gridDim.x);  // in flight during the transforms
-    __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
-    dft32<false>(v);
-    apply_powers<32>(v, tw(t));
+        for(int i = 0; i < 16; ++i) {
+            v[2 * i]     = unpack_c8(nxt[i]);
+            v[2 * i + 1] = unpack_c8(nxt[i] >> 16);
+        }
+        if(u + (int)gridDim.x < nunits) fetch(u + gridDim.x);  // in flight during the transforms
+        __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
+        {
+            const float2 w = tw(t);
 #pragma unroll
-    for(int k1 = 0; k1 < 32; ++k1) lds[k1 * 512 + t] = v[k1];
-    __syncthreads();
-    // ---- step 2: thread (k1, n3) holds n2 = 0..31
-    {
-        const int k1 = t >> 4, n3 = t & 15;
+            for(int c = 0; c < C; ++c) {
+                dft<R1, false>(v + c * R1);
+                apply_powers<R1>(v + c * R1, w);
+            }
+        }
 #pragma unroll
-        for(int n2 = 0; n2 < 32; ++n2) v[n2] = lds[k1 * 512 + n2 * 16 + n3];
+        for(int i = 0; i < 32; ++i) lds[i * kEx1Row + t] = v[i];  // row i = c R1 + k1
+        __syncthreads();
+        // ---- step 2: thread (j2 = c R1 + k1, n3) holds n2 = 0..31
+#pragma unroll
+        for(int n2 = 0; n2 < 32; ++n2) v[n2] = lds[j2 * kEx1Row + n2 * 16 + n3];
         __syncthreads();
         dft32<false>(v);
-        apply_powers<32>(v, tw(32 * n3));
+        apply_powers<32>(v, tw(R1 * n3));
 #pragma unroll
-        for(int k2 = 0; k2 < 32; ++k2) lds[(k1 * 32 + k2) * kEx2Stride + n3] = v[k2];
-    }
-    __syncthreads();
-    // ---- step 3: pairs q = t, t + 512 (q = 32 k2 + k1): DFT16 over n3 -> k3,
-    // chirp at k = q + 1024 k3, then the inverse step 3': IDFT16 -> n3,
-    // x W^(-32 k2 n3)
-    const float2* H = p.chirp[d] + (size_t)f * k16;
-#pragma unroll
-    for(int h = 0; h < 2; ++h) {
-        const int q = t + 512 * h;
-        const int k1 = q & 31, k2 = q >> 5;
-        float2* x = v + 16 * h;
-#pragma unroll
-        for(int n3 = 0; n3 < 16; ++n3) x[n3] = lds[(k1 * 32 + k2) * kEx2Stride + n3];
-        dft16<false>(x);
-#pragma unroll
-        for(int k3 = 0; k3 < 16; ++k3) x[k3] = cmul(x[k3], H[q + 1024 * k3]);
-        dft16<true>(x);
-        apply_powers<16>(x, cconj(tw(32 * k2)));
-    }
-    __syncthreads();  // all step-3 reads done before the exchange buffer is rewritten
-#pragma unroll
-    for(int h = 0; h < 2; ++h) {
-        const int q = t + 512 * h;
-        const int k1 = q & 31, k2 = q >> 5;
-#pragma unroll
-        for(int n3 = 0; n3 < 16; ++n3) lds[(k1 * 32 + k2) * kEx2Stride + n3] = v[16 * h + n3];
-    }
-    __syncthreads();
-    // ---- step 2': thread (k1, n3) holds k2 = 0..31; IDFT32 -> n2, x W^(-k1 (16 n2 + n3))
-    {
-        const int k1 = t >> 4, n3 = t & 15;
-#pragma unroll
-        for(int k2 = 0; k2 < 32; ++k2) v[k2] = lds[(k1 * 32 + k2) * kEx2Stride + n3];
+        for(int k2 = 0; k2 < 32; ++k2) lds[j2 * kEx2Row + k2 * kEx2Stride + n3] = v[k2];
         __syncthreads();
-        dft32<true>(v);
-        const float2 b = cconj(tw(k1 * n3));
-        apply_powers<32>(v, cconj(tw(16 * k1)));
+        // ---- step 3: pairs q = t, t + 512: DFT16 over n3 -> k3
 #pragma unroll
-        for(int n2 = 0; n2 < 32; ++n2) v[n2] = cmul(v[n2], b);
+        for(int h = 0; h < 2; ++h) {
+            const int q = t + 512 * h;
+            float2* x = v + 16 * h;
 #pragma unroll
-        for(int n2 = 0; n2 < 32; ++n2) lds[k1 * 512 + n2 * 16 + n3] = v[n2];
-    }
-    __syncthreads();
-    // ---- step 1': thread t = (n2, n3) holds k1 = 0..31; IDFT32 -> n1
-#pragma unroll
-    for(int k1 = 0; k1 < 32; ++k1) v[k1] = lds[k1 * 512 + t];
-    dft32<true>(v);
-    int8_t* out = p.out[d];
-    float2* outf = p.outf[d];
-    const int half = p.O / 2;
-#pragma unroll
-    for(int n1 = 0; n1 < 32; ++n1) {
-        const int pos = 512 * n1 + t;
-        const long j = p.y0 + (long)s * p.hop + pos - half;
-        if(pos >= half && pos < k16 - half && j < p.ylen) {
-            const float2 y = cscale(v[n1], inv_n);
-            const size_t o = ((size_t)f * p.PA + col) * p.ylen + j;
-            store_c8(out + o * 2, y);
-            if(outf) outf[o] = y;
+            for(int m = 0; m < 16; ++m) x[m] = lds[(q >> 5) * kEx2Row + (q & 31) * kEx2Stride + m];
+            dft16<false>(x);
         }
-    }
+        {
+            // chirp (permuted: 32 R1 k3 + 32 k1 + k2), then the inverse step 3':
+            // IDFT16 -> n3, x W_N^(-R1 n3 k2)
+            const float2* H = p.chirp[d] + (size_t)f * N;
+#pragma unroll
+            for(int h = 0; h < 2; ++h) {
+                const int q = t + 512 * h;
+                const int k2 = q & 31, k1 = (q >> 5) % R1;
+                float2* x = v + 16 * h;
+#pragma unroll
+                for(int k3 = 0; k3 < 16; ++k3) x[k3] = cmul(x[k3], H[32 * R1 * k3 + 32 * k1 + k2]);
+                dft16<true>(x);
+                apply_powers<16>(x, cconj(tw(R1 * k2)));
+            }
+            __syncthreads();  // all readers of the exchange buffer are done
+#pragma unroll
+            for(int h = 0; h < 2; ++h) {
+                const int q = t + 512 * h;
+#pragma unroll
+                for(int m = 0; m < 16; ++m) lds[(q >> 5) * kEx2Row + (q & 31) * kEx2Stride + m] = v[16 * h + m];
+            }
+            __syncthreads();
+            // ---- step 2': thread (j2, n3) holds k2 = 0..31; IDFT32 -> n2,
+            // x W_N^(-k1 (16 n2 + n3))
+#pragma unroll
+            for(int k2 = 0; k2 < 32; ++k2) v[k2] = lds[j2 * kEx2Row + k2 * kEx2Stride + n3];
+            __syncthreads();
+            dft32<true>(v);
+            {
+                const float2 b = cconj(tw(k1_2 * n3));
+                apply_powers<32>(v, cconj(tw(16 * k1_2)));
+#pragma unroll
+                for(int n2 = 0; n2 < 32; ++n2) v[n2] = cmul(v[n2], b);
+            }
+#pragma unroll
+            for(int n2 = 0; n2 < 32; ++n2) lds[j2 * kEx1Row + n2 * 16 + n3] = v[n2];
+            __syncthreads();
+            // ---- step 1': thread t holds k1 = 0..R1-1 of each column; IDFT_R1 -> n1
+#pragma unroll
+            for(int i = 0; i < 32; ++i) v[i] = lds[i * kEx1Row + t];
+#pragma unroll
+            for(int c = 0; c < C; ++c) dft<R1, true>(v + c * R1);
+            const int half = p.O / 2;
+            const int j0 = (int)(p.y0 + (long)s * p.hop) - half;  // ring index of position 0
+#pragma unroll
+            for(int c = 0; c < C; ++c) {
+                const size_t cb = ((size_t)f * p.PA + cg * C + c) * p.ylen;
+                int8_t* out = p.out[d] + cb * 2;
+                float2* outf = p.outf[d] ? p.outf[d] + cb : nullptr;
+#pragma unroll
+                for(int n1 = 0; n1 < R1; ++n1) {
+                    const int pos = 512 * n1 + t;
+                    int jj = j0 + pos;
+                    if(jj >= p.ylen) jj -= p.ylen;
+                    if(pos >= half && pos < N - half) {
+                        const float2 y = cscale(v[c * R1 + n1], inv_n);
+                        store_c8(out + (size_t)jj * 2, y);
+                        if(outf) outf[jj] = y;
+                    }
+                }
+            }
+        }
     }
 }
 
@@ -353,8 +416,9 @@ __global__ void __launch_bounds__(256) codedisp_dit_kernel(CdParams p, int L, in
     for(int q = 0; q < RP; ++q) {
         const int pos = n + q * L;
         if(pos >= half && pos < p.N - half) {
-            const long j = p.y0 + (long)s * p.hop + pos - half;
-            if(j < p.ylen) {
+            long j = p.y0 + (long)s * p.hop + pos - half;
+            if(j >= p.ylen) j -= p.ylen;
+            {
                 const float2 y = cscale(v[q], inv_n);
                 const size_t o = ((size_t)f * p.PA + col) * p.ylen + j;
                 store_c8(p.out[d] + o * 2, y);
@@ -375,7 +439,9 @@ __global__ void twiddle_kernel(float2* tw, int n)
 }
 
 // DMResponse in FP64 (the only FP64 step), stored as [f][r][m] for k = RP m + r
-__global__ void chirp_kernel(float2* out, int nchans, int N, int RP, double low_freq,
+// R1 > 0: stored in the register kernels' order, k = k1 + R1 k2 + 32 R1 k3 at
+// 32 R1 k3 + 32 k1 + k2; else [RP][N / RP] with k = RP m + r at r N / RP + m
+__global__ void chirp_kernel(float2* out, int nchans, int N, int RP, int R1, double low_freq,
                              double chan_bw, double dm_prefactor)
 {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -388,8 +454,14 @@ __global__ void chirp_kernel(float2* out, int nchans, int N, int RP, double low_
     const double phase = phase_prefactor / ((nu_0 + nu) * nu_0 * nu_0);
     double s, c;
     sincos(phase, &s, &c);
-    const int L = N / RP, m = k / RP, r = k % RP;
-    out[((size_t)chan * RP + r) * L + m] = make_float2((float)c, (float)s);
+    const float2 h = make_float2((float)c, (float)s);
+    if(R1 > 0) {
+        const int k1 = k % R1, k2 = (k / R1) % 32, k3 = k / (32 * R1);
+        out[(size_t)chan * N + 32 * R1 * k3 + 32 * k1 + k2] = h;
+    } else {
+        const int L = N / RP, m = k / RP, r = k % RP;
+        out[((size_t)chan * RP + r) * L + m] = h;
+    }
 }
 
 // new history = the last Oh samples of (old history ++ gulp), per column
@@ -403,35 +475,6 @@ __global__ void history_kernel(int8_t* dst, const int8_t* old, const int8_t* cur
     const uint16_t v = q < Oh ? *reinterpret_cast<const uint16_t*>(old + (col * Oh + q) * 2)
                               : *reinterpret_cast<const uint16_t*>(cur + (col * T + (q - Oh)) * 2);
     *reinterpret_cast<uint16_t*>(dst + idx * 2) = v;
-}
-
-// dst[col][i] = src[col][off + i] for i < n (row pitches sld, dld): carries
-// outputs computed ahead of the emitted window to the front of the next
-// gulp's buffer; when the regions overlap (latency > T) the copy goes
-// through a scratch buffer in two steps
-template <typename E>
-__global__ void carry_kernel(E* dst, const E* src, int ncols, int dld, int sld, int off, int n)
-{
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(idx >= (size_t)ncols * n) return;
-    const int i = (int)(idx % n);
-    const size_t col = idx / n;
-    dst[col * dld + i] = src[col * sld + off + i];
-}
-
-template <typename E>
-int carry(E* y, E* scratch, int ncols, int ylen, int T, int lat, hipStream_t st)
-{
-    const size_t n = (size_t)ncols * lat;
-    const dim3 grid((unsigned)((n + 255) / 256));
-    if(lat <= T) {
-        hipLaunchKernelGGL(carry_kernel<E>, grid, dim3(256), 0, st, y, y, ncols, ylen, ylen, T, lat);
-    } else {
-        hipLaunchKernelGGL(carry_kernel<E>, grid, dim3(256), 0, st, scratch, y, ncols, lat, ylen, T, lat);
-        hipLaunchKernelGGL(carry_kernel<E>, grid, dim3(256), 0, st, y, scratch, ncols, ylen, lat, 0, lat);
-    }
-    SW_LAUNCH_CHECK();
-    return SW_OK;
 }
 
 // ------------------------------------------------------------------ host
@@ -468,17 +511,45 @@ int launch_lds(int L, const CdParams& p, int blocks, hipStream_t st)
         SW_CD(512, 16, 2, 512) SW_CD(1024, 8, 2, 512) SW_CD(2048, 4, 2, 512)
         SW_CD(4096, 2, 2, 512) SW_CD(8192, 1, 2, 512)
     case 16384:
-        if constexpr(F32)
+        if constexpr(F32) {
             hipLaunchKernelGGL((codedisp_tm_kernel<16384, 1, 1, 512, true>), dim3(blocks), dim3(512), 0, st, p);
-        else
-            for(int d = 0; d < p.ndm; ++d)
-                hipLaunchKernelGGL(codedisp_16k_kernel, dim3(std::min(blocks, persistent_blocks())), dim3(512), 0,
-                                   st, p, d, blocks);
+        } else {
+            set_error("codedisp: N = 16384 runs in the register kernel");
+            return SW_ERR_INVALID;
+        }
         break;
     default: set_error("codedisp: unsupported transform length"); return SW_ERR_UNSUPPORTED;
     }
 #undef SW_CD
     SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
+// the register kernels take N = 4096, 8192, 16384 (R1 = N / 512) when the
+// columns split into units of 32 / R1
+int reg_r1(int N, int RP, int PA)
+{
+    if(RP != 1 || (N != 4096 && N != 8192 && N != 16384)) return 0;
+    const int R1 = N / 512;
+    return PA % (32 / R1) == 0 ? R1 : 0;
+}
+
+// units = (channel, segment, group of 32 / R1 columns); one DM per launch
+int launch_reg(int R1, const CdParams& p, int nunits, hipStream_t st)
+{
+    const dim3 grid((unsigned)std::min(nunits, persistent_blocks()));
+    for(int d = 0; d < p.ndm; ++d) {
+#define SW_REG(RR)                                                                              \
+    case RR:                                                                                    \
+        hipLaunchKernelGGL((codedisp_reg_kernel<RR>), grid, dim3(512), 0, st, p, d, nunits);  \
+        break;
+        switch(R1) {
+            SW_REG(8) SW_REG(16) SW_REG(32)
+        default: set_error("codedisp: no register kernel for this length"); return SW_ERR_INVALID;
+        }
+#undef SW_REG
+        SW_LAUNCH_CHECK();
+    }
     return SW_OK;
 }
 
@@ -507,17 +578,17 @@ struct sw_codedisp {
     int8_t* hist = nullptr;              // [F][PA][Oh] complex int8
     int8_t* hist2 = nullptr;             // the other history buffer (ping-pong)
     int8_t* xtm  = nullptr;              // time-major copy of an FTPA input
-    std::vector<int8_t*> ytm;            // per DM: [F][PA][T + lat] time-major outputs
-    std::vector<float2*> ytf;            // per DM, on demand: float outputs, same layout
-    std::vector<int> ylen;               // per DM: T + lat
+    std::vector<int8_t*> ytm;            // per DM: output ring [F][PA][ylen], time-major
+    std::vector<float2*> ytf;            // per DM, on demand: float output ring, same layout
+    std::vector<int> ylen;               // per DM ring length: T + lat rounded up to 4
+    bool hist_ref = false;               // history read in place from the previous gulp
+    const int8_t* prev = nullptr;        // that gulp (null before the first)
     std::vector<int> lat;                // per DM output latency
     long long gulps = 0;                 // gulps completed (push_history calls)
     float2* zbuf = nullptr;              // DIF stage output (N > kMaxL)
     std::vector<float2*> vbuf;           // per DM slot (N > kMaxL)
-    void* carry_tmp = nullptr;           // carries with latency > T
     ~sw_codedisp()
     {
-        (void)hipFree(carry_tmp);
         for(auto* c : chirp) (void)hipFree(c);
         for(auto& kv : twiddles) (void)hipFree(kv.second);
         (void)hipFree(hist);
@@ -576,6 +647,7 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
     p->PA = 2 * cfg->nantennas;
     p->T = cfg->nsamples;
     p->input_fpat = cfg->input_fpat;
+    p->hist_ref = cfg->history_in_place && cfg->input_fpat;
     p->low_freq = cfg->low_freq;
     p->chan_bw = cfg->chan_bw;
     p->tsamp = cfg->tsamp;
@@ -645,6 +717,7 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
     p->chirp.resize(p->dms.size(), nullptr);
     for(int d = 0; d < (int)p->dms.size(); ++d) {
         const int N = p->N[d], RP = N / std::min(N, sw::kMaxL);
+        const int R1 = sw::reg_r1(N, RP, p->PA);
         if(hipMalloc(&p->chirp[d], sizeof(float2) * (size_t)p->F * N) != hipSuccess) {
             sw::set_error("codedisp: chirp allocation failed");
             return SW_ERR_ALLOC;
@@ -653,14 +726,16 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
         const double pref = -1.0f * (2 * 3.1415926535897932385) * sw::kDmConst * (double)p->dms[d];
         const size_t n = (size_t)p->F * N;
         hipLaunchKernelGGL(sw::chirp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                           p->chirp[d], p->F, N, RP, p->low_freq, p->chan_bw, pref);
+                           p->chirp[d], p->F, N, RP, R1, p->low_freq, p->chan_bw, pref);
         SW_LAUNCH_CHECK();
     }
     const size_t vol = (size_t)p->F * p->PA * p->T * 2;
+    if(p->hist_ref && p->Oh > p->T) p->hist_ref = false;  // history reaches past the previous gulp
     if(p->Oh > 0) {
+        // with in-place history only the zeros before the first gulp are needed
         const size_t hb = (size_t)p->F * p->PA * p->Oh * 2;
         SW_HIP_TRY(hipMalloc(&p->hist, hb));
-        SW_HIP_TRY(hipMalloc(&p->hist2, hb));
+        if(!p->hist_ref) SW_HIP_TRY(hipMalloc(&p->hist2, hb));
         SW_HIP_TRY(hipMemsetAsync(p->hist, 0, hb, st));
     }
     if(!p->input_fpat) SW_HIP_TRY(hipMalloc(&p->xtm, vol));
@@ -672,11 +747,6 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
         SW_HIP_TRY(hipMemsetAsync(p->ytm[d], 0, yb, st));  // outputs before the stream start are zero
     }
     (void)slots;
-    int latmax_over_T = 0;
-    for(int l: p->lat)
-        if(l > p->T) latmax_over_T = std::max(latmax_over_T, l);
-    if(latmax_over_T > 0)
-        SW_HIP_TRY(hipMalloc(&p->carry_tmp, (size_t)p->F * p->PA * latmax_over_T * sizeof(float2)));
     if(zmax) {
         SW_HIP_TRY(hipMalloc(&p->zbuf, zmax));
         p->vbuf.resize(vslots, nullptr);
@@ -710,6 +780,10 @@ extern "C" int sw_codedisp_set_history(sw_codedisp* plan, const void* d_hist, vo
 {
     SW_CHECK_ARG(plan && d_hist, "null argument");
     if(plan->Oh == 0) return SW_OK;
+    if(plan->hist_ref) {
+        sw::set_error("sw_codedisp_set_history: not available with history_in_place");
+        return SW_ERR_UNSUPPORTED;
+    }
     hipStream_t st = sw::as_stream(stream);
     if(plan->input_fpat) {
         SW_HIP_TRY(hipMemcpyAsync(plan->hist, d_hist, (size_t)plan->F * plan->PA * plan->Oh * 2,
@@ -733,22 +807,13 @@ extern "C" int sw_codedisp_push_history(sw_codedisp* plan, const void* d_in, voi
     SW_CHECK_ARG(plan && d_in, "null argument");
     hipStream_t st = sw::as_stream(stream);
     const int ncols = plan->F * plan->PA;
-    // outputs computed ahead of this gulp's window move to the front
-    for(size_t d = 0; d < plan->dms.size(); ++d) {
-        const int lat = plan->lat[d];
-        if(lat == 0) continue;
-        int rc = sw::carry(reinterpret_cast<uint16_t*>(plan->ytm[d]), static_cast<uint16_t*>(plan->carry_tmp),
-                           ncols, plan->ylen[d], plan->T, lat, st);
-        if(rc) return rc;
-        if(plan->ytf[d]) {
-            rc = sw::carry(plan->ytf[d], static_cast<float2*>(plan->carry_tmp), ncols, plan->ylen[d], plan->T, lat,
-                           st);
-            if(rc) return rc;
-        }
-    }
     for(auto& g : plan->groups) g.mdone = ((long long)(plan->gulps + 1) * plan->T) / g.hop;
     ++plan->gulps;
     if(plan->Oh == 0) return SW_OK;
+    if(plan->hist_ref) {  // the caller keeps this gulp intact until the next run
+        plan->prev = static_cast<const int8_t*>(d_in);
+        return SW_OK;
+    }
     // an FTPA input was turned into xtm by the run of this gulp
     const int8_t* cur = plan->input_fpat ? static_cast<const int8_t*>(d_in) : plan->xtm;
     const size_t n = (size_t)ncols * plan->Oh;
@@ -795,8 +860,11 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const*
             p.hop = g.hop;
             p.nseg = std::max(nseg, 0);
             p.in0 = (long)(m0 * g.hop - g.O - gT);
-            p.y0 = (long)(m0 * g.hop - gT + g.lat);
-            p.ylen = plan->ylen[g.dms[0]];
+            const long long R = plan->ylen[g.dms[0]];
+            p.ylen = (int)R;
+            // ring index of stream output x is (x + lat) mod R
+            p.y0 = (long)((m0 * g.hop + g.lat) % R);
+            p.prev = plan->hist_ref ? plan->prev : nullptr;
             p.PA = plan->PA;
             p.F = plan->F;
             p.RP = g.RP;
@@ -819,8 +887,12 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const*
                 if(g.RP > 1) p.v[k] = plan->vbuf[k];
             }
             const int G = plan->PA / sw::lds_shape(g.L).C;
+            const int R1 = sw::reg_r1(g.N, g.RP, plan->PA);
             if(p.nseg == 0) {
                 // no segment completes in this gulp
+            } else if(R1 > 0) {
+                rc = sw::launch_reg(R1, p, plan->F * p.nseg * (plan->PA / (32 / R1)), st);
+                if(rc) return rc;
             } else if(g.RP == 1) {
                 rc = sw::launch_lds<false>(g.L, p, plan->F * p.nseg * G, st);
                 if(rc) return rc;
@@ -847,15 +919,31 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const*
                     SW_LAUNCH_CHECK();
                 }
             }
-            // emitted window [0, T) of the time-major result -> FTPA for the beamformers
+            // emitted window, ring [gT mod R, +T) (two pieces when it wraps),
+            // -> FTPA for the beamformers
             for(int k = 0; k < nb; ++k) {
                 const int d = todo[b0 + k];
-                rc = sw::corner_turn(plan->ytm[d], d_out[d], plan->F, plan->PA, plan->T, st, plan->ylen[d]);
+                const long long R = plan->ylen[d];
+                const int start = (int)(gT % R);
+                const int n1 = (int)std::min<long long>(plan->T, R - start), n2 = plan->T - n1;
+                const size_t PA = plan->PA;
+                rc = sw::corner_turn(plan->ytm[d] + (size_t)start * 2, d_out[d], plan->F, plan->PA, n1, st,
+                                     (int)R, plan->T);
                 if(rc) return rc;
-                if(p.outf[k]) {
-                    rc = sw::corner_turn_f64(plan->ytf[d], d_out_f32[d], plan->F, plan->PA, plan->T, st,
-                                             plan->ylen[d]);
+                if(n2 > 0) {
+                    rc = sw::corner_turn(plan->ytm[d], static_cast<int8_t*>(d_out[d]) + (size_t)n1 * PA * 2, plan->F,
+                                         plan->PA, n2, st, (int)R, plan->T);
                     if(rc) return rc;
+                }
+                if(p.outf[k]) {
+                    rc = sw::corner_turn_f64(plan->ytf[d] + start, d_out_f32[d], plan->F, plan->PA, n1, st, (int)R,
+                                             plan->T);
+                    if(rc) return rc;
+                    if(n2 > 0) {
+                        rc = sw::corner_turn_f64(plan->ytf[d], static_cast<float2*>(d_out_f32[d]) + (size_t)n1 * PA,
+                                                 plan->F, plan->PA, n2, st, (int)R, plan->T);
+                        if(rc) return rc;
+                    }
                 }
             }
         }

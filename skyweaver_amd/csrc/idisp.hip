@@ -20,6 +20,15 @@
 // bytes), i.e. four beam sums per two integer adds.  The 16-bit lanes hold
 // at most 256 biased bytes; longer sums (nchans * ts > 256) are flushed into
 // 32-bit accumulators.
+//
+// Matrix-core form (idisp_mfma_kernel, used whenever nchans % 4 == 0 and the
+// rows of one DM fit the stage): one DM's channel sum is a chain of
+// v_mfma_i32_16x16x64_i8 products.  The B operand carries data: lane
+// (n, g) = (l % 16, l / 16) holds the 16 beam bytes [b0, b0 + 16) of channel
+// f0 + g at the input row of output sample n.  The A operand is a constant
+// selector, row m holding a single 1 at byte m of every k-group, so
+// C[m][n] = sum over g of beam b0 + m of channel f0 + g at sample n: four
+// channels x 16 beams x 16 samples per instruction, summed exactly in int32.
 #include "sw_common.hpp"
 
 #include <algorithm>
@@ -29,6 +38,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 struct sw_idisp {
@@ -40,6 +50,7 @@ struct sw_idisp {
     // device tiling
     int dpw = 2, tt = 16, rows_cap = 0;
     bool wide = false;
+    bool mfma = false;                // matrix-core kernel (else the 16-bit-lane one)
     std::vector<int2> tiles;          // (first DM, count)
     int* d_delays = nullptr;
     int2* d_span = nullptr;           // [ntiles][F] (delta_min, delta_max)
@@ -225,6 +236,205 @@ idisp_kernel(const uint32_t* __restrict__ tfb, uint32_t* __restrict__ tdb,
 
 constexpr int kRowsCap = 256;  // staged rows per channel: 64 KB of LDS
 
+// ---------------------------------------------------------------- MFMA form
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int kMfWaves = 8;    // waves per workgroup
+constexpr int kMfDpw   = 2;    // DMs per wave
+constexpr int kMfRows  = 128;  // staged rows per channel
+constexpr int kMfPitch = 144;  // bytes between staged rows: 128 B of beams + 16;
+                               // 16 consecutive rows start on distinct banks
+
+template <typename P>
+__device__ __forceinline__ P zero_piece()
+{
+    if constexpr(std::is_same<P, uint4>::value)
+        return make_uint4(0, 0, 0, 0);
+    else
+        return 0u;
+}
+
+// A workgroup of NW waves owns TT = 16 NTT output samples, 128 bytes of beams
+// (a chunk) and a tile of up to NW * DPW consecutive DMs (wave w takes the
+// tile's DMs w, w + NW, ...).  Per quad of channels it stages into LDS the
+// union of the rows its DMs read, [in0 - max delay, in0 + TT ts - min delay)
+// per channel, while the rows of the next two quads are loaded into
+// registers.  Input byte (t, f, b) lives at t * stride_t + f * stride_f + b
+// (TFB or FTB).
+//
+// Work items are dealt to the 8 XCDs in contiguous runs, and a run is walked
+// in groups of kIdTbGroup time blocks x kIdTileGroup DM tiles of one chunk:
+// the 32 workgroups that one XCD runs together read overlapping rows (next
+// time blocks; DM tiles of neighbouring delays), which its L2 then serves.
+constexpr int kIdTbGroup = 8, kIdTileGroup = 4;
+
+template <int NW, int DPW, int NTT, int RCAP, bool VEC16>
+__global__ void __launch_bounds__(NW * 64)
+idisp_mfma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const int* __restrict__ delays,
+                  const int2* __restrict__ span, const int2* __restrict__ tiles, int ntiles, int nchunks,
+                  int ntb, long w0, long nwork, int nchans, int W, int ndms, int ts, int max_delay,
+                  long nsamples, int nout, float scale, long stride_t, long stride_f)
+{
+    constexpr int NT    = NW * 64;
+    constexpr int TT    = 16 * NTT;
+    constexpr int PIECE = VEC16 ? 16 : 4;  // bytes per staging load
+    constexpr int PPR   = 128 / PIECE;     // loads per staged row
+    static_assert((RCAP * PPR) % NT == 0, "a thread's loads must stay in one channel each");
+    constexpr int NPF = 4 * RCAP * PPR / NT;
+    using P = typename std::conditional<VEC16, uint4, uint32_t>::type;
+    __shared__ __attribute__((aligned(16))) int8_t stage[4 * RCAP * kMfPitch];
+
+    const long per = (nwork + 7) / 8;
+    const long wl  = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if(wl >= nwork) return;  // whole workgroup, before any barrier
+    const long w = w0 + wl;
+    constexpr int GRP = kIdTbGroup * kIdTileGroup;
+    const int gi = (int)(w % GRP);
+    const long rest  = w / GRP;
+    const int chunk  = (int)(rest % nchunks);
+    const long blk   = rest / nchunks;
+    const int ntbb   = (ntb + kIdTbGroup - 1) / kIdTbGroup;
+    const int tb     = (int)(blk % ntbb) * kIdTbGroup + gi % kIdTbGroup;
+    const int tile   = (int)(blk / ntbb) * kIdTileGroup + gi / kIdTbGroup;
+    if(tb >= ntb || tile >= ntiles) return;  // whole workgroup, before any barrier
+    const int2 td    = tiles[tile];
+    const int t0     = tb * TT;
+    const long in0   = (long)max_delay + (long)t0 * ts;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n = lane & 15, g = lane >> 4;
+    const int b0 = chunk * 128;
+    const int nq = nchans / 4;
+    const int2* tspan = span + (size_t)tile * nchans;
+
+    P preA[NPF], preB[NPF];
+    auto fetch = [&](int q, P (&pre)[NPF]) {
+#pragma unroll
+        for(int k = 0; k < NPF; ++k) {
+            const int s = threadIdx.x + NT * k;
+            const int j = (NT * k) / (RCAP * PPR);  // compile-time: NT divides RCAP * PPR
+            const int r = (s / PPR) % RCAP, c = s % PPR;
+            const int2 spj = tspan[4 * q + j];
+            const long row = in0 - spj.y + r;
+            const int b = b0 + c * PIECE;
+            P v = zero_piece<P>();
+            if(r < TT * ts + spj.y - spj.x && b < W && row < nsamples)
+                v = *reinterpret_cast<const P*>(in + row * stride_t + (long)(4 * q + j) * stride_f + b);
+            pre[k] = v;
+        }
+    };
+    auto stash = [&](const P (&pre)[NPF]) {
+#pragma unroll
+        for(int k = 0; k < NPF; ++k) {
+            const int s = threadIdx.x + NT * k;
+            const int j = (NT * k) / (RCAP * PPR);
+            const int r = (s / PPR) % RCAP, c = s % PPR;
+            *reinterpret_cast<P*>(stage + (j * RCAP + r) * kMfPitch + c * PIECE) = pre[k];
+        }
+    };
+    // this lane's channel of quad q: its span and the delays of the wave's DMs
+    auto lane_delays = [&](int q, int2& spg, int (&dly)[DPW]) {
+        spg = tspan[4 * q + g];
+#pragma unroll
+        for(int jd = 0; jd < DPW; ++jd) {
+            const int dl = wave + NW * jd;
+            dly[jd] = dl < td.y ? delays[(size_t)(td.x + dl) * nchans + 4 * q + g] : spg.y;
+        }
+    };
+
+    v4i sel = {0, 0, 0, 0};
+    sel[n >> 2] = 1 << (8 * (n & 3));
+    v4i acc[DPW][NTT][8];
+#pragma unroll
+    for(int jd = 0; jd < DPW; ++jd)
+#pragma unroll
+        for(int tt = 0; tt < NTT; ++tt)
+#pragma unroll
+            for(int bg = 0; bg < 8; ++bg) acc[jd][tt][bg] = v4i{0, 0, 0, 0};
+
+    auto products = [&](int2 spc, const int (&dc)[DPW]) {
+#pragma unroll
+        for(int jd = 0; jd < DPW; ++jd) {
+            if(wave + NW * jd >= td.y) continue;  // wave-uniform
+            const int8_t* base = stage + (g * RCAP + spc.y - dc[jd] + n * ts) * kMfPitch;
+#pragma unroll
+            for(int tt = 0; tt < NTT; ++tt)
+                for(int s = 0; s < ts; ++s) {
+                    const int8_t* rp = base + (tt * 16 * ts + s) * kMfPitch;
+#pragma unroll
+                    for(int bg = 0; bg < 8; ++bg) {
+                        const v4i b = *reinterpret_cast<const v4i*>(rp + bg * 16);
+                        acc[jd][tt][bg] = __builtin_amdgcn_mfma_i32_16x16x64_i8(sel, b, acc[jd][tt][bg], 0, 0, 0);
+                    }
+                }
+        }
+    };
+
+    // quads alternate between two register sets: A holds even quads, B odd
+    int2 spA, spB;
+    int dA[DPW], dB[DPW];
+    fetch(0, preA);
+    lane_delays(0, spA, dA);
+    if(nq > 1) {
+        fetch(1, preB);
+        lane_delays(1, spB, dB);
+    }
+    for(int q = 0; q < nq; q += 2) {
+        {
+            __syncthreads();  // the previous quad's reads are done
+            stash(preA);
+            __syncthreads();
+            const int2 spc = spA;
+            int dc[DPW];
+#pragma unroll
+            for(int jd = 0; jd < DPW; ++jd) dc[jd] = dA[jd];
+            if(q + 2 < nq) {  // in flight during two quads' products
+                fetch(q + 2, preA);
+                lane_delays(q + 2, spA, dA);
+            }
+            products(spc, dc);
+        }
+        if(q + 1 < nq) {
+            __syncthreads();
+            stash(preB);
+            __syncthreads();
+            const int2 spc = spB;
+            int dc[DPW];
+#pragma unroll
+            for(int jd = 0; jd < DPW; ++jd) dc[jd] = dB[jd];
+            if(q + 3 < nq) {
+                fetch(q + 3, preB);
+                lane_delays(q + 3, spB, dB);
+            }
+            products(spc, dc);
+        }
+    }
+
+    // C lane (n, g), register r: beam b0 + 16 bg + 4 g + r at sample t0 + 16 tt + n
+#pragma unroll
+    for(int jd = 0; jd < DPW; ++jd) {
+        const int dl = wave + NW * jd;
+        if(dl >= td.y) continue;
+#pragma unroll
+        for(int tt = 0; tt < NTT; ++tt) {
+            const int t = t0 + 16 * tt + n;
+            if(t >= nout) continue;
+            int8_t* o = tdb + ((size_t)t * ndms + td.x + dl) * W;
+#pragma unroll
+            for(int bg = 0; bg < 8; ++bg) {
+                const int b = b0 + 16 * bg + 4 * g;
+                if(b >= W) continue;
+                const v4i a = acc[jd][tt][bg];
+                const uint32_t packed = (uint32_t)(uint8_t)id_quant(a[0], scale) |
+                                        ((uint32_t)(uint8_t)id_quant(a[1], scale) << 8) |
+                                        ((uint32_t)(uint8_t)id_quant(a[2], scale) << 16) |
+                                        ((uint32_t)(uint8_t)id_quant(a[3], scale) << 24);
+                *reinterpret_cast<uint32_t*>(o + b) = packed;
+            }
+        }
+    }
+}
+
 void free_plan_device(sw_idisp* p)
 {
     (void)hipFree(p->d_delays);
@@ -344,15 +554,22 @@ extern "C" int sw_idisp_create(const sw_idisp_config* cfg, sw_idisp** out)
 
     // tiling: TT output samples per workgroup, tiles of consecutive DMs whose
     // union of read rows fits the LDS stage
-    p->wide = (long)p->nchans * p->tscrunch > 256;
-    p->tt = p->wide ? 8 : 16;
-    if(p->tt * p->tscrunch > sw::kRowsCap / 2) p->tt = 4;
-    if(p->tt * p->tscrunch > sw::kRowsCap) {
-        sw::set_error("incoherent dedispersion tscrunch too large (at most 64)");
-        return SW_ERR_UNSUPPORTED;
+    int cap = sw::kRowsCap, dmax_tile = 8;
+    p->mfma = cfg->kernel == 0 && p->nchans % 4 == 0 && 16 * p->tscrunch <= sw::kMfRows;
+    if(p->mfma) {
+        p->tt = 32 * p->tscrunch <= sw::kMfRows / 2 ? 32 : 16;
+        cap = sw::kMfRows;
+        dmax_tile = sw::kMfWaves * sw::kMfDpw;
+    } else {
+        p->wide = (long)p->nchans * p->tscrunch > 256;
+        p->tt = p->wide ? 8 : 16;
+        if(p->tt * p->tscrunch > sw::kRowsCap / 2) p->tt = 4;
+        if(p->tt * p->tscrunch > sw::kRowsCap) {
+            sw::set_error("incoherent dedispersion tscrunch too large (at most 64)");
+            return SW_ERR_UNSUPPORTED;
+        }
+        p->dpw = 2;
     }
-    p->dpw = 2;
-    const int dmax_tile = 4 * p->dpw;
     const int F = p->nchans;
     std::vector<int2> span;
     for(int d = 0; d < p->ndms;) {
@@ -367,7 +584,7 @@ extern "C" int sw_idisp_create(const sw_idisp_config* cfg, sw_idisp** out)
                 }
                 worst = std::max(worst, hi - lo);
             }
-            if(worst + p->tt * p->tscrunch > sw::kRowsCap) break;
+            if(worst + p->tt * p->tscrunch > cap) break;
             ++n;
         }
         p->tiles.push_back(make_int2(d, n));
@@ -401,6 +618,12 @@ extern "C" int sw_idisp_destroy(sw_idisp* p)
         delete p;
     }
     return SW_OK;
+}
+
+extern "C" const char* sw_idisp_kernel(const sw_idisp* p)
+{
+    if(!p) return "";
+    return p->mfma ? "idisp_mfma_kernel" : "idisp_kernel";
 }
 
 extern "C" int sw_idisp_info(const sw_idisp* p, int* max_delay, int* h_delays, float* scale)
@@ -452,6 +675,39 @@ int sw::idisp_launch(sw_idisp* p, const void* d_tfb, size_t nsamples, int nbeams
     const size_t lds = (size_t)p->rows_cap * 256;
     const long stride_t = ftb ? nwords : (long)p->nchans * nwords;
     const long stride_f = ftb ? (long)nsamples * nwords : nwords;
+    if(p->mfma) {
+        const long nchunks128 = (nbv + 127) / 128;
+        const long ntb128 = (nout + p->tt - 1) / p->tt;
+        const long ntiles = (long)p->tiles.size();
+        const long nwork = (ntiles + sw::kIdTileGroup - 1) / sw::kIdTileGroup * sw::kIdTileGroup * nchunks128 *
+                           ((ntb128 + sw::kIdTbGroup - 1) / sw::kIdTbGroup * sw::kIdTbGroup);
+        const long max_launch = 8L << 20;  // work items per launch (grid x threads < 2^32)
+        const bool v16 = nbv % 16 == 0 && stride_t % 4 == 0 && stride_f % 4 == 0;  // strides in words
+        for(long w0 = 0; w0 < nwork; w0 += max_launch) {
+            const long nw = std::min(max_launch, nwork - w0);
+            const unsigned nblk = (unsigned)(8 * ((nw + 7) / 8));
+#define SW_IDM_LAUNCH(NTT, V16)                                                                              \
+    hipLaunchKernelGGL((sw::idisp_mfma_kernel<sw::kMfWaves, sw::kMfDpw, NTT, sw::kMfRows, V16>), dim3(nblk), \
+                       dim3(sw::kMfWaves * 64), 0, st, static_cast<const int8_t*>(d_tfb),                   \
+                       static_cast<int8_t*>(d_tdb), p->d_delays, p->d_span, p->d_tiles, (int)ntiles,          \
+                       (int)nchunks128, (int)ntb128, w0, nw, p->nchans, (int)nbv, p->ndms, p->tscrunch, p->max_delay,      \
+                       (long)nsamples, nout, p->scale, stride_t * 4, stride_f * 4)
+            if(p->tt == 32) {
+                if(v16)
+                    SW_IDM_LAUNCH(2, true);
+                else
+                    SW_IDM_LAUNCH(2, false);
+            } else {
+                if(v16)
+                    SW_IDM_LAUNCH(1, true);
+                else
+                    SW_IDM_LAUNCH(1, false);
+            }
+#undef SW_IDM_LAUNCH
+            SW_LAUNCH_CHECK();
+        }
+        return SW_OK;
+    }
     const bool pf = p->rows_cap <= sw::kPfRows && (nwords & 3) == 0;
     for(int tb0 = 0; tb0 < ntb; tb0 += 65535) {
         const dim3 grid((unsigned)p->tiles.size(), (unsigned)nchunks, (unsigned)std::min(65535, ntb - tb0));

@@ -49,10 +49,11 @@ taftp_to_fpat_kernel(const uint32_t* __restrict__ in, uint8_t* __restrict__ out,
 }
 
 // [F][R][C] -> [F][C][R] of E (16-bit complex int8 or 8-byte complex
-// float) elements, 64 x 64 tiles; source rows are ld >= C elements apart
+// float) elements, 64 x 64 tiles; source rows are ld >= C elements apart,
+// destination channels R * dcc >= R * C elements apart
 template <typename E>
 __global__ void __launch_bounds__(256)
-corner_turn_kernel(const E* __restrict__ src, E* __restrict__ dst, int R, int C, int ld,
+corner_turn_kernel(const E* __restrict__ src, E* __restrict__ dst, int R, int C, int ld, int dcc,
                    int tiles_r, int tiles_c)
 {
     __shared__ E tile[64][65];
@@ -62,7 +63,7 @@ corner_turn_kernel(const E* __restrict__ src, E* __restrict__ dst, int R, int C,
     const int tr = rem / tiles_c, tc = rem % tiles_c;
     const int r0 = tr * 64, c0 = tc * 64;
     const E* s = src + (size_t)f * R * ld;
-    E* d       = dst + (size_t)f * R * C;
+    E* d       = dst + (size_t)f * R * dcc;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     for(int i = ty; i < 64; i += 4) {
         const int r = r0 + i, c = c0 + tx;
@@ -80,7 +81,7 @@ corner_turn_kernel(const E* __restrict__ src, E* __restrict__ dst, int R, int C,
 // consecutive rows of a column.
 __global__ void __launch_bounds__(256)
 corner_turn_x4_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, int R, int C, int ld,
-                      int tiles_r, int tiles_c)
+                      int dcc, int tiles_r, int tiles_c)
 {
     __shared__ uint16_t tile[64][68];  // 136-byte rows: 8-byte aligned, banks staggered
     const int bid = blockIdx.x;
@@ -89,7 +90,7 @@ corner_turn_x4_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ d
     const int tr = rem / tiles_c, tc = rem % tiles_c;
     const int r0 = tr * 64, c0 = tc * 64;
     const uint16_t* s = src + (size_t)f * R * ld;
-    uint16_t* d       = dst + (size_t)f * R * C;
+    uint16_t* d       = dst + (size_t)f * R * dcc;
     const int g = threadIdx.x & 15, q = threadIdx.x >> 4;
 #pragma unroll
     for(int k = 0; k < 4; ++k) {
@@ -184,25 +185,29 @@ constexpr int kStatChunks = 8;
 
 } // namespace
 
-int corner_turn(const void* src, void* dst, int nchans, int R, int C, hipStream_t st, int src_ld)
+int corner_turn(const void* src, void* dst, int nchans, int R, int C, hipStream_t st, int src_ld,
+                int dst_chan_cols)
 {
     const int ld = src_ld > 0 ? src_ld : C;
+    const int dcc = dst_chan_cols > 0 ? dst_chan_cols : C;
     const int tr = (R + 63) / 64, tc = (C + 63) / 64;
     const bool x4 = R % 4 == 0 && C % 4 == 0 && ld % 4 == 0 && ((uintptr_t)src & 7) == 0 &&
                     ((uintptr_t)dst & 7) == 0;
     hipLaunchKernelGGL(x4 ? corner_turn_x4_kernel : corner_turn_kernel<uint16_t>,
                        dim3((unsigned)((size_t)nchans * tr * tc)), dim3(256), 0, st,
-                       static_cast<const uint16_t*>(src), static_cast<uint16_t*>(dst), R, C, ld, tr, tc);
+                       static_cast<const uint16_t*>(src), static_cast<uint16_t*>(dst), R, C, ld, dcc, tr, tc);
     SW_LAUNCH_CHECK();
     return SW_OK;
 }
 
-int corner_turn_f64(const void* src, void* dst, int nchans, int R, int C, hipStream_t st, int src_ld)
+int corner_turn_f64(const void* src, void* dst, int nchans, int R, int C, hipStream_t st, int src_ld,
+                    int dst_chan_cols)
 {
     const int ld = src_ld > 0 ? src_ld : C;
+    const int dcc = dst_chan_cols > 0 ? dst_chan_cols : C;
     const int tr = (R + 63) / 64, tc = (C + 63) / 64;
     hipLaunchKernelGGL(corner_turn_kernel<float2>, dim3((unsigned)((size_t)nchans * tr * tc)), dim3(256), 0, st,
-                       static_cast<const float2*>(src), static_cast<float2*>(dst), R, C, ld, tr, tc);
+                       static_cast<const float2*>(src), static_cast<float2*>(dst), R, C, ld, dcc, tr, tc);
     SW_LAUNCH_CHECK();
     return SW_OK;
 }
@@ -237,7 +242,7 @@ extern "C" int sw_corner_turn(const void* d_src, void* d_dst, int nchans, int ro
 {
     SW_CHECK_ARG(d_src && d_dst && d_src != d_dst, "null or aliased buffer");
     SW_CHECK_ARG(nchans > 0 && rows > 0 && cols > 0, "empty input");
-    return sw::corner_turn(d_src, d_dst, nchans, rows, cols, sw::as_stream(stream), 0);
+    return sw::corner_turn(d_src, d_dst, nchans, rows, cols, sw::as_stream(stream), 0, 0);
 }
 
 extern "C" int sw_statistics_fpat(const void* d_fpat, float* d_stats, int nchans, int nsamples,

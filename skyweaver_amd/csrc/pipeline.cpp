@@ -21,6 +21,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace sw {
@@ -36,7 +37,8 @@ struct sw_pipeline {
     sw_codedisp* cd    = nullptr;
     // device buffers
     void* taftp_d      = nullptr;
-    int8_t* ftpa       = nullptr;
+    int8_t* ftpa       = nullptr;  // this gulp, FPAT
+    int8_t* ftpa_prev  = nullptr;  // the previous gulp: the dedisperser reads its history there
     float* stats       = nullptr;
     void* stats_ws     = nullptr;
     std::vector<int8_t*> ftpa_dd;
@@ -87,7 +89,7 @@ struct sw_pipeline {
         if(cd) sw_codedisp_destroy(cd);
         if(idp) sw_idpipe_destroy(idp);
         if(delays) sw_delay_model_close(delays);
-        void* bufs[] = {taftp_d, ftpa, stats, stats_ws, ib_raw, ib_q, cb_out, weights, packed, scl, bmap, antw,
+        void* bufs[] = {taftp_d, ftpa, ftpa_prev, stats, stats_ws, ib_raw, ib_q, cb_out, weights, packed, scl, bmap, antw,
                         d_delays, d_freqs};
         for(void* b : bufs) (void)hipFree(b);
         for(auto* b : ftpa_dd) (void)hipFree(b);
@@ -205,12 +207,14 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
     cd.fft_len   = p->cfg.fft_len;
     cd.overlap   = p->cfg.overlap;
     cd.input_fpat = 1;
+    cd.history_in_place = 1;
     int rc = sw_codedisp_create(&cd, p->stream, &p->cd);
     if(rc) return rc;
 
     const size_t fb = p->ftpa_bytes();
     SW_HIP_TRY(hipMalloc(&p->taftp_d, fb));
     SW_HIP_TRY(hipMalloc(&p->ftpa, fb));
+    SW_HIP_TRY(hipMalloc(&p->ftpa_prev, fb));
 
     p->ftpa_dd.resize(c.ndms, nullptr);
     for(auto& b : p->ftpa_dd) SW_HIP_TRY(hipMalloc(&b, fb));
@@ -427,7 +431,9 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
     }
     SW_HIP_TRY(hipEventRecord(p->ev_w[1], st));
     SW_HIP_TRY(hipEventRecord(p->ev[0], st));
-    // the gulp is held time-major (FPAT) up to the dedisperser
+    // the gulp is held time-major (FPAT) up to the dedisperser; the two FPAT
+    // buffers alternate so that the previous gulp stays readable as history
+    std::swap(p->ftpa, p->ftpa_prev);
     rc = sw_transpose_fpat(in, p->ftpa, c.nsamples / 256, nant_in, A, F, st);
     if(rc) return rc;
     SW_HIP_TRY(hipEventRecord(p->ev[1], st));

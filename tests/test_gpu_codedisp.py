@@ -156,3 +156,40 @@ def test_codedisp_uneven_batch_and_history(sw, orc, dev):
                                         chirp, Ns[d], Os[d])
                 ei, ef = _window(ei, ef, T - lats[d], T)
                 _check(got, obf[d].download(), ei, ef, f"gulp 2 DM {dms[d]} N {Ns[d]}")
+
+
+@pytest.mark.parametrize("T,dms,Ns", [
+    (40960, [0.0, 20.0, 100.0, 180.0], [4096, 8192, 16384, 16384]),  # history read in place
+    (8192, [50.0, 180.0], [8192, 16384]),  # history longer than a gulp: copied
+])
+def test_codedisp_history_in_place(sw, orc, dev, T, dms, Ns):
+    """Time-major (FPAT) input with the history read from the previous gulp's
+    buffer: bit for bit the plan that copies its history, over three gulps
+    whose output windows wrap the output rings, and the oracle on the
+    continuous stream."""
+    from skyweaver_amd import DeviceBuffer as DB
+    F, A = 2, 64
+    kw = dict(fft_len=Ns, input_fpat=True)
+    plan_c = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, **kw)
+    plan_r = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, history_in_place=True, **kw)
+    gulps = [sw.synth_voltages((F, T, 2, A, 2), seed=0x5EED0040 + g, sigma=16.0) for g in range(3)]
+    kept = []  # every gulp's buffer stays alive (and unchanged) until the next run
+    for gi, g in enumerate(gulps):
+        d_g = DB.from_host(np.ascontiguousarray(g.transpose(0, 2, 3, 1, 4)))  # FPAT
+        kept.append(d_g)
+        oc = [DB.empty(g.shape, np.int8, fill=0x55) for _ in dms]
+        orr = [DB.empty(g.shape, np.int8, fill=0x33) for _ in dms]
+        orf = [DB.empty(g.shape, np.float32) for _ in dms]
+        plan_c.run(d_g, oc)
+        plan_r.run(d_g, orr, orf)
+        plan_c.push_history(d_g)
+        plan_r.push_history(d_g)
+        for d in range(len(dms)):
+            np.testing.assert_array_equal(orr[d].download(), oc[d].download(), err_msg=f"gulp {gi} DM {dms[d]}")
+        if gi == 2:
+            d = len(dms) - 1
+            N, O = plan_r.info(d)
+            chirp = orc.dm_chirp(dms[d], F_LOW, CHAN_BW, F, N)
+            ei, ef = _oracle_stream(orc, np.zeros((F, O, 2, A, 2), np.int8), np.concatenate(gulps, 1), chirp, N, O)
+            ei, ef = _window(ei, ef, 2 * T - plan_r.latency(d), T)
+            _check(orr[d].download(), orf[d].download(), ei, ef, f"in place gulp 3 DM {dms[d]} N {N}")

@@ -57,10 +57,13 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("valu", [False, True], ids=["auto", "valu"])
 @pytest.mark.parametrize("case", range(len(CASES)))
-def test_vs_oracle(sw, orc, dev, case):
+def test_vs_oracle(sw, orc, dev, case, valu):
     F, B, nvec, ts, dms, nout, (fc, bw, cbts) = CASES[case]
-    idd = sw.IncoherentDedisperser(F, fc, bw, cbts, dms, tscrunch=ts)
+    idd = sw.IncoherentDedisperser(F, fc, bw, cbts, dms, tscrunch=ts, valu_kernel=valu)
+    want_kernel = "idisp_mfma_kernel" if not valu and F % 4 == 0 and 16 * ts <= 128 else "idisp_kernel"
+    assert idd.kernel == want_kernel
     f = sw.channel_frequencies(fc, bw, F)
     d_o, mx_o = orc.id_delays(dms, f, cbts / (bw / F))
     np.testing.assert_array_equal(idd.delays, d_o)

@@ -212,6 +212,143 @@ __device__ __forceinline__ void apply_powers(float2* v, float2 w1)
     }
 }
 
+// Forward steps 1-3 of an N = 512 R1 transform of C = 32 / R1 columns: v
+// holds the thread's inputs (value c R1 + n1 = sample 512 n1 + t of column
+// c) and ends holding X[k] of the pairs q = t, t + 512 (16 values each,
+// k3 = 0..15).  The caller has synchronised since the exchange buffer was
+// last read.
+template <int R1>
+__device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddles& tw, int t)
+{
+    constexpr int C = 32 / R1;
+    const int n3 = t & 15, j2 = t >> 4;
+    {
+        const float2 w = tw(t);
+#pragma unroll
+        for(int c = 0; c < C; ++c) {
+            dft<R1, false>(v + c * R1);
+            apply_powers<R1>(v + c * R1, w);
+        }
+    }
+#pragma unroll
+    for(int i = 0; i < 32; ++i) lds[i * kEx1Row + t] = v[i];  // row i = c R1 + k1
+    __syncthreads();
+    // step 2: thread (j2 = c R1 + k1, n3) holds n2 = 0..31
+#pragma unroll
+    for(int n2 = 0; n2 < 32; ++n2) v[n2] = lds[j2 * kEx1Row + n2 * 16 + n3];
+    __syncthreads();
+    dft32<false>(v);
+    apply_powers<32>(v, tw(R1 * n3));
+#pragma unroll
+    for(int k2 = 0; k2 < 32; ++k2) lds[j2 * kEx2Row + k2 * kEx2Stride + n3] = v[k2];
+    __syncthreads();
+    // step 3: pairs q = t, t + 512 (row q >> 5, k2 = q & 31): DFT16 over n3 -> k3
+#pragma unroll
+    for(int h = 0; h < 2; ++h) {
+        const int q = t + 512 * h;
+        float2* x = v + 16 * h;
+#pragma unroll
+        for(int m = 0; m < 16; ++m) x[m] = lds[(q >> 5) * kEx2Row + (q & 31) * kEx2Stride + m];
+        dft16<false>(x);
+    }
+}
+
+// Y = X H with the chirp stored permuted (index 32 R1 k3 + 32 k1 + k2), then
+// the inverse steps 3'-1': v ends holding y[512 n1 + t] (unnormalised) at
+// c R1 + n1.
+template <int R1>
+__device__ __forceinline__ void reg_chirp_inverse(float2* v, float2* lds, const Twiddles& tw, int t,
+                                                  const float2* __restrict__ H)
+{
+    constexpr int C = 32 / R1;
+    const int n3 = t & 15, j2 = t >> 4;
+    const int k1_2 = j2 % R1;
+#pragma unroll
+    for(int h = 0; h < 2; ++h) {
+        const int q = t + 512 * h;
+        const int k2 = q & 31, k1 = (q >> 5) % R1;
+        float2* x = v + 16 * h;
+#pragma unroll
+        for(int k3 = 0; k3 < 16; ++k3) x[k3] = cmul(x[k3], H[32 * R1 * k3 + 32 * k1 + k2]);
+        // step 3': IDFT16 -> n3, x W_N^(-R1 n3 k2)
+        dft16<true>(x);
+        apply_powers<16>(x, cconj(tw(R1 * k2)));
+    }
+    __syncthreads();  // all readers of the exchange buffer are done
+#pragma unroll
+    for(int h = 0; h < 2; ++h) {
+        const int q = t + 512 * h;
+#pragma unroll
+        for(int m = 0; m < 16; ++m) lds[(q >> 5) * kEx2Row + (q & 31) * kEx2Stride + m] = v[16 * h + m];
+    }
+    __syncthreads();
+    // step 2': thread (j2, n3) holds k2 = 0..31; IDFT32 -> n2, x W_N^(-k1 (16 n2 + n3))
+#pragma unroll
+    for(int k2 = 0; k2 < 32; ++k2) v[k2] = lds[j2 * kEx2Row + k2 * kEx2Stride + n3];
+    __syncthreads();
+    dft32<true>(v);
+    {
+        const float2 b = cconj(tw(k1_2 * n3));
+        apply_powers<32>(v, cconj(tw(16 * k1_2)));
+#pragma unroll
+        for(int n2 = 0; n2 < 32; ++n2) v[n2] = cmul(v[n2], b);
+    }
+#pragma unroll
+    for(int n2 = 0; n2 < 32; ++n2) lds[j2 * kEx1Row + n2 * 16 + n3] = v[n2];
+    __syncthreads();
+    // step 1': thread t holds k1 = 0..R1-1 of each column; IDFT_R1 -> n1
+#pragma unroll
+    for(int i = 0; i < 32; ++i) v[i] = lds[i * kEx1Row + t];
+#pragma unroll
+    for(int c = 0; c < C; ++c) dft<R1, true>(v + c * R1);
+}
+
+// Raw complex int8 samples b + 512 i + t, i = 0..2 n - 1, of one column,
+// two per register, where b is the first sample of a segment relative to
+// the gulp start (negative: history).
+template <int NP>
+__device__ __forceinline__ void load_column(const CdParams& p, int f, int col, long b, int t, uint32_t* dst)
+{
+    constexpr int NS = 1024 * NP;  // samples spanned
+    const size_t cc = (size_t)f * p.PA + col;
+    const uint16_t* cur = reinterpret_cast<const uint16_t*>(p.cur) + cc * p.T;
+    if(b >= 0 && b + NS <= p.T) {  // inside this gulp (uniform branch)
+        const uint16_t* x = cur + b;
+#pragma unroll
+        for(int i = 0; i < NP; ++i)
+            dst[i] = (uint32_t)x[1024 * i + t] | ((uint32_t)x[1024 * i + 512 + t] << 16);
+    } else if(b < 0 && b + NS <= p.T) {  // history, then this gulp
+        // lo[tau] for tau < 0: the previous gulp's tail or the history buffer
+        const uint16_t* lo = p.prev ? reinterpret_cast<const uint16_t*>(p.prev) + cc * p.T + p.T
+                                    : reinterpret_cast<const uint16_t*>(p.hist) + cc * p.Oh + p.Oh;
+        const int b32 = (int)b;
+#pragma unroll
+        for(int i = 0; i < NP; ++i) {
+            const int t0 = b32 + 1024 * i + t, t1 = t0 + 512;
+            dst[i] = (uint32_t)(t0 < 0 ? lo : cur)[t0] | ((uint32_t)(t1 < 0 ? lo : cur)[t1] << 16);
+        }
+    } else {
+#pragma unroll
+        for(int i = 0; i < NP; ++i)
+            dst[i] = input_raw(p, f, col, (int)(b + 1024 * i + t)) |
+                     (input_raw(p, f, col, (int)(b + 1024 * i + 512 + t)) << 16);
+    }
+}
+
+// Quantise y / N and write the kept positions [O/2, N - O/2) of a segment
+// into the output ring (ring index j0 + position, wrapped once).
+__device__ __forceinline__ void store_output(const CdParams& p, int d, size_t colbase, int j0, int pos, int N,
+                                             float2 y)
+{
+    const int half = p.O / 2;
+    if(pos >= half && pos < N - half) {
+        int jj = j0 + pos;
+        if(jj >= p.ylen) jj -= p.ylen;
+        store_c8(p.out[d] + (colbase + jj) * 2, y);
+        if(p.outf[d]) p.outf[d][colbase + jj] = y;
+    }
+}
+
 // Persistent: a workgroup (one per CU, the exchange buffer fills the LDS)
 // walks units u = blockIdx.x + k gridDim.x, u = (channel, segment, column
 // group), and loads the next unit's input while transforming the current
@@ -230,46 +367,15 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
     auto fetch = [&](int u) {
         const int cg = u % ngrp, fs = u / ngrp;
         const int f = fs / p.nseg, s = fs % p.nseg;
-        const long b = p.in0 + (long)s * p.hop;  // first input sample, relative to the gulp start
+        const long b = p.in0 + (long)s * p.hop;
 #pragma unroll
-        for(int c = 0; c < C; ++c) {
-            const size_t cc = (size_t)f * p.PA + cg * C + c;
-            const uint16_t* cur = reinterpret_cast<const uint16_t*>(p.cur) + cc * p.T;
-            uint32_t* dst = nxt + c * (R1 / 2);
-            if(b >= 0 && b + N <= p.T) {  // inside this gulp (uniform branch)
-                const uint16_t* x = cur + b;
-#pragma unroll
-                for(int i = 0; i < R1 / 2; ++i)
-                    dst[i] = (uint32_t)x[1024 * i + t] | ((uint32_t)x[1024 * i + 512 + t] << 16);
-            } else if(b < 0 && b + N <= p.T) {  // history, then this gulp
-                // lo[tau] for tau < 0: the previous gulp's tail or the history buffer
-                const uint16_t* lo = p.prev ? reinterpret_cast<const uint16_t*>(p.prev) + cc * p.T + p.T
-                                            : reinterpret_cast<const uint16_t*>(p.hist) + cc * p.Oh + p.Oh;
-                const int b32 = (int)b;
-#pragma unroll
-                for(int i = 0; i < R1 / 2; ++i) {
-                    const int t0 = b32 + 1024 * i + t, t1 = t0 + 512;
-                    dst[i] = (uint32_t)(t0 < 0 ? lo : cur)[t0] | ((uint32_t)(t1 < 0 ? lo : cur)[t1] << 16);
-                }
-            } else {
-#pragma unroll
-                for(int i = 0; i < R1 / 2; ++i)
-                    dst[i] = input_raw(p, f, cg * C + c, (int)(b + 1024 * i + t)) |
-                             (input_raw(p, f, cg * C + c, (int)(b + 1024 * i + 512 + t)) << 16);
-            }
-        }
+        for(int c = 0; c < C; ++c) load_column<R1 / 2>(p, f, cg * C + c, b, t, nxt + c * (R1 / 2));
     };
-    // thread roles: step 2 (and 2') row j = c R1 + k1 with n3; step 3 pairs
-    // q = t + 512 h -> row q >> 5, k2 = q & 31
-    const int n3 = t & 15, j2 = t >> 4;
-    const int k1_2 = j2 % R1;
     if(blockIdx.x < nunits) fetch(blockIdx.x);
     for(int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int cg = u % ngrp, fs = u / ngrp;
         const int f = fs / p.nseg, s = fs % p.nseg;
         float2 v[32];
-
-        // ---- forward, step 1: thread t = (n2, n3) holds n1 = 0..R1-1 of each column
 #pragma unroll
         for(int i = 0; i < 16; ++i) {
             v[2 * i]     = unpack_c8(nxt[i]);
@@ -277,96 +383,83 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
         }
         if(u + (int)gridDim.x < nunits) fetch(u + gridDim.x);  // in flight during the transforms
         __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
-        {
-            const float2 w = tw(t);
+        reg_forward<R1>(v, lds, tw, t);
+        reg_chirp_inverse<R1>(v, lds, tw, t, p.chirp[d] + (size_t)f * N);
+        const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
 #pragma unroll
-            for(int c = 0; c < C; ++c) {
-                dft<R1, false>(v + c * R1);
-                apply_powers<R1>(v + c * R1, w);
-            }
+        for(int c = 0; c < C; ++c) {
+            const size_t cb = ((size_t)f * p.PA + cg * C + c) * p.ylen;
+#pragma unroll
+            for(int n1 = 0; n1 < R1; ++n1)
+                store_output(p, d, cb, j0, 512 * n1 + t, N, cscale(v[c * R1 + n1], inv_n));
+        }
+    }
+}
+
+// N = 32768 as two interleaved 16384-point halves (radix-2 decimation in
+// frequency, RP = 2, L = N / 2), fused into one persistent kernel:
+//   Z_r[n] = W_N^(n r) (x[n] + (-1)^r x[n + L]),  X[2m + r] = FFT_L(Z_r)[m]
+//   V_r = IFFT_L(X[2m + r] H[2m + r])
+//   y[n] = V_0[n] + W_N^(-n) V_1[n],  y[n + L] = V_0[n] - W_N^(-n) V_1[n]
+// Each thread holds samples n = 512 n1 + t (and n + L) of one column.  V_0
+// waits in a per-workgroup slice of `scratch` (each thread reads back only
+// what it wrote) while V_1 is transformed; the chirp is stored [F][r][L] in
+// the register order of the halves.  The raw samples are loaded for each
+// half (the second load hits L2) rather than held across a transform, which
+// would not fit the registers.
+__global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, int nunits, float2* scratch)
+{
+    constexpr int L = 16384, N = 2 * L;
+    __shared__ float2 lds[kRegLds + TwiddleLds<L>::SIZE];
+    const Twiddles tw = load_twiddles<L, 512>(lds + kRegLds, p.tw);
+    const int t = threadIdx.x;
+    const float inv_n = 1.0f / (float)N;
+    uint32_t raw[32];  // x[512 n1 + t] at 0..15, x[L + 512 n1 + t] at 16..31, two per register
+    for(int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        // the thread's slots, formed afresh per unit (an opaque offset keeps
+        // the compiler from holding 32 precomputed addresses in registers)
+        int off = t;
+        asm volatile("" : "+v"(off));
+        float2* mine = scratch + (size_t)blockIdx.x * L + off;  // [n1][512]
+        const float2* wn = p.twN + off;                         // W_N^(512 n1 + t) at wn[512 n1]
+        const int col = u % p.PA, fs = u / p.PA;
+        const int f = fs / p.nseg, s = fs % p.nseg;
+        const long b = p.in0 + (long)s * p.hop;
+        const float2* H = p.chirp[d] + (size_t)f * N;
+        float2 v[32];
+        // ---- r = 0: Z_0 = x[n] + x[n + L]
+        load_column<32>(p, f, col, b, t, raw);
+#pragma unroll
+        for(int i = 0; i < 16; ++i) {
+            v[2 * i]     = cadd(unpack_c8(raw[i]), unpack_c8(raw[16 + i]));
+            v[2 * i + 1] = cadd(unpack_c8(raw[i] >> 16), unpack_c8(raw[16 + i] >> 16));
+        }
+        __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
+        reg_forward<32>(v, lds, tw, t);
+        reg_chirp_inverse<32>(v, lds, tw, t, H);
+#pragma unroll
+        for(int n1 = 0; n1 < 32; ++n1) mine[512 * n1] = v[n1];
+        // ---- r = 1: Z_1 = W_N^n (x[n] - x[n + L])
+        load_column<32>(p, f, col, b, t, raw);
+#pragma unroll
+        for(int i = 0; i < 16; ++i) {
+            v[2 * i]     = csub(unpack_c8(raw[i]), unpack_c8(raw[16 + i]));
+            v[2 * i + 1] = csub(unpack_c8(raw[i] >> 16), unpack_c8(raw[16 + i] >> 16));
         }
 #pragma unroll
-        for(int i = 0; i < 32; ++i) lds[i * kEx1Row + t] = v[i];  // row i = c R1 + k1
+        for(int n1 = 0; n1 < 32; ++n1) v[n1] = cmul(v[n1], wn[512 * n1]);
         __syncthreads();
-        // ---- step 2: thread (j2 = c R1 + k1, n3) holds n2 = 0..31
+        reg_forward<32>(v, lds, tw, t);
+        reg_chirp_inverse<32>(v, lds, tw, t, H + L);
+        // ---- y[n] = V_0 + W_N^(-n) V_1, y[n + L] = V_0 - W_N^(-n) V_1
+        const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;
+        const size_t cb = ((size_t)f * p.PA + col) * p.ylen;
 #pragma unroll
-        for(int n2 = 0; n2 < 32; ++n2) v[n2] = lds[j2 * kEx1Row + n2 * 16 + n3];
-        __syncthreads();
-        dft32<false>(v);
-        apply_powers<32>(v, tw(R1 * n3));
-#pragma unroll
-        for(int k2 = 0; k2 < 32; ++k2) lds[j2 * kEx2Row + k2 * kEx2Stride + n3] = v[k2];
-        __syncthreads();
-        // ---- step 3: pairs q = t, t + 512: DFT16 over n3 -> k3
-#pragma unroll
-        for(int h = 0; h < 2; ++h) {
-            const int q = t + 512 * h;
-            float2* x = v + 16 * h;
-#pragma unroll
-            for(int m = 0; m < 16; ++m) x[m] = lds[(q >> 5) * kEx2Row + (q & 31) * kEx2Stride + m];
-            dft16<false>(x);
-        }
-        {
-            // chirp (permuted: 32 R1 k3 + 32 k1 + k2), then the inverse step 3':
-            // IDFT16 -> n3, x W_N^(-R1 n3 k2)
-            const float2* H = p.chirp[d] + (size_t)f * N;
-#pragma unroll
-            for(int h = 0; h < 2; ++h) {
-                const int q = t + 512 * h;
-                const int k2 = q & 31, k1 = (q >> 5) % R1;
-                float2* x = v + 16 * h;
-#pragma unroll
-                for(int k3 = 0; k3 < 16; ++k3) x[k3] = cmul(x[k3], H[32 * R1 * k3 + 32 * k1 + k2]);
-                dft16<true>(x);
-                apply_powers<16>(x, cconj(tw(R1 * k2)));
-            }
-            __syncthreads();  // all readers of the exchange buffer are done
-#pragma unroll
-            for(int h = 0; h < 2; ++h) {
-                const int q = t + 512 * h;
-#pragma unroll
-                for(int m = 0; m < 16; ++m) lds[(q >> 5) * kEx2Row + (q & 31) * kEx2Stride + m] = v[16 * h + m];
-            }
-            __syncthreads();
-            // ---- step 2': thread (j2, n3) holds k2 = 0..31; IDFT32 -> n2,
-            // x W_N^(-k1 (16 n2 + n3))
-#pragma unroll
-            for(int k2 = 0; k2 < 32; ++k2) v[k2] = lds[j2 * kEx2Row + k2 * kEx2Stride + n3];
-            __syncthreads();
-            dft32<true>(v);
-            {
-                const float2 b = cconj(tw(k1_2 * n3));
-                apply_powers<32>(v, cconj(tw(16 * k1_2)));
-#pragma unroll
-                for(int n2 = 0; n2 < 32; ++n2) v[n2] = cmul(v[n2], b);
-            }
-#pragma unroll
-            for(int n2 = 0; n2 < 32; ++n2) lds[j2 * kEx1Row + n2 * 16 + n3] = v[n2];
-            __syncthreads();
-            // ---- step 1': thread t holds k1 = 0..R1-1 of each column; IDFT_R1 -> n1
-#pragma unroll
-            for(int i = 0; i < 32; ++i) v[i] = lds[i * kEx1Row + t];
-#pragma unroll
-            for(int c = 0; c < C; ++c) dft<R1, true>(v + c * R1);
-            const int half = p.O / 2;
-            const int j0 = (int)(p.y0 + (long)s * p.hop) - half;  // ring index of position 0
-#pragma unroll
-            for(int c = 0; c < C; ++c) {
-                const size_t cb = ((size_t)f * p.PA + cg * C + c) * p.ylen;
-                int8_t* out = p.out[d] + cb * 2;
-                float2* outf = p.outf[d] ? p.outf[d] + cb : nullptr;
-#pragma unroll
-                for(int n1 = 0; n1 < R1; ++n1) {
-                    const int pos = 512 * n1 + t;
-                    int jj = j0 + pos;
-                    if(jj >= p.ylen) jj -= p.ylen;
-                    if(pos >= half && pos < N - half) {
-                        const float2 y = cscale(v[c * R1 + n1], inv_n);
-                        store_c8(out + (size_t)jj * 2, y);
-                        if(outf) outf[jj] = y;
-                    }
-                }
-            }
+        for(int n1 = 0; n1 < 32; ++n1) {
+            const float2 a = mine[512 * n1];
+            const float2 b = cmul(v[n1], cconj(wn[512 * n1]));
+            store_output(p, d, cb, j0, 512 * n1 + t, N, cscale(cadd(a, b), inv_n));
+            store_output(p, d, cb, j0, L + 512 * n1 + t, N, cscale(csub(a, b), inv_n));
         }
     }
 }
@@ -439,8 +532,8 @@ __global__ void twiddle_kernel(float2* tw, int n)
 }
 
 // DMResponse in FP64 (the only FP64 step), stored as [f][r][m] for k = RP m + r
-// R1 > 0: stored in the register kernels' order, k = k1 + R1 k2 + 32 R1 k3 at
-// 32 R1 k3 + 32 k1 + k2; else [RP][N / RP] with k = RP m + r at r N / RP + m
+// [RP][L = N / RP] with k = RP m + r at r L + m', where m' = m, or for the
+// register kernels (R1 > 0) m = k1 + R1 k2 + 32 R1 k3 at m' = 32 R1 k3 + 32 k1 + k2
 __global__ void chirp_kernel(float2* out, int nchans, int N, int RP, int R1, double low_freq,
                              double chan_bw, double dm_prefactor)
 {
@@ -454,14 +547,13 @@ __global__ void chirp_kernel(float2* out, int nchans, int N, int RP, int R1, dou
     const double phase = phase_prefactor / ((nu_0 + nu) * nu_0 * nu_0);
     double s, c;
     sincos(phase, &s, &c);
-    const float2 h = make_float2((float)c, (float)s);
+    const int L = N / RP, m = k / RP, r = k % RP;
+    int mp = m;
     if(R1 > 0) {
-        const int k1 = k % R1, k2 = (k / R1) % 32, k3 = k / (32 * R1);
-        out[(size_t)chan * N + 32 * R1 * k3 + 32 * k1 + k2] = h;
-    } else {
-        const int L = N / RP, m = k / RP, r = k % RP;
-        out[((size_t)chan * RP + r) * L + m] = h;
+        const int k1 = m % R1, k2 = (m / R1) % 32, k3 = m / (32 * R1);
+        mp = 32 * R1 * k3 + 32 * k1 + k2;
     }
+    out[((size_t)chan * RP + r) * L + mp] = make_float2((float)c, (float)s);
 }
 
 // new history = the last Oh samples of (old history ++ gulp), per column
@@ -526,18 +618,28 @@ int launch_lds(int L, const CdParams& p, int blocks, hipStream_t st)
 }
 
 // the register kernels take N = 4096, 8192, 16384 (R1 = N / 512) when the
-// columns split into units of 32 / R1
+// columns split into units of 32 / R1, and N = 32768 as two halves of 16384
+// (RP = 2); 0: the LDS / global-memory kernels
 int reg_r1(int N, int RP, int PA)
 {
+    if(RP == 2 && N == 32768) return 32;
     if(RP != 1 || (N != 4096 && N != 8192 && N != 16384)) return 0;
     const int R1 = N / 512;
     return PA % (32 / R1) == 0 ? R1 : 0;
 }
 
-// units = (channel, segment, group of 32 / R1 columns); one DM per launch
-int launch_reg(int R1, const CdParams& p, int nunits, hipStream_t st)
+// units = (channel, segment, group of 32 / R1 columns), or (channel,
+// segment, column) for N = 32768; one DM per launch
+int launch_reg(int R1, const CdParams& p, int nunits, float2* scratch, hipStream_t st)
 {
     const dim3 grid((unsigned)std::min(nunits, persistent_blocks()));
+    if(p.RP == 2) {
+        for(int d = 0; d < p.ndm; ++d) {
+            hipLaunchKernelGGL(codedisp_reg2_kernel, grid, dim3(512), 0, st, p, d, nunits, scratch);
+            SW_LAUNCH_CHECK();
+        }
+        return SW_OK;
+    }
     for(int d = 0; d < p.ndm; ++d) {
 #define SW_REG(RR)                                                                              \
     case RR:                                                                                    \
@@ -586,6 +688,7 @@ struct sw_codedisp {
     std::vector<int> lat;                // per DM output latency
     long long gulps = 0;                 // gulps completed (push_history calls)
     float2* zbuf = nullptr;              // DIF stage output (N > kMaxL)
+    float2* reg2 = nullptr;              // N = 32768 kernel: one 16384-point half per workgroup
     std::vector<float2*> vbuf;           // per DM slot (N > kMaxL)
     ~sw_codedisp()
     {
@@ -597,6 +700,7 @@ struct sw_codedisp {
         for(auto* y : ytm) (void)hipFree(y);
         for(auto* y : ytf) (void)hipFree(y);
         (void)hipFree(zbuf);
+        (void)hipFree(reg2);
         for(auto* v : vbuf) (void)hipFree(v);
     }
 };
@@ -705,15 +809,22 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
         }
     }
     size_t zmax = 0, slots = 1, vslots = 0;
+    bool reg2 = false;
     for(auto& g : p->groups) {
         if(!sw::twiddle_table(p.get(), g.L, st)) return SW_ERR_ALLOC;
         slots = std::max(slots, std::min<size_t>(sw::kMaxDm, g.dms.size()));
         if(g.RP > 1) {
             if(!sw::twiddle_table(p.get(), g.N, st)) return SW_ERR_ALLOC;
-            zmax = std::max(zmax, (size_t)p->F * g.nseg * g.N * p->PA * sizeof(float2));
-            vslots = std::max(vslots, std::min<size_t>(sw::kMaxDm, g.dms.size()));
+            if(sw::reg_r1(g.N, g.RP, p->PA) > 0) {
+                reg2 = true;
+            } else {
+                zmax = std::max(zmax, (size_t)p->F * g.nseg * g.N * p->PA * sizeof(float2));
+                vslots = std::max(vslots, std::min<size_t>(sw::kMaxDm, g.dms.size()));
+            }
         }
     }
+    if(reg2)
+        SW_HIP_TRY(hipMalloc(&p->reg2, sizeof(float2) * 16384 * (size_t)sw::persistent_blocks()));
     p->chirp.resize(p->dms.size(), nullptr);
     for(int d = 0; d < (int)p->dms.size(); ++d) {
         const int N = p->N[d], RP = N / std::min(N, sw::kMaxL);
@@ -872,6 +983,7 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const*
             p.ndm = nb;
             p.tw  = plan->twiddles.at(g.L);
             p.twN = g.RP > 1 ? plan->twiddles.at(g.N) : nullptr;
+            const int R1 = sw::reg_r1(g.N, g.RP, plan->PA);
             for(int k = 0; k < nb; ++k) {
                 const int d = todo[b0 + k];
                 SW_CHECK_ARG(d_out[d], "null output buffer");
@@ -884,14 +996,14 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const*
                     SW_HIP_TRY(hipMemsetAsync(plan->ytf[d], 0, yb, st));
                 }
                 p.outf[k]  = (d_out_f32 && d_out_f32[d]) ? plan->ytf[d] : nullptr;
-                if(g.RP > 1) p.v[k] = plan->vbuf[k];
+                if(g.RP > 1 && R1 == 0) p.v[k] = plan->vbuf[k];  // global-memory DIF/DIT path
             }
             const int G = plan->PA / sw::lds_shape(g.L).C;
-            const int R1 = sw::reg_r1(g.N, g.RP, plan->PA);
             if(p.nseg == 0) {
                 // no segment completes in this gulp
             } else if(R1 > 0) {
-                rc = sw::launch_reg(R1, p, plan->F * p.nseg * (plan->PA / (32 / R1)), st);
+                const int units = g.RP == 2 ? plan->F * p.nseg * plan->PA : plan->F * p.nseg * (plan->PA / (32 / R1));
+                rc = sw::launch_reg(R1, p, units, plan->reg2, st);
                 if(rc) return rc;
             } else if(g.RP == 1) {
                 rc = sw::launch_lds<false>(g.L, p, plan->F * p.nseg * G, st);

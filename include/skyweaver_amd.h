@@ -279,7 +279,8 @@ int sw_codedisp_destroy(sw_codedisp* plan);
 int sw_codedisp_info(const sw_codedisp* plan, int dm_idx, int* fft_len, int* overlap,
                      int* latency);
 /* Dedisperse the gulp d_ftpa (FTPA [F][T][2][A]) for the DMs
- * [dm_begin, dm_begin + dm_count): d_out[d] receives FTPA int8 for DM d;
+ * [dm_begin, dm_begin + dm_count): d_out[d] receives FTPA int8 for DM d
+ * (NULL: the output stays in the plan's time-major ring, sw_codedisp_window);
  * d_out_f32 (NULL or per-DM NULL allowed) receives the unquantised y/N as
  * float (re, im) pairs.  batched = 1 shares each forward FFT between up to 8
  * DMs of equal (N, O) in one launch; batched = 0 launches one DM at a time.
@@ -288,6 +289,11 @@ int sw_codedisp_info(const sw_codedisp* plan, int dm_idx, int* fft_len, int* ove
 int sw_codedisp_run(sw_codedisp* plan, const void* d_ftpa, void* const* d_out,
                     void* const* d_out_f32, int dm_begin, int dm_count, int batched,
                     void* stream);
+/* The last run's int8 output of DM dm in the plan's time-major ring:
+ * sample t of column (f, p, a) at ring + ((f*2A + p*A + a) * ld + offset + t) * 2
+ * (ld a multiple of the gulp length, offset a multiple of it).  Valid until
+ * the next run. */
+int sw_codedisp_window(const sw_codedisp* plan, int dm, const void** d_ring, int* ld, int* offset);
 /* End the gulp d_ftpa: advance the overlap-save history and the segment
  * grid, and carry outputs computed ahead of this gulp's window. */
 int sw_codedisp_push_history(sw_codedisp* plan, const void* d_ftpa, void* stream);

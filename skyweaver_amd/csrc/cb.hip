@@ -248,13 +248,17 @@ struct EpiParams {
     int s = 0;
 };
 
-template <int S, int TS, int NSTEPS>
+// TMI: the voltages come time-major from the coherent dedisperser's ring
+// (sample t of column c = p A + a of channel f at ftpa + ((f 2A + c) ld + w0
+// + t) 2) and are corner-turned through LDS into the FTPA rows the operands
+// are read from; else ftpa is FTPA [F][T][2][A].
+template <int S, int TS, int NSTEPS, bool TMI>
 __global__ void __launch_bounds__(kMfmaWaves * 64)
 cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ packed,
                int8_t* __restrict__ out, const float* __restrict__ scale,
                const float* __restrict__ offset, const int* __restrict__ bmap,
                const float* __restrict__ ib_raw, int nchans, int nsamples, int nbeams,
-               int ntchunks, int ib_sub)
+               int ntchunks, int ib_sub, long ld, long w0)
 {
     constexpr int NV = nstokes(S);
     constexpr int A  = NSTEPS * 16;  // antennas (K' = 2A = 32 * NSTEPS)
@@ -265,7 +269,14 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     constexpr int WTILE  = 32 * 2 * KP / 16;     // uint4 per tile in global memory
     constexpr int WPER   = (WTILE + kMfmaWaves * 64 - 1) / (kMfmaWaves * 64);
     constexpr int NOUT   = TS <= 16 ? 16 / TS : 1;  // outputs per lane per tile
-    __shared__ uint4 wlds[2][32 * WPITCH / 16];
+    // LDS: the weight tiles (double-buffered), or first, with TMI, the
+    // workgroup's 32 kMfmaWaves time rows of both polarisations (rows of
+    // 4A bytes padded by 16)
+    constexpr int WL16   = 32 * WPITCH / 16;
+    constexpr int SPITCH = 4 * A + 16;
+    constexpr int ST16   = TMI ? 32 * kMfmaWaves * SPITCH / 16 : 0;
+    __shared__ uint4 smem[2 * WL16 > ST16 ? 2 * WL16 : ST16];
+    uint4 (*wlds)[WL16] = reinterpret_cast<uint4 (*)[WL16]>(smem);
     // XCD-aware mapping: consecutive workgroups go round-robin over the 8
     // XCDs; keep all time chunks of a channel on one XCD so its packed
     // weights stay resident in that XCD's L2.
@@ -298,10 +309,27 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     const int trow = 16 * ((row >> 2) & 1) + 4 * (row >> 3) + (row & 3);
 
     // ---- A operands: 32 time rows x K' for both polarisations
+    if constexpr(TMI) {
+        // corner turn: 16-byte runs of 8 samples of each column into rows
+        const int t0w = tc * kMfmaWaves * 32;
+        uint8_t* stg = reinterpret_cast<uint8_t*>(smem);
+        for(int i = tid; i < 2 * A * 4 * kMfmaWaves; i += kMfmaWaves * 64) {
+            const int c = i % (2 * A), q = i / (2 * A);
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if(t0w + 8 * q < nsamples)
+                v = *reinterpret_cast<const uint4*>(ftpa + (((size_t)f * 2 * A + c) * ld + w0 + t0w + 8 * q) * 2);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for(int k = 0; k < 8; ++k)
+                *reinterpret_cast<uint16_t*>(stg + (8 * q + k) * SPITCH + c * 2) = (uint16_t)(w[k >> 1] >> (16 * (k & 1)));
+        }
+        __syncthreads();
+    }
     v4i va[2][NSTEPS];
 #pragma unroll
     for(int p = 0; p < 2; ++p) {
-        const uint8_t* vrow = ftpa + (((size_t)f * nsamples + tl + trow) * 2 + p) * A * 2;
+        const uint8_t* vrow = TMI ? reinterpret_cast<const uint8_t*>(smem) + (wave * 32 + trow) * SPITCH + p * A * 2
+                                  : ftpa + (((size_t)f * nsamples + tl + trow) * 2 + p) * A * 2;
 #pragma unroll
         for(int s = 0; s < NSTEPS; ++s) {
             const int k  = 32 * s + 16 * grp;
@@ -360,6 +388,7 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     };
     uint4 wreg[WPER];
     fetch(0, wreg);
+    if constexpr(TMI) __syncthreads();  // every wave has its operands: the rows' LDS becomes the weight tiles
     stash(0, wreg);
     __syncthreads();
 
@@ -490,18 +519,18 @@ int launch_plain(const uint8_t* ftpa, const uint8_t* w, int8_t* out, const float
     return SW_OK;
 }
 
-template <int S, int TS>
+template <int S, int TS, bool TMI>
 int launch_mfma_ts(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, const float* sc,
                    const float* off, const int* bmap, const float* ib, const sw_bf_dims& d,
-                   hipStream_t st)
+                   hipStream_t st, long ld, long w0)
 {
     const int ntchunks = (d.nsamples + 32 * kMfmaWaves - 1) / (32 * kMfmaWaves);
     dim3 grid(ntchunks * d.nchans);
 #define SW_MFMA(NS)                                                                        \
     case NS:                                                                               \
-        hipLaunchKernelGGL((cb_mfma_kernel<S, TS, NS>), grid, dim3(kMfmaWaves * 64), 0,    \
+        hipLaunchKernelGGL((cb_mfma_kernel<S, TS, NS, TMI>), grid, dim3(kMfmaWaves * 64), 0, \
                            st, ftpa, packed, out, sc, off, bmap, ib, d.nchans,             \
-                           d.nsamples, d.nbeams, ntchunks, d.ib_subtract);                 \
+                           d.nsamples, d.nbeams, ntchunks, d.ib_subtract, ld, w0);         \
         break;
     switch(d.nantennas / 16) {
         SW_MFMA(1) SW_MFMA(2) SW_MFMA(3) SW_MFMA(4) SW_MFMA(5) SW_MFMA(6) SW_MFMA(7) SW_MFMA(8)
@@ -514,18 +543,18 @@ int launch_mfma_ts(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, cons
     return SW_OK;
 }
 
-template <int S>
+template <int S, bool TMI>
 int launch_mfma(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, const float* sc,
                 const float* off, const int* bmap, const float* ib, const sw_bf_dims& d,
-                hipStream_t st)
+                hipStream_t st, long ld = 0, long w0 = 0)
 {
     switch(d.tscrunch) {
-    case 1: return launch_mfma_ts<S, 1>(ftpa, packed, out, sc, off, bmap, ib, d, st);
-    case 2: return launch_mfma_ts<S, 2>(ftpa, packed, out, sc, off, bmap, ib, d, st);
-    case 4: return launch_mfma_ts<S, 4>(ftpa, packed, out, sc, off, bmap, ib, d, st);
-    case 8: return launch_mfma_ts<S, 8>(ftpa, packed, out, sc, off, bmap, ib, d, st);
-    case 16: return launch_mfma_ts<S, 16>(ftpa, packed, out, sc, off, bmap, ib, d, st);
-    case 32: return launch_mfma_ts<S, 32>(ftpa, packed, out, sc, off, bmap, ib, d, st);
+    case 1: return launch_mfma_ts<S, 1, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0);
+    case 2: return launch_mfma_ts<S, 2, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0);
+    case 4: return launch_mfma_ts<S, 4, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0);
+    case 8: return launch_mfma_ts<S, 8, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0);
+    case 16: return launch_mfma_ts<S, 16, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0);
+    case 32: return launch_mfma_ts<S, 32, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0);
     default:
         set_error("sw_cb_beamform: MFMA form supports tscrunch in {1, 2, 4, 8, 16, 32}");
         return SW_ERR_UNSUPPORTED;
@@ -579,6 +608,30 @@ extern "C" int sw_cb_pack_weights(const void* d_weights, void* d_packed, int nch
     return SW_OK;
 }
 
+namespace sw {
+// The MFMA form on the coherent dedisperser's time-major ring (Stokes I):
+// sample t of column (f, p, a) at d_ring + ((f 2A + p A + a) ld + w0 + t) 2.
+int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_packed, const float* d_scale,
+                          const float* d_offset, const int* d_beamset_map, const float* d_ib_raw, void* d_out,
+                          const sw_bf_dims& d, hipStream_t st)
+{
+    int rc = check_dims(d);
+    if(rc) return rc;
+    SW_CHECK_ARG(d_ring && d_packed && d_scale && d_offset && d_beamset_map && d_ib_raw && d_out, "null buffer");
+    SW_CHECK_ARG((reinterpret_cast<uintptr_t>(d_ring) & 15) == 0 && (reinterpret_cast<uintptr_t>(d_packed) & 15) == 0
+                     && ld % 8 == 0 && w0 % 8 == 0 && d.nsamples % 8 == 0,
+                 "ring and packed weights must be 16-byte aligned, rows 8-sample aligned");
+    SW_CHECK_ARG(d.stokes == SW_STOKES_I, "time-major beamformer input: Stokes I only");
+    if(!cb_mfma_supported(d)) {
+        set_error("cb_beamform_packed_tm: shape not supported by the MFMA form");
+        return SW_ERR_UNSUPPORTED;
+    }
+    return launch_mfma<SW_STOKES_I, true>(static_cast<const uint8_t*>(d_ring), static_cast<const uint8_t*>(d_packed),
+                                          static_cast<int8_t*>(d_out), d_scale, d_offset, d_beamset_map, d_ib_raw, d,
+                                          st, ld, w0);
+}
+} // namespace sw
+
 extern "C" int sw_cb_beamform_packed(const void* d_ftpa, const void* d_packed,
                                      const float* d_scale, const float* d_offset,
                                      const int* d_beamset_map, const float* d_ib_raw,
@@ -603,11 +656,11 @@ extern "C" int sw_cb_beamform_packed(const void* d_ftpa, const void* d_packed,
     auto* out = static_cast<int8_t*>(d_out);
     hipStream_t st = sw::as_stream(stream);
     switch(d.stokes) {
-    case SW_STOKES_I: return sw::launch_mfma<SW_STOKES_I>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
-    case SW_STOKES_Q: return sw::launch_mfma<SW_STOKES_Q>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
-    case SW_STOKES_U: return sw::launch_mfma<SW_STOKES_U>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
-    case SW_STOKES_V: return sw::launch_mfma<SW_STOKES_V>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
-    default: return sw::launch_mfma<SW_STOKES_IQUV>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    case SW_STOKES_I: return sw::launch_mfma<SW_STOKES_I, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    case SW_STOKES_Q: return sw::launch_mfma<SW_STOKES_Q, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    case SW_STOKES_U: return sw::launch_mfma<SW_STOKES_U, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    case SW_STOKES_V: return sw::launch_mfma<SW_STOKES_V, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
+    default: return sw::launch_mfma<SW_STOKES_IQUV, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
     }
 }
 

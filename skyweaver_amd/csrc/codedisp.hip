@@ -682,7 +682,8 @@ struct sw_codedisp {
     int8_t* xtm  = nullptr;              // time-major copy of an FTPA input
     std::vector<int8_t*> ytm;            // per DM: output ring [F][PA][ylen], time-major
     std::vector<float2*> ytf;            // per DM, on demand: float output ring, same layout
-    std::vector<int> ylen;               // per DM ring length: T + lat rounded up to 4
+    std::vector<int> ylen;               // per DM ring length: T + lat rounded up to a multiple of T
+    std::vector<int> wstart;             // per DM: ring index of the last run's emitted window
     bool hist_ref = false;               // history read in place from the previous gulp
     const int8_t* prev = nullptr;        // that gulp (null before the first)
     std::vector<int> lat;                // per DM output latency
@@ -800,12 +801,15 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
     }
     p->lat.assign(p->dms.size(), 0);
     p->ylen.assign(p->dms.size(), p->T);
+    p->wstart.assign(p->dms.size(), 0);
     for(auto& g : p->groups) {
         // history: the input a gulp's first segment can reach back to
         p->Oh = std::max(p->Oh, g.lat > 0 ? g.N : g.O);
         for(int d : g.dms) {
             p->lat[d] = g.lat;
-            p->ylen[d] = (p->T + g.lat + 3) / 4 * 4;  // 8-byte aligned rows for the corner turn
+            // a whole number of gulps (>= T + lat): every emitted window is one
+            // contiguous, gulp-aligned slice [w0, w0 + T) of the ring
+            p->ylen[d] = (p->T + g.lat + p->T - 1) / p->T * p->T;
         }
     }
     size_t zmax = 0, slots = 1, vslots = 0;
@@ -913,6 +917,15 @@ static const int8_t* time_major_input(sw_codedisp* plan, const void* d_in, hipSt
     return plan->xtm;
 }
 
+extern "C" int sw_codedisp_window(const sw_codedisp* plan, int dm, const void** d_ring, int* ld, int* offset)
+{
+    SW_CHECK_ARG(plan && dm >= 0 && dm < (int)plan->dms.size(), "bad plan or DM index");
+    if(d_ring) *d_ring = plan->ytm[dm];
+    if(ld) *ld = plan->ylen[dm];
+    if(offset) *offset = plan->wstart[dm];
+    return SW_OK;
+}
+
 extern "C" int sw_codedisp_push_history(sw_codedisp* plan, const void* d_in, void* stream)
 {
     SW_CHECK_ARG(plan && d_in, "null argument");
@@ -986,7 +999,6 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const*
             const int R1 = sw::reg_r1(g.N, g.RP, plan->PA);
             for(int k = 0; k < nb; ++k) {
                 const int d = todo[b0 + k];
-                SW_CHECK_ARG(d_out[d], "null output buffer");
                 p.chirp[k] = plan->chirp[d];
                 p.out[k]   = plan->ytm[d];
                 if(d_out_f32 && d_out_f32[d] && !plan->ytf[d]) {
@@ -1031,12 +1043,15 @@ extern "C" int sw_codedisp_run(sw_codedisp* plan, const void* d_in, void* const*
                     SW_LAUNCH_CHECK();
                 }
             }
-            // emitted window, ring [gT mod R, +T) (two pieces when it wraps),
-            // -> FTPA for the beamformers
+            // emitted window, ring [gT mod R, +T) (two pieces if it wraps, which
+            // the gulp-multiple ring length rules out), -> FTPA for the
+            // beamformers unless d_out[d] is null (left in the ring)
             for(int k = 0; k < nb; ++k) {
                 const int d = todo[b0 + k];
                 const long long R = plan->ylen[d];
                 const int start = (int)(gT % R);
+                plan->wstart[d] = start;
+                if(!d_out[d]) continue;
                 const int n1 = (int)std::min<long long>(plan->T, R - start), n2 = plan->T - n1;
                 const size_t PA = plan->PA;
                 rc = sw::corner_turn(plan->ytm[d] + (size_t)start * 2, d_out[d], plan->F, plan->PA, n1, st,

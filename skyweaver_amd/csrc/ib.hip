@@ -183,6 +183,92 @@ ib_kernel(const uint8_t* __restrict__ ftpa, float* __restrict__ raw, int8_t* __r
     }
 }
 
+// Time-major input (the coherent dedisperser's ring): sample t of column
+// c = p A + a of channel f at ring + ((f 2A + c) ld + w0 + t) 2.  Stokes I
+// with 0/1 antenna weights only.  One lane per output sample t' of channel
+// f; a lane reads its ts consecutive samples of every column (64 lanes read
+// 64 ts 2 contiguous bytes), sums re^2 + im^2 of the column with
+// v_dot4_i32_i8 and adds the column sum to the beamsets that weight the
+// antenna 1.  As in ib_kernel the integer total equals the oracle's float
+// chain while below 2^24; a lane at or above it recomputes its outputs with
+// the float chain in the oracle's order (time, then antenna ascending).
+constexpr int kTmThreads = 256;
+constexpr int kTmMaxCols = 256;  // 128 antennas
+
+template <int SMAX, bool V16>
+__global__ void __launch_bounds__(kTmThreads)
+ib_tm_kernel(const uint8_t* __restrict__ ring, long ld, long w0, float* __restrict__ raw,
+             int8_t* __restrict__ out, const float* __restrict__ scale, const float* __restrict__ offset,
+             const float* __restrict__ antw, int nchans, int nsamples, int nant, int ts, int nbeamsets)
+{
+    __shared__ uint8_t colw[SMAX][kTmMaxCols];  // weight (0/1) of column c in beamset s
+    const int PA = 2 * nant;
+    for(int i = threadIdx.x; i < nbeamsets * PA; i += kTmThreads) {
+        const int sb = i / PA, c = i % PA;
+        colw[sb][c] = antw[sb * nant + c % nant] != 0.0f ? 1 : 0;
+    }
+    __syncthreads();
+    const int f = blockIdx.y;
+    const int nsamps_out = nsamples / ts;
+    const int tp = blockIdx.x * kTmThreads + threadIdx.x;
+    if(tp >= nsamps_out) return;  // after the only barrier
+    const uint8_t* base = ring + ((size_t)f * PA * ld + w0 + (size_t)tp * ts) * 2;
+    int isum[SMAX];
+#pragma unroll
+    for(int s = 0; s < SMAX; ++s) isum[s] = 0;
+#pragma unroll 4
+    for(int c = 0; c < PA; ++c) {
+        const uint8_t* x = base + (size_t)c * ld * 2;
+        int acc = 0;  // sum over the window of re^2 + im^2 of column c
+        if constexpr(V16) {  // ts % 8 == 0: 16-byte aligned runs
+            for(int k = 0; k < ts * 2; k += 16) {
+                const uint4 v = *reinterpret_cast<const uint4*>(x + k);
+                acc = __builtin_amdgcn_sdot4((int)v.x, (int)v.x, acc, false);
+                acc = __builtin_amdgcn_sdot4((int)v.y, (int)v.y, acc, false);
+                acc = __builtin_amdgcn_sdot4((int)v.z, (int)v.z, acc, false);
+                acc = __builtin_amdgcn_sdot4((int)v.w, (int)v.w, acc, false);
+            }
+        } else {
+            for(int k = 0; k < ts; ++k) {
+                const int v = (int)*reinterpret_cast<const uint16_t*>(x + 2 * k);
+                acc = __builtin_amdgcn_sdot4(v, v, acc, false);
+            }
+        }
+#pragma unroll
+        for(int s = 0; s < SMAX; ++s)
+            if(s < nbeamsets && colw[s][c]) isum[s] += acc;
+    }
+    float power[SMAX];
+    bool big = false;
+#pragma unroll
+    for(int s = 0; s < SMAX; ++s) {
+        big = big || isum[s] >= (1 << 24);
+        power[s] = (float)isum[s];
+    }
+    if(big) {  // the oracle's float chain
+#pragma unroll
+        for(int s = 0; s < SMAX; ++s) power[s] = 0.0f;
+        for(int tt = 0; tt < ts; ++tt)
+            for(int a = 0; a < nant; ++a) {
+                const uint16_t d0 = *reinterpret_cast<const uint16_t*>(base + ((size_t)a * ld + tt) * 2);
+                const uint16_t d1 = *reinterpret_cast<const uint16_t*>(base + ((size_t)(nant + a) * ld + tt) * 2);
+                const float st = stokes<SW_STOKES_I>((float)(int8_t)d0, (float)(int8_t)(d0 >> 8),
+                                                     (float)(int8_t)d1, (float)(int8_t)(d1 >> 8));
+#pragma unroll
+                for(int s = 0; s < SMAX; ++s)
+                    if(s < nbeamsets) power[s] += st * antw[s * nant + a];
+            }
+    }
+#pragma unroll
+    for(int s = 0; s < SMAX; ++s) {
+        if(s < nbeamsets) {
+            const size_t o = ((size_t)s * nsamps_out + tp) * nchans + f;
+            raw[o] = power[s];
+            out[o] = clamp_i8(rescale<SW_STOKES_I>(power[s], offset[s * nchans + f], scale[s * nchans + f]));
+        }
+    }
+}
+
 template <int S>
 int launch_ib(const uint8_t* ftpa, float* raw, int8_t* out, const float* scale,
               const float* offset, const float* antw, const sw_bf_dims& d, hipStream_t st)
@@ -208,6 +294,44 @@ int launch_ib(const uint8_t* ftpa, float* raw, int8_t* out, const float* scale,
 }
 
 } // namespace
+
+// Incoherent beam from the dedisperser's time-major ring (see ib_tm_kernel):
+// Stokes I, 0/1 antenna weights, d.nantennas <= 128.
+int ib_beamform_tm(const void* d_ring, long ld, long w0, float* d_raw, void* d_out, const float* d_scale,
+                   const float* d_offset, const float* d_ant_weights, const sw_bf_dims& d, hipStream_t st)
+{
+    SW_CHECK_ARG(d_ring && d_raw && d_out && d_scale && d_offset && d_ant_weights, "null buffer");
+    SW_CHECK_ARG(d.stokes == SW_STOKES_I && 2 * d.nantennas <= kTmMaxCols, "time-major IB: Stokes I, <= 128 antennas");
+    SW_CHECK_ARG(d.nbeamsets > 0 && d.nbeamsets <= 16 && d.nchans > 0 && d.nchans <= 65535, "bad dimensions");
+    SW_CHECK_ARG(d.tscrunch > 0 && d.nsamples % d.tscrunch == 0, "nsamples must be a multiple of tscrunch");
+    SW_CHECK_ARG((reinterpret_cast<uintptr_t>(d_ring) & 15) == 0 && ld % 8 == 0 && w0 % 8 == 0,
+                 "ring must be 16-byte aligned with 8-sample aligned rows");
+    const int nout = d.nsamples / d.tscrunch;
+    const dim3 grid((nout + kTmThreads - 1) / kTmThreads, d.nchans);
+    const bool v16 = d.tscrunch % 8 == 0;
+    auto* in = static_cast<const uint8_t*>(d_ring);
+    auto* out = static_cast<int8_t*>(d_out);
+#define SW_IBT(SM)                                                                                         \
+    if(v16)                                                                                                \
+        hipLaunchKernelGGL((ib_tm_kernel<SM, true>), grid, dim3(kTmThreads), 0, st, in, ld, w0, d_raw, out, \
+                           d_scale, d_offset, d_ant_weights, d.nchans, d.nsamples, d.nantennas, d.tscrunch, \
+                           d.nbeamsets);                                                                   \
+    else                                                                                                   \
+        hipLaunchKernelGGL((ib_tm_kernel<SM, false>), grid, dim3(kTmThreads), 0, st, in, ld, w0, d_raw, out,\
+                           d_scale, d_offset, d_ant_weights, d.nchans, d.nsamples, d.nantennas, d.tscrunch, \
+                           d.nbeamsets);
+    if(d.nbeamsets <= 1) {
+        SW_IBT(1)
+    } else if(d.nbeamsets <= 4) {
+        SW_IBT(4)
+    } else {
+        SW_IBT(16)
+    }
+#undef SW_IBT
+    SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
 } // namespace sw
 
 extern "C" int sw_ib_beamform(const void* d_ftpa, float* d_raw, void* d_out,

@@ -17,6 +17,7 @@
 // the gulp), and the scalings are indexed per beamset (beamset*F + f).
 #include "sw_common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -27,6 +28,11 @@
 namespace sw {
 int cb_mfma_supported(const sw_bf_dims& d);
 size_t fpat_stats_workspace(int nchans, int npol, int nant);
+int ib_beamform_tm(const void* d_ring, long ld, long w0, float* d_raw, void* d_out, const float* d_scale,
+                   const float* d_offset, const float* d_ant_weights, const sw_bf_dims& d, hipStream_t st);
+int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_packed, const float* d_scale,
+                          const float* d_offset, const int* d_beamset_map, const float* d_ib_raw, void* d_out,
+                          const sw_bf_dims& d, hipStream_t st);
 }
 
 struct sw_pipeline {
@@ -457,7 +463,13 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
     float* ib_offsets = p->scl + 2 * S * F;
     float* ib_scaling = p->scl + 3 * S * F;
 
+    // Stokes I with 0/1 antenna weights (what delay models give): the MFMA
+    // beamformer and the incoherent beam read the dedisperser's time-major
+    // output ring directly, with no corner turn to FTPA
+    bool ring_direct = p->use_mfma && c.stokes == SW_STOKES_I && 2 * A <= 256 && c.nsamples % 8 == 0;
+    for(float w : p->h_antw) ring_direct = ring_direct && (w == 0.0f || w == 1.0f);
     std::vector<void*> outs(p->ftpa_dd.begin(), p->ftpa_dd.end());
+    if(ring_direct) std::fill(outs.begin(), outs.end(), nullptr);
     rc = sw_codedisp_run(p->cd, p->ftpa, outs.data(), nullptr, 0, c.ndms, c.batched_dms, st);
     if(rc) return rc;
     rc = sw_codedisp_push_history(p->cd, p->ftpa, st);
@@ -471,10 +483,21 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
         sw_codedisp_info(p->cd, dm, &fl, &ov, &lat);
         const double tsamp = c.nchans / c.bandwidth;
         SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm], st));
-        rc = sw_ib_beamform(p->ftpa_dd[dm], p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, &d, st);
+        const void* ring = nullptr;
+        int ld = 0, w0 = 0;
+        if(ring_direct) {
+            rc = sw_codedisp_window(p->cd, dm, &ring, &ld, &w0);
+            if(rc) return rc;
+            rc = sw::ib_beamform_tm(ring, ld, w0, p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, d, st);
+        } else {
+            rc = sw_ib_beamform(p->ftpa_dd[dm], p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, &d, st);
+        }
         if(rc) return rc;
         SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm + 1], st));
-        if(p->use_mfma)
+        if(ring_direct)
+            rc = sw::cb_beamform_packed_tm(ring, ld, w0, p->packed, cb_scaling, cb_offsets, p->bmap, p->ib_raw,
+                                           p->cb_out, d, st);
+        else if(p->use_mfma)
             rc = sw_cb_beamform_packed(p->ftpa_dd[dm], p->packed, cb_scaling, cb_offsets, p->bmap,
                                        p->ib_raw, p->cb_out, &d, st);
         else

@@ -122,19 +122,23 @@ def test_codedisp_dm0_is_identity(sw, dev):
         np.testing.assert_array_equal(outs[k].download(), exp)
 
 
-def test_codedisp_uneven_batch_and_history(sw, orc, dev):
+@pytest.mark.parametrize("paired", [False, True], ids=["own_overlap", "paired_overlap"])
+def test_codedisp_uneven_batch_and_history(sw, orc, dev, paired):
     """Eight DMs with uneven FFT lengths over two gulps: batched == one DM per
-    launch bit for bit, and both match the oracle on the continuous stream."""
+    launch bit for bit, and both match the oracle on the continuous stream.
+    paired: the DMs of each FFT length share one overlap (the larger), so the
+    batched launches share each forward transform between two DMs."""
     from skyweaver_amd import DeviceBuffer as DB
     F, A, T = 2, 64, 24576
     dms = [0.0, 5.0, 20.0, 50.0, 100.0, 180.0, 260.0, 350.0]
     Ns = [4096, 4096, 8192, 8192, 16384, 16384, 32768, 32768]
-    plan_b = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, fft_len=Ns)
-    plan_u = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, fft_len=Ns)
+    ov = [2, 2, 10, 10, 32, 32, 62, 62] if paired else None
+    plan_b = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, fft_len=Ns, overlap=ov)
+    plan_u = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, fft_len=Ns, overlap=ov)
     Os = [plan_b.info(d)[1] for d in range(8)]
-    assert Os == [0, 2, 4, 10, 18, 32, 46, 62]
+    assert Os == (ov if paired else [0, 2, 4, 10, 18, 32, 46, 62])
     lats = [plan_b.latency(d) for d in range(8)]
-    assert lats[0] == 0 and all(lats[d] == Ns[d] - Os[d] for d in range(1, 8))
+    assert lats == [0 if T % (Ns[d] - Os[d]) == 0 else Ns[d] - Os[d] for d in range(8)]
     g1 = sw.synth_voltages((F, T, 2, A, 2), seed=0x5EED0003, sigma=16.0)
     g2 = sw.synth_voltages((F, T, 2, A, 2), seed=0x5EED0013, sigma=16.0)
     for gi, g in enumerate((g1, g2)):

@@ -90,19 +90,23 @@ def test_pipeline_end_to_end(sw, orc, dev):
     assert got["cb"][1][3] == pytest.approx(TSAMP * TS, rel=1e-9)
 
 
-def test_pipeline_forms_identical_and_host_input(sw, dev):
-    """plain (v_dot4, one DM per launch) == chosen (MFMA, batched) bit for bit,
-    and a host-memory gulp gives the same result as a device-resident one."""
+@pytest.mark.parametrize("stokes,sigma", [("IQUV", 16.0), ("I", 16.0), ("I", 100.0)])
+def test_pipeline_forms_identical_and_host_input(sw, dev, stokes, sigma):
+    """plain (v_dot4, one DM per launch, FTPA beamformer input) == chosen
+    (MFMA, batched; for Stokes I with 0/1 antenna weights the beamformers
+    read the dedisperser's time-major ring) bit for bit, and a host-memory
+    gulp gives the same result as a device-resident one.  sigma 100 takes the
+    incoherent beam's sums past 2^24, where it follows the float chain."""
     from skyweaver_amd import DeviceBuffer as DB
     H = T // 256
-    g = sw.synth_voltages((H, A, F, 256, 2, 2), seed=7, sigma=16.0)
+    g = sw.synth_voltages((H, A, F, 256, 2, 2), seed=7, sigma=sigma)
     weights = sw.synth_weights((F, B, A, 2), seed=8)
     bmap = (np.arange(B) % 2).astype(np.int32)
     antw = np.ones((2, A), np.float32)
     antw[1, 40:] = 0
     dms, fft_len = [0.0, 20.0, 50.0], [1024, 1024, 1024]
-    a = _run_pipeline(sw, [DB.from_host(g)], weights, dms, fft_len, "mfma", True, bmap, antw, "IQUV")
-    b = _run_pipeline(sw, [g], weights, dms, fft_len, "plain", False, bmap, antw, "IQUV")
+    a = _run_pipeline(sw, [DB.from_host(g)], weights, dms, fft_len, "mfma", True, bmap, antw, stokes)
+    b = _run_pipeline(sw, [g], weights, dms, fft_len, "plain", False, bmap, antw, stokes)
     for key in ("cb", "ib", "stats"):
         assert len(a[key]) == len(b[key])
         for x, y in zip(a[key], b[key]):

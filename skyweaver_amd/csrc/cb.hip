@@ -132,6 +132,7 @@ struct Epi {
 
 // ======================================================== plain (v_dot4) form
 constexpr int kPlainThreads = 256;  // 4 waves x 64 beams
+constexpr int kBeamLds = 1024;      // beams whose epilogue parameters the MFMA form stages in LDS
 constexpr int kPlainTB      = 64;   // samples staged per LDS pass
 
 template <int S, int A4>
@@ -258,7 +259,7 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
                int8_t* __restrict__ out, const float* __restrict__ scale,
                const float* __restrict__ offset, const int* __restrict__ bmap,
                const float* __restrict__ ib_raw, int nchans, int nsamples, int nbeams,
-               int ntchunks, int ib_sub, long ld, long w0)
+               int ntchunks, int ib_sub, int nbeamsets, long ld, long w0)
 {
     constexpr int NV = nstokes(S);
     constexpr int A  = NSTEPS * 16;  // antennas (K' = 2A = 32 * NSTEPS)
@@ -277,6 +278,13 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     constexpr int ST16   = TMI ? 32 * kMfmaWaves * SPITCH / 16 : 0;
     __shared__ uint4 smem[2 * WL16 > ST16 ? 2 * WL16 : ST16];
     uint4 (*wlds)[WL16] = reinterpret_cast<uint4 (*)[WL16]>(smem);
+    // epilogue parameters of the channel's beams and the workgroup's
+    // incoherent-beam powers, staged once (up to kBeamLds beams)
+    constexpr bool kPre = EpiParams<S, NOUT>::kPreIb;
+    constexpr int NO_WG = 32 * kMfmaWaves / (TS <= 32 ? TS : 32);  // output samples of the workgroup
+    __shared__ int tb_s[kPre ? kBeamLds : 1];
+    __shared__ float tb_sc[kPre ? kBeamLds : 1], tb_off[kPre ? kBeamLds : 1];
+    __shared__ float tb_ib[kPre ? 16 * NO_WG * NV : 1];
     // XCD-aware mapping: consecutive workgroups go round-robin over the 8
     // XCDs; keep all time chunks of a channel on one XCD so its packed
     // weights stay resident in that XCD's L2.
@@ -308,6 +316,25 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     // no exchange between lanes.
     const int trow = 16 * ((row >> 2) & 1) + 4 * (row >> 3) + (row & 3);
 
+    const int nsamps_out = nsamples / TS;
+    const bool tabs = kPre && nbeams <= kBeamLds && nbeamsets <= 16;  // uniform
+    if constexpr(kPre) {
+        if(tabs) {
+            for(int b = tid; b < nbeams; b += kMfmaWaves * 64) {
+                const int sb = bmap[b];
+                tb_s[b]   = sb;
+                tb_sc[b]  = scale[sb * nchans + f];
+                tb_off[b] = ib_sub ? 0.0f : offset[sb * nchans + f];
+            }
+            const int o0 = tc * kMfmaWaves * 32 / TS;
+            for(int i = tid; i < nbeamsets * NO_WG * NV; i += kMfmaWaves * 64) {
+                const int k = i % NV, o = (i / NV) % NO_WG, sb = i / (NV * NO_WG);
+                float v = 0.0f;
+                if(ib_sub && o0 + o < nsamps_out) v = ib_raw[(((size_t)sb * nsamps_out + o0 + o) * nchans + f) * NV + k];
+                tb_ib[i] = v;
+            }
+        }
+    }
     // ---- A operands: 32 time rows x K' for both polarisations
     if constexpr(TMI) {
         // corner turn: 16-byte runs of 8 samples of each column into rows
@@ -363,17 +390,25 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
         corr[p] = c;
     }
 
-    const int nsamps_out = nsamples / TS;
     const int nbt = (nbeams + 31) / 32;
     const uint4* wsrc = reinterpret_cast<const uint4*>(packed + (size_t)f * nbeams * 2 * KP);
     const size_t wtotal = (size_t)nbeams * 2 * KP / 16;  // uint4 of this channel
     // global -> registers for tile bt (beams past the end read as zero)
     auto fetch = [&](int bt, uint4* r) {
+        const uint4* src = wsrc + (size_t)bt * WTILE;
+        if((size_t)(bt + 1) * WTILE <= wtotal) {  // a whole tile (uniform)
 #pragma unroll
-        for(int i = 0; i < WPER; ++i) {
-            const int idx = tid + i * kMfmaWaves * 64;
-            const size_t g = (size_t)bt * WTILE + idx;
-            r[i] = (idx < WTILE && g < wtotal) ? wsrc[g] : make_uint4(0, 0, 0, 0);
+            for(int i = 0; i < WPER; ++i) {
+                const int idx = tid + i * kMfmaWaves * 64;
+                r[i] = (WTILE % (kMfmaWaves * 64) == 0 || idx < WTILE) ? src[idx] : make_uint4(0, 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for(int i = 0; i < WPER; ++i) {
+                const int idx = tid + i * kMfmaWaves * 64;
+                const size_t g = (size_t)bt * WTILE + idx;
+                r[i] = (idx < WTILE && g < wtotal) ? wsrc[g] : make_uint4(0, 0, 0, 0);
+            }
         }
     };
     auto stash = [&](int buf, const uint4* r) {
@@ -399,7 +434,21 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
         const bool bvalid = b < nbeams;
         // epilogue parameters of this lane's beam
         EpiParams<S, NOUT> ep;
-        if(active && bvalid) {
+        if(tabs) {
+            if constexpr(kPre) {
+                if(active && bvalid) {
+                    ep.s = tb_s[b];
+                    ep.sc = tb_sc[b];
+                    ep.off = tb_off[b];
+#pragma unroll
+                    for(int o = 0; o < NOUT; ++o) {
+                        const int ol = TS <= 16 ? (32 * wave + 16 * grp + o * TS) / TS : wave;
+#pragma unroll
+                        for(int k = 0; k < NV; ++k) ep.ib[o][k] = tb_ib[(ep.s * NO_WG + ol) * NV + k];
+                    }
+                }
+            }
+        } else if(active && bvalid) {
             ep.s = bmap[b];
             const int so = ep.s * nchans + f;
             ep.sc = scale[so];
@@ -530,7 +579,7 @@ int launch_mfma_ts(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, cons
     case NS:                                                                               \
         hipLaunchKernelGGL((cb_mfma_kernel<S, TS, NS, TMI>), grid, dim3(kMfmaWaves * 64), 0, \
                            st, ftpa, packed, out, sc, off, bmap, ib, d.nchans,             \
-                           d.nsamples, d.nbeams, ntchunks, d.ib_subtract, ld, w0);         \
+                           d.nsamples, d.nbeams, ntchunks, d.ib_subtract, d.nbeamsets, ld, w0); \
         break;
     switch(d.nantennas / 16) {
         SW_MFMA(1) SW_MFMA(2) SW_MFMA(3) SW_MFMA(4) SW_MFMA(5) SW_MFMA(6) SW_MFMA(7) SW_MFMA(8)

@@ -100,19 +100,21 @@ def _same_outputs(a, b):
                 all(np.array_equal(a[k], b[k]) for k in a))
 
 
-def bench_target_8dm(sw, d_taftp, weights, steps, warmup):
-    """The same gulp with 8 coherent DMs of uneven FFT lengths: chosen form
-    vs plain form, outputs compared on the same gulp of the stream."""
-    log("8 coherent DMs: chosen form")
+def bench_target_8dm(sw, d_taftp, weights, steps, warmup, fft_len=None, label=""):
+    """The same gulp with 8 coherent DMs: chosen form vs plain form (the
+    same FFT lengths for both), outputs compared on the same gulp of the
+    stream."""
+    fft_len = FFT_LEN8 if fft_len is None else fft_len
+    log(f"8 coherent DMs{label}: chosen form")
     ms_c, _, st_c, tfb_c = run_form(sw, d_taftp, weights, "mfma", True, steps, warmup, capture=True,
-                                    dms=DMS8, fft_len=FFT_LEN8)
-    log("8 coherent DMs: plain form")
+                                    dms=DMS8, fft_len=fft_len)
+    log(f"8 coherent DMs{label}: plain form")
     ms_p, _, st_p, tfb_p = run_form(sw, d_taftp, weights, "plain", False, steps, warmup, capture=True,
-                                    dms=DMS8, fft_len=FFT_LEN8)
+                                    dms=DMS8, fft_len=fft_len)
     v_c = B * F * T * len(DMS8) / (ms_c * 1e-3)
     v_p = B * F * T * len(DMS8) / (ms_p * 1e-3)
     return {"workload": "64 antennas x 64 channels x 65536 samples, 800 beams, 8 coherent DMs "
-                        f"{DMS8} (FFT {FFT_LEN8}), tscrunch 16, Stokes I",
+                        f"{DMS8} (FFT {list(fft_len)}), tscrunch 16, Stokes I",
             "steps": steps, "warmup": warmup,
             "chosen": {"ms_per_step": round(ms_c, 4), "value": v_c, "xrealtime": round(T * TSAMP / (ms_c * 1e-3), 2),
                        "stage_ms": {k: round(v, 4) for k, v in st_c.items()}},
@@ -253,9 +255,17 @@ def main():
             gbs = nbytes / (stage[k] * 1e-3) / 1e9
             hbm[k] = {"GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 3)}
 
-    target = None
+    target = target_tuned = None
     if not args.no_8dm and not args.no_plain:
         target = bench_target_8dm(sw, d_taftp, weights, max(2, args.steps // 5), 2)
+        # the same 8 DMs with FFT lengths picked by the autotuner (one set for both forms)
+        log("autotuning the 8 DMs' FFT lengths")
+        tuned, tms = sw.CoherentDedisperser.autotune(F, A, T, CFREQ - BW / 2, BW / F, TSAMP, DMS8, tune_chans=8)
+        target_tuned = bench_target_8dm(sw, d_taftp, weights, max(2, args.steps // 5), 2, fft_len=tuned,
+                                        label=" (autotuned FFT lengths)")
+        target_tuned["autotune_ms_per_gulp"] = {
+            "fft_len": [1024, 2048, 4096, 8192, 16384, 32768],
+            "per_dm": {str(dm): [round(float(x), 3) for x in tms[d]] for d, dm in enumerate(DMS8)}}
 
     idisp = None
     if not args.no_idisp:
@@ -301,6 +311,7 @@ def main():
         "plain": plain,
         "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
         "target_8dm": target,
+        "target_8dm_autotuned_fft": target_tuned,
         "incoherent_dedispersion": idisp,
         "cpu_baseline": cpu,
     }

@@ -276,6 +276,15 @@ typedef struct sw_codedisp sw_codedisp;
 
 int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, sw_codedisp** plan);
 int sw_codedisp_destroy(sw_codedisp* plan);
+/* FFT-length autotuner (no reference counterpart; the reference uses one
+ * length for the whole gulp).  For every DM of cfg, with the overlap the
+ * plan would choose, times one gulp of synthetic input through the
+ * dedisperser on min(tune_chans, nchans) channels for each length in
+ * {1024, ..., 32768} of at least 4 O, and writes the fastest to
+ * fft_len_out[ndms]; ms_out (NULL allowed) receives [ndms][6] times per
+ * gulp scaled to nchans (-1: length not tried). */
+int sw_codedisp_autotune(const sw_codedisp_config* cfg, int tune_chans, int* fft_len_out,
+                         float* ms_out, void* stream);
 int sw_codedisp_info(const sw_codedisp* plan, int dm_idx, int* fft_len, int* overlap,
                      int* latency);
 /* Dedisperse the gulp d_ftpa (FTPA [F][T][2][A]) for the DMs
@@ -417,6 +426,9 @@ typedef struct sw_pipeline_config {
     const float* const* id_dms;
     const int* id_ndms;
     int id_dispatch_size;   /* 0: max(10 max_delay, 8192) */
+    /* 1: when fft_len is NULL, pick each coherent DM's FFT length with
+     * sw_codedisp_autotune (on 8 channels) at create time */
+    int autotune_fft;
 } sw_pipeline_config;
 
 typedef struct sw_obs_header {

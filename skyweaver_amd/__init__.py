@@ -98,6 +98,7 @@ def _declare(L):
     L.sw_codedisp_create.argtypes = [ctypes.POINTER(CodedispConfig), _vp, _pvp]
     L.sw_codedisp_destroy.argtypes = [_vp]
     L.sw_codedisp_info.argtypes = [_vp, _c, ctypes.POINTER(_c), ctypes.POINTER(_c), ctypes.POINTER(_c)]
+    L.sw_codedisp_autotune.argtypes = [ctypes.POINTER(CodedispConfig), _c, _vp, _vp, _vp]
     L.sw_codedisp_run.argtypes = [_vp, _vp, _pvp, _pvp, _c, _c, _c, _vp]
     L.sw_codedisp_push_history.argtypes = [_vp, _vp, _vp]
     L.sw_codedisp_set_history.argtypes = [_vp, _vp, _vp]
@@ -512,6 +513,24 @@ class CoherentDedisperser:
         check(lib().sw_codedisp_info(self.handle, d, ctypes.byref(N), ctypes.byref(O), ctypes.byref(L)))
         return N.value, O.value
 
+    @staticmethod
+    def autotune(nchans, nantennas, nsamples, low_freq, chan_bw, tsamp, dms, overlap=None, tune_chans=8,
+                 input_fpat=True, stream=None):
+        """sw_codedisp_autotune: (fastest FFT length per DM, times [ndms][6] in ms per gulp for
+        lengths 1024..32768, -1 where not tried)."""
+        import numpy as np
+        dm = np.ascontiguousarray(dms, np.float32)
+        n = len(dm)
+        ov = np.ascontiguousarray(overlap if overlap is not None else [-1] * n, np.int32)
+        cfg = CodedispConfig(nchans, nantennas, nsamples, low_freq, chan_bw, tsamp, n,
+                             dm.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), None,
+                             ov.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 1 if input_fpat else 0, 0)
+        out = np.zeros(n, np.int32)
+        ms = np.zeros((n, 6), np.float32)
+        check(lib().sw_codedisp_autotune(ctypes.byref(cfg), tune_chans, out.ctypes.data, ms.ctypes.data,
+                                         _stream(stream)))
+        return [int(x) for x in out], ms
+
     def latency(self, d):
         """Output latency of DM d in samples: a gulp g emits stream samples [gT - L, (g+1)T - L)."""
         L = ctypes.c_int()
@@ -555,7 +574,7 @@ class PipelineConfigC(ctypes.Structure):
                 ("coherent_dms", ctypes.POINTER(ctypes.c_float)), ("fft_len", ctypes.POINTER(ctypes.c_int)),
                 ("overlap", ctypes.POINTER(ctypes.c_int)), ("cb_form", ctypes.c_int), ("batched_dms", ctypes.c_int),
                 ("id_enable", ctypes.c_int), ("id_tscrunch", ctypes.c_void_p), ("id_dms", ctypes.c_void_p),
-                ("id_ndms", ctypes.c_void_p), ("id_dispatch_size", ctypes.c_int)]
+                ("id_ndms", ctypes.c_void_p), ("id_dispatch_size", ctypes.c_int), ("autotune_fft", ctypes.c_int)]
 
 
 class ObsHeaderC(ctypes.Structure):
@@ -599,8 +618,9 @@ class BeamformerPipeline:
 
     def __init__(self, nantennas, nchans, nbeams, nsamples, tscrunch, dms, centre_frequency, bandwidth,
                  stokes="I", ib_subtract=True, output_level=24.0, fft_len=None, overlap=None, form="mfma",
-                 batched=True, total_nchans=4096, id_plan=None, id_dispatch_size=0):
-        """id_plan: None (no incoherent dedispersion) or, per coherent DM, (tscrunch, incoherent DMs)."""
+                 batched=True, total_nchans=4096, id_plan=None, id_dispatch_size=0, autotune_fft=False):
+        """id_plan: None (no incoherent dedispersion) or, per coherent DM, (tscrunch, incoherent DMs).
+        autotune_fft (with fft_len None): FFT lengths chosen by sw_codedisp_autotune."""
         import numpy as np
         L = lib()
         if not hasattr(L, "_pipeline_declared"):
@@ -617,6 +637,7 @@ class BeamformerPipeline:
                               self._fft.ctypes.data_as(ip) if self._fft is not None else None,
                               self._ov.ctypes.data_as(ip) if self._ov is not None else None,
                               FORMS[form], 1 if batched else 0)
+        cfg.autotune_fft = 1 if autotune_fft else 0
         if id_plan is not None:
             assert len(id_plan) == n
             self._id_ts = np.array([b[0] for b in id_plan], np.int32)

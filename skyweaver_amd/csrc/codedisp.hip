@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <vector>
@@ -869,6 +870,100 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
     }
     SW_HIP_TRY(hipStreamSynchronize(st));
     *out = p.release();
+    return SW_OK;
+}
+
+namespace sw {
+namespace {
+// deterministic pseudo-random complex int8 (a 32-bit mix of the index)
+__global__ void tune_fill_kernel(uint32_t* x, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    uint32_t h = (uint32_t)i * 0x9E3779B1u;
+    h ^= h >> 15;
+    h *= 0x85EBCA77u;
+    h ^= h >> 13;
+    x[i] = h & 0x3F3F3F3Fu;  // |value| < 64: no int8 saturation in the transforms
+}
+} // namespace
+} // namespace sw
+
+extern "C" int sw_codedisp_autotune(const sw_codedisp_config* cfg, int tune_chans, int* fft_len_out,
+                                    float* ms_out, void* stream)
+{
+    SW_CHECK_ARG(cfg && fft_len_out && cfg->ndms > 0 && cfg->dms && tune_chans > 0, "bad argument");
+    hipStream_t st = sw::as_stream(stream);
+    const int nd = cfg->ndms;
+    static const int kCand[] = {1024, 2048, 4096, 8192, 16384, 32768};
+    constexpr int kNc = sizeof(kCand) / sizeof(kCand[0]);
+    // the overlaps the plan would choose
+    std::vector<int> O(nd);
+    for(int d = 0; d < nd; ++d)
+        O[d] = (cfg->overlap && cfg->overlap[d] >= 0)
+                   ? cfg->overlap[d]
+                   : sw::even_ceil(sw::smear_samples(cfg->low_freq, cfg->chan_bw, cfg->tsamp, cfg->dms[d]));
+    sw_codedisp_config sub = *cfg;
+    sub.nchans = std::min(tune_chans, cfg->nchans);
+    sub.overlap = O.data();
+    sub.history_in_place = 0;
+    const size_t words = (size_t)sub.nchans * 2 * cfg->nantennas * cfg->nsamples / 2;
+    uint32_t* d_in = nullptr;
+    SW_HIP_TRY(hipMalloc(&d_in, words * 4));
+    hipLaunchKernelGGL(sw::tune_fill_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, d_in, words);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    std::vector<float> best(nd, 1e30f), times((size_t)nd * kNc, -1.0f);
+    std::vector<int> N(nd, 0);
+    std::vector<void*> outs(nd, nullptr);  // outputs stay in the ring, as in the pipeline
+    int rc = SW_OK;
+    for(int c = 0; c < kNc && rc == SW_OK; ++c) {
+        // DMs for which this length keeps the overlap waste below a quarter
+        std::vector<int> fl(nd), use;
+        for(int d = 0; d < nd; ++d) {
+            const bool ok = kCand[c] >= 4 * O[d];
+            fl[d] = ok ? kCand[c] : 0;
+            if(ok) use.push_back(d);
+        }
+        if(use.empty()) continue;
+        sub.fft_len = fl.data();
+        sw_codedisp* plan = nullptr;
+        rc = sw_codedisp_create(&sub, stream, &plan);
+        if(rc) break;
+        for(int d : use) {
+            float acc = 0.0f;
+            for(int it = 0; it < 5 && rc == SW_OK; ++it) {  // two warm-up gulps, three timed
+                if(it >= 2) (void)hipEventRecord(e0, st);
+                rc = sw_codedisp_run(plan, d_in, outs.data(), nullptr, d, 1, 0, stream);
+                if(rc) break;
+                if(it >= 2) {
+                    (void)hipEventRecord(e1, st);
+                    (void)hipEventSynchronize(e1);
+                    float ms = 0.0f;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    acc += ms;
+                }
+                rc = sw_codedisp_push_history(plan, d_in, stream);
+            }
+            const float ms = acc / 3.0f * (float)cfg->nchans / (float)sub.nchans;
+            times[(size_t)d * kNc + c] = ms;
+            if(rc == SW_OK && ms < best[d]) {
+                best[d] = ms;
+                N[d] = kCand[c];
+            }
+        }
+        sw_codedisp_destroy(plan);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(d_in);
+    if(rc) return rc;
+    for(int d = 0; d < nd; ++d) {
+        SW_CHECK_ARG(N[d] > 0, "autotune: no FFT length fits the overlap");
+        fft_len_out[d] = N[d];
+    }
+    if(ms_out) std::memcpy(ms_out, times.data(), sizeof(float) * times.size());
     return SW_OK;
 }
 

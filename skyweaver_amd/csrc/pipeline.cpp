@@ -212,6 +212,13 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
     cd.dms       = p->dms.data();
     cd.fft_len   = p->cfg.fft_len;
     cd.overlap   = p->cfg.overlap;
+    std::vector<int> tuned(c.ndms, 0);
+    if(c.autotune_fft && !c.fft_len) {
+        cd.input_fpat = 1;
+        int rc = sw_codedisp_autotune(&cd, 8, tuned.data(), nullptr, p->stream);
+        if(rc) return rc;
+        cd.fft_len = tuned.data();
+    }
     cd.input_fpat = 1;
     cd.history_in_place = 1;
     int rc = sw_codedisp_create(&cd, p->stream, &p->cd);

@@ -197,3 +197,28 @@ def test_codedisp_history_in_place(sw, orc, dev, T, dms, Ns):
             ei, ef = _oracle_stream(orc, np.zeros((F, O, 2, A, 2), np.int8), np.concatenate(gulps, 1), chirp, N, O)
             ei, ef = _window(ei, ef, 2 * T - plan_r.latency(d), T)
             _check(orr[d].download(), orf[d].download(), ei, ef, f"in place gulp 3 DM {dms[d]} N {N}")
+
+
+def test_codedisp_autotune(sw, dev):
+    """The FFT-length autotuner returns, per DM, a timed power-of-two length
+    of at least 4 O (the fastest of those it timed), and a pipeline built
+    with autotune_fft runs."""
+    F, A, T = 4, 16, 32768
+    dms = [0.0, 50.0, 350.0]
+    lens, ms = sw.CoherentDedisperser.autotune(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, tune_chans=2)
+    cands = [1024, 2048, 4096, 8192, 16384, 32768]
+    plan = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms)
+    for d in range(len(dms)):
+        O = plan.info(d)[1]
+        assert lens[d] in cands and lens[d] >= 4 * O
+        tried = [c for c in range(6) if ms[d][c] >= 0]
+        assert all(cands[c] >= 4 * O for c in tried) and ms[d][cands.index(lens[d])] == min(ms[d][c] for c in tried)
+    from skyweaver_amd import DeviceBuffer as DB
+    p = sw.BeamformerPipeline(A, F, 32, 4096, 16, dms, 1284e6, CHAN_BW * F, autotune_fft=True)
+    p.set_weights(sw.synth_weights((F, 32, A, 2), seed=1))
+    p.init(A)
+    got = []
+    p.set_sinks(cb=lambda blk: got.append(blk.dm_idx))
+    p(DB.from_host(sw.synth_voltages((16, A, F, 256, 2, 2), seed=2, sigma=16.0)))
+    p.close()
+    assert got == [0, 1, 2]

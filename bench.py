@@ -42,7 +42,7 @@ VALU_ADD16_PEAK = 78.6e12   # 16-bit lane adds/s: 256 CUs x 64 lanes x 2 lanes p
 # HBM bytes per cb_mfma_kernel launch at this workload, from rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE calibrated on a 1 GiB copy kernel
 # (profiles/hbm_traffic_pipeline_64ant_64ch_65536samp_800beam_dm50.md)
-CB_TRAFFIC_BYTES = 1.330e9
+CB_TRAFFIC_BYTES = 1.331e9
 
 # incoherent dedispersion workload: 256 channels (4 x 64 over 53.5 MHz at
 # 1284 MHz), 512 beams, CB tscrunch 8, 1024 DM trials 0:1000:0.9765625, one
@@ -206,6 +206,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-idisp", action="store_true", help="skip the incoherent dedispersion measurement")
     ap.add_argument("--no-8dm", action="store_true", help="skip the 8-DM chosen vs plain measurement")
+    ap.add_argument("--only-8dm", action="store_true",
+                    help="time only the chosen form on the 8-DM workload (for kernel traces) and print its line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path is built for one GPU); --gpus {args.gpus} is not "
@@ -222,6 +224,12 @@ def main():
     taftp = sw.synth_voltages((H, A, F, 256, 2, 2), seed=SEED_V, sigma=16.0, nthreads=16)
     weights = sw.synth_weights((F, B, A, 2), seed=SEED_W)
     d_taftp = DeviceBuffer.from_host(taftp)
+    if args.only_8dm:
+        ms8, _, st8, _ = run_form(sw, d_taftp, weights, "mfma", True, args.steps, args.warmup, dms=DMS8,
+                                  fft_len=FFT_LEN8)
+        print(json.dumps({"workload": f"8 coherent DMs {DMS8} (FFT {FFT_LEN8}), chosen form",
+                          "ms_per_step": round(ms8, 4), "stage_ms": {k: round(v, 4) for k, v in st8.items()}}))
+        return
 
     log("chosen form (MFMA beamformer, batched DMs)")
     ms, wall, stage, tfb_c = run_form(sw, d_taftp, weights, "mfma", True, args.steps, args.warmup,

@@ -14,6 +14,11 @@
 //                                   cpp/skyweaver/detail/MultiFileWriter.cu:34-248,
 //                                   cpp/skyweaver/src/FileOutputStream.cpp:30-147
 //   InvalidDelayEpoch         cpp/skyweaver/DelayManager.cuh:19-34
+//   read_dada_header / validate_header / update_config
+//                             cpp/skyweaver/src/ObservationHeader.cpp:6-69
+//   MultiFileReader           cpp/skyweaver/MultiFileReader.cuh,
+//                             cpp/skyweaver/src/MultiFileReader.cu:12-303
+//   run_pipeline              cpp/skyweaver/src/skyweaver_cli.cu:109-240
 //
 // Differences that follow from the design: the reference's compile-time
 // parameters (SKYWEAVER_NANTENNAS, _NCHANS, _NBEAMS, _CB_TSCRUNCH, ...) are
@@ -35,11 +40,13 @@
 #include <fstream>
 #include <functional>
 #include <iomanip>
+#include <limits>
 #include <map>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace skyweaver_amd {
@@ -202,6 +209,16 @@ class PipelineConfig {
     std::vector<int> const& fft_lengths() const { return _fft_len; }
     void fft_lengths(std::vector<int> const& v) { _fft_len = v; }
 
+    // input (MultiFileReader, run_pipeline)
+    std::vector<std::string> const& input_files() const { return _input_files; }
+    void input_files(std::vector<std::string> const& f) { _input_files = f; }
+    bool check_input_contiguity() const { return _check_contiguity; }
+    void check_input_contiguity(bool c) { _check_contiguity = c; }
+    float start_time() const { return _start_time; }
+    void start_time(float t) { _start_time = t; }
+    float duration() const { return _duration; }
+    void duration(float d) { _duration = d; }
+
   private:
     std::size_t _nantennas = 64, _nchans = 64, _nbeams = 64, _tscrunch = 1;
     bool _ib_subtract = true;
@@ -215,21 +232,112 @@ class PipelineConfig {
     float _level = 24.0f;
     int _cb_form = SW_CB_MFMA;
     std::vector<int> _fft_len;
+    std::vector<std::string> _input_files;
+    bool _check_contiguity = false;
+    float _start_time = 0.0f, _duration = std::numeric_limits<float>::infinity();
 };
 
 // ------------------------------------------------------- ObservationHeader
 struct ObservationHeader {
-    std::size_t nantennas = 0;
     std::size_t nchans = 0;
-    std::size_t obs_nchans = 0;
+    std::size_t npol = 0;
+    std::size_t nbits = 0;
+    std::size_t nantennas = 0;
+    std::size_t sample_clock_start = 0;
     std::size_t chan0_idx = 0;
-    long double sample_clock = 1712e6;
-    long double utc_start = 0;
-    long double frequency = 0;
+    std::size_t obs_nchans = 0;
     long double bandwidth = 0;
-    long double obs_frequency = 0;
     long double obs_bandwidth = 0;
+    long double frequency = 0;
+    long double obs_frequency = 0;
+    long double tsamp = 0;
+    long double sample_clock = 1712e6;
+    long double sync_time = 0;
+    long double utc_start = 0;
+    long double mjd_start = 0;
+    std::size_t obs_offset = 0;  // bytes of this file's data from the observation start
+    std::string source_name, ra, dec, telescope, instrument;
 };
+
+namespace detail {
+// psrdada's ascii_header_get(header, key, "%s", buf): the first line whose
+// first word is `key`; the value is the next whitespace-delimited word.
+inline std::string dada_header_value(char const* text, std::size_t n, char const* key)
+{
+    const std::string hdr(text, strnlen(text, n));
+    const std::size_t klen = std::strlen(key);
+    std::size_t pos = 0;
+    while(pos < hdr.size()) {
+        std::size_t end = hdr.find('\n', pos);
+        if(end == std::string::npos) end = hdr.size();
+        std::size_t b = pos;
+        while(b < end && (hdr[b] == ' ' || hdr[b] == '\t')) ++b;
+        if(end - b > klen && hdr.compare(b, klen, key) == 0 && std::isspace((unsigned char)hdr[b + klen])) {
+            std::istringstream ss(hdr.substr(b + klen, end - b - klen));
+            std::string v;
+            if(ss >> v) {
+                if(v == "unset") throw std::runtime_error(std::string("The header key ") + key + " was unset");
+                return v;
+            }
+        }
+        pos = end + 1;
+    }
+    throw std::runtime_error(std::string("Could not find ") + key + " key in DADA header.");
+}
+} // namespace detail
+
+// read_dada_header (cpp/skyweaver/src/ObservationHeader.cpp:6-35): every key
+// is required; integers parse as strtoul(.., 0), reals as strtold.
+inline void read_dada_header(char const* text, std::size_t nbytes, ObservationHeader& h)
+{
+    auto u = [&](char const* k) { return (std::size_t)std::strtoul(detail::dada_header_value(text, nbytes, k).c_str(), nullptr, 0); };
+    auto r = [&](char const* k) { return std::strtold(detail::dada_header_value(text, nbytes, k).c_str(), nullptr); };
+    auto str = [&](char const* k) { return detail::dada_header_value(text, nbytes, k); };
+    h.nchans = u("NCHAN");
+    h.obs_nchans = u("OBS_NCHAN");
+    h.npol = u("NPOL");
+    h.nbits = u("NBIT");
+    h.nantennas = u("NANT");
+    h.sample_clock_start = u("SAMPLE_CLOCK_START");
+    h.bandwidth = r("BW");
+    h.obs_bandwidth = r("OBS_BW");
+    h.frequency = r("FREQ");
+    h.obs_frequency = r("OBS_FREQ");
+    h.tsamp = r("TSAMP");
+    h.sample_clock = r("SAMPLE_CLOCK");
+    h.sync_time = r("SYNC_TIME");
+    h.utc_start = r("UTC_START");
+    h.mjd_start = r("MJD_START");
+    h.source_name = str("SOURCE");
+    h.ra = str("RA");
+    h.dec = str("DEC");
+    h.telescope = str("TELESCOPE");
+    h.instrument = str("INSTRUMENT");
+    h.chan0_idx = u("CHAN0_IDX");
+    h.obs_offset = u("OBS_OFFSET");
+}
+
+// validate_header / update_config (cpp/skyweaver/src/ObservationHeader.cpp:37-69)
+inline void validate_header(ObservationHeader const& header, PipelineConfig const& config)
+{
+    if(header.nantennas > config.nantennas())
+        throw std::runtime_error("Input data contains too many antennas for current build, "
+                                 "-DSKYWEAVER_NANTENNAS should be larger than the data nantennas");
+    if(header.nchans != config.nchans())
+        throw std::runtime_error("Input data contains incorrect number of channels for current build, "
+                                 "-DSKYWEAVER_NCHANS should be equal to the data nchans");
+    if(header.npol != config.npol())
+        throw std::runtime_error("Input data contains incorrect number of polarisations for current build, "
+                                 "-DSKYWEAVER_NPOL should be equal to the data npol");
+    if(header.nbits != 8)
+        throw std::runtime_error("Input data contains incorrect number of bits per sample, "
+                                 "currently only 8-bit input supported");
+}
+inline void update_config(PipelineConfig& config, ObservationHeader const& header)
+{
+    config.bandwidth((double)header.bandwidth);
+    config.centre_frequency((double)header.frequency);
+}
 
 // ------------------------------------------------------------- block views
 // What a handler receives: the data pointer and the DescribedVector metadata
@@ -696,5 +804,191 @@ struct NullHandler {
     void init(H const&) {}
     void operator()(BlockView const&, std::size_t = 0) {}
 };
+
+// ------------------------------------------------------------ file input
+// MultiFileReader (cpp/skyweaver/src/MultiFileReader.cu:35-303): the data of
+// a sequence of DADA files (each a header of dada_header_size bytes, then
+// TAFTP voltages) as one byte stream.  Positions count data bytes from the
+// start of the first file's data.  (The reference also resets its position
+// to the header size whenever it opens a file; here tellg() is always the
+// data offset, which the reference's own tests do not observe.)
+class MultiFileReader {
+  public:
+    explicit MultiFileReader(PipelineConfig const& config)
+        : _files(config.input_files()), _hsize(config.dada_header_size())
+    {
+        std::vector<char> hdr(_hsize);
+        for(auto const& file : _files) {
+            std::ifstream f(file, std::ios::in | std::ios::binary);
+            if(!f.is_open()) throw std::runtime_error("Could not open file " + file);
+            f.read(hdr.data(), (std::streamsize)_hsize);
+            if((std::size_t)f.gcount() != _hsize) throw std::runtime_error("Reached end of file reading the header of " + file);
+            ObservationHeader h;
+            read_dada_header(hdr.data(), _hsize, h);
+            _headers.push_back(h);
+            f.seekg(0, std::ios::end);
+            const std::size_t size = (std::size_t)f.tellg();
+            _sizes.push_back(size - _hsize);
+            _total += size - _hsize;
+        }
+        if(_files.empty()) throw std::runtime_error("No input files");
+        if(config.check_input_contiguity()) check_contiguity();
+        open(0);
+    }
+    MultiFileReader(MultiFileReader const&) = delete;
+    MultiFileReader& operator=(MultiFileReader const&) = delete;
+
+    ObservationHeader const& get_header() const { return _headers[0]; }
+    ObservationHeader const& get_current_header() const { return _headers[_idx]; }
+    std::size_t get_total_size() const { return _total; }
+    bool can_read(std::size_t bytes) const { return _total - _pos >= bytes; }
+    bool eof() const { return _eof; }
+    bool good() const { return _in.good() && !_eof; }
+    std::size_t tellg() const { return _pos; }
+    bool is_open() const { return _in.is_open(); }
+    void close()
+    {
+        if(_in.is_open()) _in.close();
+        _eof = true;
+    }
+
+    // seekg(pos, beg | cur | end); a target past the data end throws, the
+    // end itself sets eof
+    void seekg(long pos, std::ios_base::seekdir dir = std::ios::beg)
+    {
+        long long target = pos;
+        if(dir == std::ios::cur) target = (long long)_pos + pos;
+        if(dir == std::ios::end) target = (long long)_total + pos;
+        if(target < 0 || (std::size_t)target > _total)
+            throw std::runtime_error("Invalid seek position: " + std::to_string(target));
+        std::size_t cum = 0;
+        for(std::size_t i = 0; i < _files.size(); ++i) {
+            if((std::size_t)target < cum + _sizes[i]) {
+                if(i != _idx || !_in.is_open()) open(i);
+                _in.clear();
+                _in.seekg((std::streamoff)(_hsize + (target - cum)), std::ios::beg);
+                _pos = (std::size_t)target;
+                _eof = false;
+                return;
+            }
+            cum += _sizes[i];
+        }
+        _pos = _total;
+        _eof = true;
+    }
+
+    // Reads up to bytes, continuing into the next files; returns the count
+    std::streamsize read(char* dst, std::streamsize bytes)
+    {
+        std::streamsize total = 0;
+        while(bytes > 0 && !_eof) {
+            if(!_in.is_open()) throw std::runtime_error("File not open: " + _files[_idx]);
+            _in.read(dst + total, bytes);
+            const std::streamsize got = _in.gcount();
+            total += got;
+            bytes -= got;
+            _pos += (std::size_t)got;
+            if(bytes > 0) {  // this file is exhausted
+                if(_idx + 1 < _files.size())
+                    open(_idx + 1);
+                else
+                    _eof = true;
+            }
+        }
+        if(_pos >= _total) _eof = true;
+        return total;
+    }
+
+  private:
+    void open(std::size_t i)
+    {
+        if(_in.is_open()) _in.close();
+        _in.clear();
+        _in.open(_files[i], std::ios::in | std::ios::binary);
+        if(!_in.good()) throw std::runtime_error("Could not open file " + _files[i]);
+        _in.seekg((std::streamoff)_hsize, std::ios::beg);
+        _idx = i;
+    }
+    // each file's OBS_OFFSET must equal the data bytes of the files before it
+    void check_contiguity() const
+    {
+        std::size_t so_far = _sizes[0];
+        for(std::size_t i = 1; i < _sizes.size(); ++i) {
+            if(_headers[i].obs_offset != so_far)
+                throw std::runtime_error("These are not contiguous:" + _files[i - 1] + " and " + _files[i]);
+            so_far += _sizes[i];
+        }
+        if(so_far != _total) throw std::runtime_error("Total size does not match the sum of individual sizes");
+    }
+
+    std::vector<std::string> _files;
+    std::size_t _hsize;
+    std::vector<ObservationHeader> _headers;
+    std::vector<std::size_t> _sizes;
+    std::size_t _total = 0, _pos = 0, _idx = 0;
+    bool _eof = false;
+    std::ifstream _in;
+};
+
+// run_pipeline (cpp/skyweaver/src/skyweaver_cli.cu:109-240): gulps of
+// gulp_length_samps samples from the reader, starting start_time() seconds
+// in (floored to whole heaps), double-buffered: a thread reads the next gulp
+// into pinned host memory while the pipeline processes the current one.  As
+// in the reference the gulp whose read reaches the end of the data is not
+// processed, and processing stops once duration() of data has been done.
+// Returns the number of gulps processed.
+template <typename Pipeline>
+std::size_t run_pipeline(Pipeline& pipeline, PipelineConfig const& config, MultiFileReader& reader,
+                         ObservationHeader const& header)
+{
+    const std::size_t bytes_per_sample = header.nantennas * config.nchans() * config.npol() * 2;
+    const std::size_t input_bytes = bytes_per_sample * config.gulp_length_samps();
+    const double tsamp = (double)(header.obs_nchans / header.obs_bandwidth);
+    std::size_t offset = (std::size_t)(config.start_time() / tsamp);
+    offset = offset / config.nsamples_per_heap() * config.nsamples_per_heap();
+    pipeline.init(header, offset * tsamp);
+    reader.seekg((long)(offset * bytes_per_sample), std::ios::beg);
+
+    struct Pinned {
+        void* p = nullptr;
+        explicit Pinned(std::size_t n) { check(sw_host_alloc(&p, n)); }
+        ~Pinned() { sw_host_free(p); }
+        char* data() const { return static_cast<char*>(p); }
+    };
+    Pinned a(input_bytes), b(input_bytes);
+    Pinned *fill = &a, *ready = &b;
+    struct View {  // the contiguous-container view operator() takes
+        char const* p;
+        std::size_t n;
+        char const* data() const { return p; }
+        std::size_t size() const { return n; }
+    };
+    reader.read(fill->data(), (std::streamsize)input_bytes);
+    std::size_t gulps = 0;
+    double data_time = 0.0;
+    while(!reader.eof()) {
+        std::swap(fill, ready);
+        std::string read_error;
+        std::thread next([&] {
+            try {
+                reader.read(fill->data(), (std::streamsize)input_bytes);
+            } catch(std::exception const& e) {
+                read_error = e.what();
+            }
+        });
+        try {
+            pipeline(View{ready->data(), input_bytes});
+        } catch(...) {
+            next.join();
+            throw;
+        }
+        next.join();
+        if(!read_error.empty()) throw std::runtime_error("Error in input file read: " + read_error);
+        ++gulps;
+        data_time += config.gulp_length_samps() * tsamp;
+        if(data_time >= config.duration()) break;
+    }
+    return gulps;
+}
 
 } // namespace skyweaver_amd

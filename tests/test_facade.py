@@ -21,3 +21,12 @@ def test_facade_host(tmp_path):
     r = subprocess.run([_bin("facade_host_test"), str(tmp_path)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     assert "passed" in r.stdout
+
+
+def test_dada_reader(tmp_path):
+    """read_dada_header on the reference fixture's headers and MultiFileReader
+    over ten files built from them (cpp/skyweaver/test/src/MultiFileReaderTester.cu)."""
+    r = subprocess.run([_bin("dada_reader_test"), os.path.join(HERE, "golden", "dada"), str(tmp_path)],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert "passed" in r.stdout

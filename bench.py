@@ -6,8 +6,8 @@ Workload (one gulp = one step): 64 antennas x 64 channels (209 kHz around
 16, TAFTP order, resident in HBM), 800 random-phase int8 beams, one coherent
 DM of 50 pc/cm^3 (FFT length 16384, overlap 10 = intra-channel smear),
 tscrunch 16, Stokes I, with the incoherent beam and per-channel statistics.
-Stages per gulp: transpose, statistics, coherent dedispersion, incoherent
-beam, coherent beam.
+Stages per gulp: transpose fused with the statistics' power sums, coherent
+dedispersion, incoherent beam, coherent beam.
 
 Two forms run on the same GPU and data:
   chosen: int8 MFMA coherent beamformer, DM-batched dedispersion (the value)
@@ -257,11 +257,14 @@ def main():
                 "traffic": CB_TRAFFIC_BYTES, "ops_per_launch": ops, "avg_launch_ms": round(cb_ms, 4),
                 "note": "int8 operations (1 MAC = 2 ops); peak is the dense int8 MFMA rate"}
     hbm = {}
-    for k, nbytes in (("transpose", 2 * T * F * 2 * A * 2), ("statistics", T * F * 2 * A * 2),
-                      ("incoherent_beam", T * F * 2 * A * 2)):
+    # the transpose stage is taftp_to_fpat_stats_kernel: the statistics' power
+    # sums are taken in the same pass (one read of the gulp, one write), and
+    # only their finalisation is left in the statistics stage
+    for k, nbytes in (("transpose", 2 * T * F * 2 * A * 2), ("incoherent_beam", T * F * 2 * A * 2)):
         if stage[k] > 0:
             gbs = nbytes / (stage[k] * 1e-3) / 1e9
             hbm[k] = {"GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 3)}
+    hbm["transpose"]["kernel"] = "taftp_to_fpat_stats_kernel (transpose + statistics power sums)"
 
     target = target_tuned = None
     if not args.no_8dm and not args.no_plain:

@@ -323,6 +323,17 @@ int sw_corner_turn(const void* d_src, void* d_dst, int nchans, int rows, int col
 /* Statistics of an FPAT gulp; same results as sw_statistics on FTPA. */
 int sw_statistics_fpat(const void* d_fpat, float* d_stats, int nchans, int nsamples, int npol,
                        int nantennas, void* d_workspace, void* stream);
+/* sw_transpose_fpat and sw_statistics_fpat in one pass over the input: the
+ * gulp is read once, transposed to FPAT and its exact power sums taken on
+ * the way (results identical to the two calls).  Fuses
+ * Transposer::transpose (cpp/skyweaver/Transposer.cuh:66-69) with
+ * StatisticsCalculator::calculate_statistics
+ * (cpp/skyweaver/StatisticsCalculator.cuh:65), which the reference runs as
+ * two passes (cpp/skyweaver/detail/BeamformerPipeline.cu:157-167).
+ * d_workspace: NULL (library scratch) or at least
+ * max(8, ceil(nheap_groups/16)) * nchans * 2 * nant_out * 32 bytes. */
+int sw_transpose_fpat_stats(const void* d_taftp, void* d_fpat, float* d_stats, int nheap_groups,
+                            int nant_in, int nant_out, int nchans, void* d_workspace, void* stream);
 
 /* ---------------------------------------------------- BeamformerPipeline
  * BeamformerPipeline<CBH, IBH, StatsH, Traits>: ctor(PipelineConfig const&,

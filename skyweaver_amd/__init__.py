@@ -94,6 +94,7 @@ def _declare(L):
     L.sw_transpose_fpat.argtypes = [_vp, _vp, _c, _c, _c, _c, _vp]
     L.sw_corner_turn.argtypes = [_vp, _vp, _c, _c, _c, _vp]
     L.sw_statistics_fpat.argtypes = [_vp, _vp, _c, _c, _c, _c, _vp, _vp]
+    L.sw_transpose_fpat_stats.argtypes = [_vp, _vp, _vp, _c, _c, _c, _c, _vp, _vp]
     _pvp = ctypes.POINTER(ctypes.c_void_p)
     L.sw_codedisp_create.argtypes = [ctypes.POINTER(CodedispConfig), _vp, _pvp]
     L.sw_codedisp_destroy.argtypes = [_vp]
@@ -295,6 +296,12 @@ def microbench(which: str) -> float:
 
 def transpose_fpat(taftp, fpat, nheaps, nant_in, nant_out, nchans, stream=None):
     check(lib().sw_transpose_fpat(_ptr(taftp), _ptr(fpat), nheaps, nant_in, nant_out, nchans, _stream(stream)))
+
+
+def transpose_fpat_stats(taftp, fpat, stats, nheaps, nant_in, nant_out, nchans, workspace=None, stream=None):
+    """TAFTP -> FPAT fused with the statistics (one pass over the gulp)."""
+    check(lib().sw_transpose_fpat_stats(_ptr(taftp), _ptr(fpat), _ptr(stats), nheaps, nant_in, nant_out, nchans,
+                                        _ptr(workspace), _stream(stream)))
 
 
 def corner_turn(src, dst, nchans, rows, cols, stream=None):

@@ -157,6 +157,111 @@ fpat_stats_partial(const uint32_t* __restrict__ tm, long long* __restrict__ part
     if(threadIdx.x < 4) part[((size_t)chunk * ncols + col) * 4 + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// TAFTP -> FPAT as taftp_to_fpat_kernel, fused with the statistics' exact
+// power sums, so that the gulp is read from HBM once for both.  One wave per
+// (chunk of kFusedHeaps heaps, antenna, channel): per heap it loads the
+// channel's 256 samples x 2 pols (16 bytes a lane), writes the two pols'
+// 512-byte runs and adds the sums of x, x^2, x^3, x^4 over re and im with
+// dot-product instructions: sdot4 for x and x^2; for x^3 and x^4 the values
+// are sign-extended to 16-bit pairs (v_perm), squared with a packed 16-bit
+// multiply (x^2 <= 2^14 fits) and summed by sdot2 / udot2.  Bounds per lane
+// and pol over one chunk: |sum x^3| <= 16 heaps x 8 values x 2^21 = 2^28 and
+// a heap's x^4 sum <= 8 x 2^28 = 2^31 (unsigned), so the 32-bit partials are
+// exact; the wave reduction and the x^4 total run in 64 bits.  Partials go
+// to part[chunk][col][4] (col = (f*2 + p)*A + a) for fpat_stats_finalize.
+constexpr int kFusedHeaps = 16;
+
+typedef short sw_short2 __attribute__((ext_vector_type(2)));
+typedef unsigned short sw_ushort2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void power_sums(uint32_t x, int& s1, int& s2, int& s3, uint32_t& s4)
+{
+    s1 = __builtin_amdgcn_sdot4((int)x, 0x01010101, s1, false);
+    s2 = __builtin_amdgcn_sdot4((int)x, (int)x, s2, false);
+    // [x0, x2] and [x1, x3] as sign-extended 16-bit pairs (selector 8 / 9
+    // replicate the sign bits of bytes 1 / 3)
+    const sw_short2 e0 = __builtin_bit_cast(sw_short2, __builtin_amdgcn_perm(x << 8, x << 8, 0x09030801u));
+    const sw_short2 e1 = __builtin_bit_cast(sw_short2, __builtin_amdgcn_perm(x, x, 0x09030801u));
+    const sw_short2 q0 = e0 * e0, q1 = e1 * e1;
+    s3 = __builtin_amdgcn_sdot2(q0, e0, s3, false);
+    s3 = __builtin_amdgcn_sdot2(q1, e1, s3, false);
+    s4 = __builtin_amdgcn_udot2(__builtin_bit_cast(sw_ushort2, q0), __builtin_bit_cast(sw_ushort2, q0), s4, false);
+    s4 = __builtin_amdgcn_udot2(__builtin_bit_cast(sw_ushort2, q1), __builtin_bit_cast(sw_ushort2, q1), s4, false);
+}
+
+__device__ __forceinline__ long long wave_sum(long long v)
+{
+#pragma unroll
+    for(int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(256)
+taftp_to_fpat_stats_kernel(const uint4* __restrict__ in, uint2* __restrict__ out, long long* __restrict__ part,
+                           int nant_in, int nant_out, int nchans, int nheaps, int nchunks)
+{
+    const int lane    = threadIdx.x & 63;
+    const size_t wave = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if(wave >= (size_t)nchunks * nant_out * nchans) return;  // whole waves only
+    const int f     = (int)(wave % nchans);
+    const size_t r  = wave / nchans;
+    const int a     = (int)(r % nant_out);
+    const int chunk = (int)(r / nant_out);
+    const int h0 = chunk * kFusedHeaps, h1 = min(nheaps, h0 + kFusedHeaps);
+    const size_t col_words = (size_t)nheaps * 64;  // one column: T samples = T/4 uint2
+    uint2* o0 = out + (((size_t)f * 2 + 0) * nant_out + a) * col_words + lane;
+    uint2* o1 = out + (((size_t)f * 2 + 1) * nant_out + a) * col_words + lane;
+    const uint4* src = in + ((size_t)a * nchans + f) * 64 + lane;
+    const size_t heap_stride = (size_t)nant_in * nchans * 64;
+    int s1[2] = {0, 0}, s2[2] = {0, 0}, s3[2] = {0, 0};
+    unsigned long long s4[2] = {0, 0};
+    auto heap = [&](int h, const uint4& v) {
+        // v.x = [p0re p0im p1re p1im] of sample 4*lane, etc.
+        const uint32_t p0lo = __builtin_amdgcn_perm(v.y, v.x, 0x05040100u);
+        const uint32_t p0hi = __builtin_amdgcn_perm(v.w, v.z, 0x05040100u);
+        const uint32_t p1lo = __builtin_amdgcn_perm(v.y, v.x, 0x07060302u);
+        const uint32_t p1hi = __builtin_amdgcn_perm(v.w, v.z, 0x07060302u);
+        o0[(size_t)h * 64] = make_uint2(p0lo, p0hi);
+        o1[(size_t)h * 64] = make_uint2(p1lo, p1hi);
+        uint32_t q0 = 0, q1 = 0;
+        power_sums(p0lo, s1[0], s2[0], s3[0], q0);
+        power_sums(p0hi, s1[0], s2[0], s3[0], q0);
+        power_sums(p1lo, s1[1], s2[1], s3[1], q1);
+        power_sums(p1hi, s1[1], s2[1], s3[1], q1);
+        s4[0] += q0;
+        s4[1] += q1;
+    };
+    if(a < nant_in) {  // wave-uniform
+        int h = h0;
+        for(; h + 4 <= h1; h += 4) {  // four loads in flight
+            uint4 v[4];
+#pragma unroll
+            for(int k = 0; k < 4; ++k) v[k] = src[(size_t)(h + k) * heap_stride];
+#pragma unroll
+            for(int k = 0; k < 4; ++k) heap(h + k, v[k]);
+        }
+        for(; h < h1; ++h) heap(h, src[(size_t)h * heap_stride]);
+    } else {  // padded antenna: zeros (and zero sums)
+        for(int h = h0; h < h1; ++h) {
+            o0[(size_t)h * 64] = make_uint2(0, 0);
+            o1[(size_t)h * 64] = make_uint2(0, 0);
+        }
+    }
+    const size_t ncols = (size_t)nchans * 2 * nant_out;
+#pragma unroll
+    for(int p = 0; p < 2; ++p) {
+        const long long t1 = wave_sum(s1[p]), t2 = wave_sum(s2[p]), t3 = wave_sum(s3[p]);
+        const long long t4 = wave_sum((long long)s4[p]);
+        if(lane == 0) {
+            long long* d = part + ((size_t)chunk * ncols + ((size_t)f * 2 + p) * nant_out + a) * 4;
+            d[0] = t1;
+            d[1] = t2;
+            d[2] = t3;
+            d[3] = t4;
+        }
+    }
+}
+
 __global__ void fpat_stats_finalize(const long long* __restrict__ part, float4* __restrict__ stats,
                                     int nchunks, int ncols, long long nvalues)
 {
@@ -212,9 +317,10 @@ int corner_turn_f64(const void* src, void* dst, int nchans, int R, int C, hipStr
     return SW_OK;
 }
 
-size_t fpat_stats_workspace(int nchans, int npol, int nant)
+size_t fpat_stats_workspace(int nchans, int npol, int nant, int nsamples)
 {
-    return (size_t)kStatChunks * nchans * npol * nant * 4 * sizeof(long long);
+    const int fused_chunks = (nsamples / 256 + kFusedHeaps - 1) / kFusedHeaps;
+    return (size_t)std::max(kStatChunks, fused_chunks) * nchans * npol * nant * 4 * sizeof(long long);
 }
 
 } // namespace sw
@@ -237,6 +343,34 @@ extern "C" int sw_transpose_fpat(const void* d_taftp, void* d_fpat, int nheap_gr
     return SW_OK;
 }
 
+extern "C" int sw_transpose_fpat_stats(const void* d_taftp, void* d_fpat, float* d_stats, int nheap_groups,
+                                      int nant_in, int nant_out, int nchans, void* d_workspace, void* stream)
+{
+    SW_CHECK_ARG(d_taftp && d_fpat && d_stats, "null buffer");
+    SW_CHECK_ARG(nheap_groups > 0 && nchans > 0, "empty input");
+    SW_CHECK_ARG(nant_in > 0 && nant_in <= nant_out,
+                 "Input number of antennas must be <= to the maximum nantennas");
+    SW_CHECK_ARG((reinterpret_cast<uintptr_t>(d_taftp) & 15) == 0 &&
+                     (reinterpret_cast<uintptr_t>(d_fpat) & 7) == 0,
+                 "input must be 16-byte and output 8-byte aligned");
+    const int nsamples = nheap_groups * 256;
+    SW_CHECK_ARG((size_t)nsamples * 2 < (1ull << 22), "too many samples per call for exact sums");
+    void* ws = d_workspace ? d_workspace : sw::scratch(sw::fpat_stats_workspace(nchans, 2, nant_out, nsamples), 2);
+    if(!ws) return SW_ERR_ALLOC;
+    const int nchunks = (nheap_groups + sw::kFusedHeaps - 1) / sw::kFusedHeaps;
+    const size_t nwaves = (size_t)nchunks * nant_out * nchans;
+    hipLaunchKernelGGL(sw::taftp_to_fpat_stats_kernel, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0,
+                       sw::as_stream(stream), static_cast<const uint4*>(d_taftp), static_cast<uint2*>(d_fpat),
+                       static_cast<long long*>(ws), nant_in, nant_out, nchans, nheap_groups, nchunks);
+    SW_LAUNCH_CHECK();
+    const int ncols = nchans * 2 * nant_out;
+    hipLaunchKernelGGL(sw::fpat_stats_finalize, dim3((ncols + 255) / 256), dim3(256), 0, sw::as_stream(stream),
+                       static_cast<const long long*>(ws), reinterpret_cast<float4*>(d_stats), nchunks, ncols,
+                       (long long)nsamples * 2);
+    SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
 extern "C" int sw_corner_turn(const void* d_src, void* d_dst, int nchans, int rows, int cols,
                               void* stream)
 {
@@ -252,7 +386,7 @@ extern "C" int sw_statistics_fpat(const void* d_fpat, float* d_stats, int nchans
     SW_CHECK_ARG(nchans > 0 && nsamples > 1 && nsamples % 2 == 0 && nantennas > 0, "bad dimensions");
     SW_CHECK_ARG((size_t)nsamples * 2 < (1ull << 22), "too many samples per call for exact sums");
     const int ncols = nchans * npol * nantennas;
-    void* ws = d_workspace ? d_workspace : sw::scratch(sw::fpat_stats_workspace(nchans, npol, nantennas), 2);
+    void* ws = d_workspace ? d_workspace : sw::scratch(sw::fpat_stats_workspace(nchans, npol, nantennas, nsamples), 2);
     if(!ws) return SW_ERR_ALLOC;
     const int chunk = (nsamples / sw::kStatChunks + 1) & ~1;
     hipLaunchKernelGGL(sw::fpat_stats_partial, dim3(ncols, sw::kStatChunks), dim3(256), 0,

@@ -27,7 +27,7 @@
 
 namespace sw {
 int cb_mfma_supported(const sw_bf_dims& d);
-size_t fpat_stats_workspace(int nchans, int npol, int nant);
+size_t fpat_stats_workspace(int nchans, int npol, int nant, int nsamples);
 int ib_beamform_tm(const void* d_ring, long ld, long w0, float* d_raw, void* d_out, const float* d_scale,
                    const float* d_offset, const float* d_ant_weights, const sw_bf_dims& d, hipStream_t st);
 int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_packed, const float* d_scale,
@@ -232,7 +232,7 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
     p->ftpa_dd.resize(c.ndms, nullptr);
     for(auto& b : p->ftpa_dd) SW_HIP_TRY(hipMalloc(&b, fb));
     SW_HIP_TRY(hipMalloc(&p->stats, sizeof(float) * 4 * c.nchans * 2 * c.nantennas));
-    SW_HIP_TRY(hipMalloc(&p->stats_ws, sw::fpat_stats_workspace(c.nchans, 2, c.nantennas)));
+    SW_HIP_TRY(hipMalloc(&p->stats_ws, sw::fpat_stats_workspace(c.nchans, 2, c.nantennas, c.nsamples)));
     p->h_stats.resize((size_t)4 * c.nchans * 2 * c.nantennas);
     const int nv = p->nv();
     SW_HIP_TRY(hipMalloc(&p->cb_out, p->tout() * c.nchans * c.nbeams * nv));
@@ -447,11 +447,12 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
     // the gulp is held time-major (FPAT) up to the dedisperser; the two FPAT
     // buffers alternate so that the previous gulp stays readable as history
     std::swap(p->ftpa, p->ftpa_prev);
-    rc = sw_transpose_fpat(in, p->ftpa, c.nsamples / 256, nant_in, A, F, st);
+    // one pass over the gulp: transpose fused with the statistics' power
+    // sums (the "transpose" stage time includes them; the "statistics"
+    // stage is left with nothing to time)
+    rc = sw_transpose_fpat_stats(in, p->ftpa, p->stats, c.nsamples / 256, nant_in, A, F, p->stats_ws, st);
     if(rc) return rc;
     SW_HIP_TRY(hipEventRecord(p->ev[1], st));
-    rc = sw_statistics_fpat(p->ftpa, p->stats, F, c.nsamples, 2, A, p->stats_ws, st);
-    if(rc) return rc;
     SW_HIP_TRY(hipEventRecord(p->ev[2], st));
     if(p->call_count == 0) {
         // update_scalings on the first gulp (BeamformerPipeline.cu:171-176)

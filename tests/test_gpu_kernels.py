@@ -75,6 +75,39 @@ def test_statistics_parity(sw, orc, dev, F, T, A):
     np.testing.assert_allclose(got, exp, rtol=1e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("H,A_in,A_out,F", [(16, 16, 64, 64), (37, 57, 64, 5), (3, 80, 80, 8), (33, 4, 16, 16)])
+def test_transpose_fpat_stats_fused(sw, orc, dev, H, A_in, A_out, F):
+    """The fused TAFTP -> FPAT transpose + statistics equals the separate
+    transpose and statistics (FPAT bytes and float statistics bit-identical)
+    and the oracle; full-scale values (-128 everywhere in one column, +127
+    in another) reach the kernel's 32-bit partial-sum bounds."""
+    print(f"transpose_fpat_stats H={H} A_in={A_in} A_out={A_out} F={F}", flush=True)
+    taftp = _rand_c8((H, A_in, F, 256, 2), 99 + H, sigma=30.0)
+    taftp[:, 0, 0] = -128
+    taftp[:, A_in - 1, F - 1, :, 1] = 127
+    T = H * 256
+    d_in = _to_dev(taftp, dev)
+    d_fpat = _empty((F, 2, A_out, T, 2), np.int8, dev)
+    d_st = _empty((F, 2, A_out, 4), np.float32, dev)
+    sw.transpose_fpat_stats(d_in, d_fpat, d_st, H, A_in, A_out, F)
+    d_fpat2 = _empty((F, 2, A_out, T, 2), np.int8, dev)
+    d_st2 = _empty((F, 2, A_out, 4), np.float32, dev)
+    sw.transpose_fpat(d_in, d_fpat2, H, A_in, A_out, F)
+    sw.statistics_fpat(d_fpat2, d_st2, F, T, A_out)
+    fpat = _to_host(d_fpat)
+    ftpa = orc.transpose(taftp, A_out)  # [F][T][P][A][2]
+    np.testing.assert_array_equal(fpat, ftpa.transpose(0, 2, 3, 1, 4))
+    np.testing.assert_array_equal(fpat, _to_host(d_fpat2))
+    st = _to_host(d_st)
+    np.testing.assert_array_equal(st.view(np.uint32), _to_host(d_st2).view(np.uint32))
+    live = np.zeros((F, 2, A_out), bool)
+    live[:, :, :A_in] = True
+    live[0, :, 0] = False  # constant column: std 0, skew/kurtosis undefined
+    exp = orc.statistics(ftpa)
+    np.testing.assert_allclose(st[live].astype(np.float64), exp[live], rtol=1e-5, atol=1e-6)
+    np.testing.assert_array_equal(st[0, :, 0, :2], np.array([[-128.0, 0.0]] * 2, np.float32))
+
+
 # --------------------------------------------------------- incoherent beamformer
 def _ib_scales(F, S, A, ts, level=32.0):
     ib_scale = level ** 2

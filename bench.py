@@ -287,7 +287,8 @@ def main():
     if not args.no_cpu:
         import oracle
         nthreads = min(16, os.cpu_count() or 1)
-        nchan = args.cpu_channels or max(1, min(F, nthreads))
+        # three channels per thread: about 13 s of CPU work on 16 threads
+        nchan = args.cpu_channels or max(1, min(F, 3 * nthreads))
         log(f"CPU oracle sample: {nchan} channels on {nthreads} threads")
         secs = cpu_baseline(sw, oracle, taftp, weights, nchan, nthreads)
         cpu_value = B * nchan * T * len(DMS) / secs

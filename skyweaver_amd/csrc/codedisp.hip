@@ -14,16 +14,23 @@
 // the dedispersed input sample j - O/2.
 //
 // Data are time-major ([F][P*A][T] complex int8, one column per (pol,
-// antenna)) so that every transform reads and writes contiguous runs:
-//  * N <= 16384: one workgroup per (channel, segment, group of C columns)
-//    keeps C x N points in LDS: load -> forward FFT -> per DM of the launch:
-//    chirp, inverse FFT, quantise, store.  With two LDS buffers (C*N <= 8192)
-//    the forward FFT is shared by every DM of the launch; with one (N = 16384)
-//    it is recomputed per DM.
-//  * N > 16384 = RP x L: a radix-RP decimation-in-frequency stage, the LDS
-//    kernel on each of the RP sub-sequences (chirp permuted to match) and a
-//    radix-RP decimation-in-time stage that quantises.
-// Outputs are corner-turned to FTPA for the beamformers.
+// antenna)) so that every transform reads contiguous runs:
+//  * N = 4096, 8192, 16384 (codedisp_reg_kernel<N/512>): a column is held in
+//    the registers of 512 threads (32 points each) and transformed as
+//    DFT-R1 x DFT-32 x DFT-16 with two LDS exchanges per direction; the
+//    spectrum stays in registers in a permuted order that the chirp table
+//    shares.  Persistent workgroups walk (channel, segment, column group).
+//  * N = 32768 (codedisp_reg2_kernel): two 16384-point halves of a radix-2
+//    decimation in frequency, fused in one persistent kernel.
+//  * other lengths: one workgroup per (channel, segment, group of C columns)
+//    keeps C x N points in LDS (Stockham passes); with two LDS buffers the
+//    forward FFT is shared by every DM of the launch.  N > 16384 there is a
+//    radix-RP decimation-in-frequency stage, the LDS kernel on each of the
+//    RP sub-sequences and a radix-RP decimation-in-time stage.
+// Segments lie on one grid over the whole stream and outputs go to a
+// per-DM time-major ring (stream sample x at ring index (x + latency) mod
+// ylen), from which the beamformers read each gulp's window in place; a
+// corner turn to FTPA is made only when the caller asks for an FTPA output.
 #include "fft.hpp"
 
 #include <algorithm>

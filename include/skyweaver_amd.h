@@ -474,6 +474,22 @@ int sw_pipeline_init(sw_pipeline* pipeline, const sw_obs_header* header);
  * taftp is a device pointer (no host transfer).  Returns 0 (the reference's
  * operator() always returns false). */
 int sw_pipeline_process(sw_pipeline* pipeline, const void* taftp, size_t bytes, int on_device);
+/* The same split in two so that the host-to-device copy of the next gulp
+ * overlaps the processing of the current one (the reference copies inside
+ * operator(), cpp/skyweaver/detail/BeamformerPipeline.cu:281-288, and
+ * overlaps only the file read, cpp/skyweaver/src/skyweaver_cli.cu:196-220).
+ * sw_pipeline_prefetch starts an asynchronous copy of a host gulp into one
+ * of two device gulp buffers (allocated on the first call) on the
+ * pipeline's copy stream and returns; at most two gulps may be pending.
+ * The host buffer must stay unchanged until the sw_pipeline_process_prefetched
+ * call that processes it returns (pinned memory, sw_host_alloc, makes the
+ * copy asynchronous).  sw_pipeline_process_prefetched processes the oldest
+ * pending gulp exactly as sw_pipeline_process does. */
+int sw_pipeline_prefetch(sw_pipeline* pipeline, const void* h_taftp, size_t bytes);
+int sw_pipeline_process_prefetched(sw_pipeline* pipeline);
+/* Wait until every copy and kernel the pipeline has enqueued has finished
+ * (for a caller about to free host buffers it prefetched from). */
+int sw_pipeline_synchronize(sw_pipeline* pipeline);
 /* Per-stage device times of the last gulp (ms), in the order transpose,
  * statistics, dedispersion, incoherent beam, coherent beam, incoherent
  * dedispersion (with its aggregation), weights (delay-model path), other;

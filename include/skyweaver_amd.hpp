@@ -584,6 +584,23 @@ class BeamformerPipeline {
         ++_calls;
         return false;
     }
+    // operator() in two halves: prefetch starts the host-to-device copy of a
+    // gulp (at most two pending; the host buffer must stay unchanged until
+    // the process_prefetched call that consumes it returns), and
+    // process_prefetched processes the oldest pending gulp.
+    template <typename Container>
+    void prefetch(Container const& taftp_on_host)
+    {
+        check(sw_pipeline_prefetch(_p, taftp_on_host.data(), taftp_on_host.size() * sizeof(*taftp_on_host.data())));
+    }
+    bool process_prefetched()
+    {
+        check(sw_pipeline_process_prefetched(_p));
+        ++_calls;
+        return false;
+    }
+    // wait for every copy and kernel the pipeline has enqueued
+    void synchronize() { check(sw_pipeline_synchronize(_p)); }
     std::vector<float> stage_times() const
     {
         std::vector<float> v(SW_NSTAGES);
@@ -932,11 +949,12 @@ class MultiFileReader {
 
 // run_pipeline (cpp/skyweaver/src/skyweaver_cli.cu:109-240): gulps of
 // gulp_length_samps samples from the reader, starting start_time() seconds
-// in (floored to whole heaps), double-buffered: a thread reads the next gulp
-// into pinned host memory while the pipeline processes the current one.  As
-// in the reference the gulp whose read reaches the end of the data is not
-// processed, and processing stops once duration() of data has been done.
-// Returns the number of gulps processed.
+// in (floored to whole heaps).  Three stages overlap: a thread reads gulp
+// k + 2 from the files into pinned host memory while gulp k + 1 is copied to
+// the device (prefetch) and gulp k is processed; three pinned buffers
+// rotate.  As in the reference the gulp whose read reaches the end of the
+// data is not processed, and processing stops once duration() of data has
+// been done.  Returns the number of gulps processed.
 template <typename Pipeline>
 std::size_t run_pipeline(Pipeline& pipeline, PipelineConfig const& config, MultiFileReader& reader,
                          ObservationHeader const& header)
@@ -951,42 +969,65 @@ std::size_t run_pipeline(Pipeline& pipeline, PipelineConfig const& config, Multi
 
     struct Pinned {
         void* p = nullptr;
-        explicit Pinned(std::size_t n) { check(sw_host_alloc(&p, n)); }
+        std::size_t n;
+        explicit Pinned(std::size_t bytes) : n(bytes) { check(sw_host_alloc(&p, bytes)); }
         ~Pinned() { sw_host_free(p); }
         char* data() const { return static_cast<char*>(p); }
-    };
-    Pinned a(input_bytes), b(input_bytes);
-    Pinned *fill = &a, *ready = &b;
-    struct View {  // the contiguous-container view operator() takes
-        char const* p;
-        std::size_t n;
-        char const* data() const { return p; }
         std::size_t size() const { return n; }
     };
-    reader.read(fill->data(), (std::streamsize)input_bytes);
+    Pinned b0(input_bytes), b1(input_bytes), b2(input_bytes);
+    Pinned* buf[3] = {&b0, &b1, &b2};
+    struct Drain {  // no copy from the buffers may be in flight when they are freed
+        Pipeline& p;
+        ~Drain()
+        {
+            try {
+                p.synchronize();
+            } catch(...) {
+            }
+        }
+    } drain{pipeline};
+    std::string read_error;
+    auto read_into = [&](Pinned* b) {
+        try {
+            reader.read(b->data(), (std::streamsize)input_bytes);
+        } catch(std::exception const& e) {
+            read_error = e.what();
+        }
+    };
+    auto check_read = [&] {
+        if(!read_error.empty()) throw std::runtime_error("Error in input file read: " + read_error);
+    };
+    // gulp k is processed iff its read did not reach the end of the data
+    read_into(buf[0]);
+    check_read();
+    if(reader.eof()) return 0;
+    pipeline.prefetch(*buf[0]);
+    read_into(buf[1]);
+    check_read();
+    bool next_ok = !reader.eof();
     std::size_t gulps = 0;
     double data_time = 0.0;
-    while(!reader.eof()) {
-        std::swap(fill, ready);
-        std::string read_error;
-        std::thread next([&] {
-            try {
-                reader.read(fill->data(), (std::streamsize)input_bytes);
-            } catch(std::exception const& e) {
-                read_error = e.what();
-            }
-        });
+    for(std::size_t k = 0;; ++k) {
+        if(next_ok) pipeline.prefetch(*buf[(k + 1) % 3]);
+        std::thread next;
+        // buffer (k + 2) % 3 held gulp k - 1, whose copy ended before it was processed
+        if(next_ok) next = std::thread(read_into, buf[(k + 2) % 3]);
         try {
-            pipeline(View{ready->data(), input_bytes});
+            pipeline.process_prefetched();
         } catch(...) {
-            next.join();
+            if(next.joinable()) next.join();
             throw;
         }
-        next.join();
-        if(!read_error.empty()) throw std::runtime_error("Error in input file read: " + read_error);
         ++gulps;
+        const bool more = next_ok;
+        if(next.joinable()) {
+            next.join();
+            check_read();
+            next_ok = !reader.eof();
+        }
         data_time += config.gulp_length_samps() * tsamp;
-        if(data_time >= config.duration()) break;
+        if(data_time >= config.duration() || !more) break;
     }
     return gulps;
 }

@@ -222,6 +222,30 @@ class DeviceBuffer:
             pass
 
 
+class HostBuffer:
+    """Pinned host allocation (sw_host_alloc) with a numpy view."""
+
+    def __init__(self, nbytes: int):
+        import numpy as np
+        p = ctypes.c_void_p()
+        check(lib().sw_host_alloc(ctypes.byref(p), int(nbytes)))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+        self.array = np.ctypeslib.as_array((ctypes.c_uint8 * self.nbytes).from_address(self.ptr))
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            lib().sw_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Stream:
     def __init__(self):
         p = ctypes.c_void_p()
@@ -609,6 +633,9 @@ def _declare_pipeline(L):
     L.sw_pipeline_set_beamsets.argtypes = [_vp, _c, _vp, _vp]
     L.sw_pipeline_init.argtypes = [_vp, ctypes.POINTER(ObsHeaderC)]
     L.sw_pipeline_process.argtypes = [_vp, _vp, _sz, _c]
+    L.sw_pipeline_prefetch.argtypes = [_vp, _vp, _sz]
+    L.sw_pipeline_process_prefetched.argtypes = [_vp]
+    L.sw_pipeline_synchronize.argtypes = [_vp]
     L.sw_pipeline_stage_times.argtypes = [_vp, ctypes.POINTER(ctypes.c_float), _c]
     L.sw_pipeline_set_delay_file.argtypes = [_vp, ctypes.c_char_p]
     L.sw_pipeline_set_tdb_sink.argtypes = [_vp, SINK_FN, _vp]
@@ -710,6 +737,22 @@ class BeamformerPipeline:
             a = np.ascontiguousarray(taftp)
             check(lib().sw_pipeline_process(self.handle, a.ctypes.data, a.nbytes, 0))
         return False
+
+    def prefetch(self, taftp):
+        """Start the host-to-device copy of a host gulp (a HostBuffer, or a numpy
+        array kept alive and unchanged by the caller until it is processed)."""
+        if isinstance(taftp, HostBuffer):
+            check(lib().sw_pipeline_prefetch(self.handle, taftp.ptr, taftp.nbytes))
+        else:
+            check(lib().sw_pipeline_prefetch(self.handle, taftp.ctypes.data, taftp.nbytes))
+
+    def process_prefetched(self):
+        """Process the oldest prefetched gulp."""
+        check(lib().sw_pipeline_process_prefetched(self.handle))
+        return False
+
+    def synchronize(self):
+        check(lib().sw_pipeline_synchronize(self.handle))
 
     def stage_times(self):
         buf = (ctypes.c_float * len(STAGES))()

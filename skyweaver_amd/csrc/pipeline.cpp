@@ -84,6 +84,13 @@ struct sw_pipeline {
     hipEvent_t ev_w[2]{};            // weights generation
     std::vector<hipEvent_t> ev_dm;  // per DM: ib start, cb start, cb end, id end
     float stage_ms[SW_NSTAGES]{};
+    // prefetched input (sw_pipeline_prefetch): two device gulp buffers filled
+    // on a copy stream while the pipeline stream works on the previous gulp
+    hipStream_t copy_stream = nullptr;
+    void* pre_d[2]{};
+    hipEvent_t pre_copied[2]{}, pre_consumed[2]{};
+    bool pre_used[2]{};    // pre_consumed recorded at least once
+    int pre_head = 0, pre_count = 0;
 
     size_t ftpa_bytes() const { return (size_t)cfg.nchans * cfg.nsamples * 2 * cfg.nantennas * 2; }
     int nv() const { return cfg.stokes == SW_STOKES_IQUV ? 4 : 1; }
@@ -102,6 +109,13 @@ struct sw_pipeline {
         for(auto e : ev) if(e) (void)hipEventDestroy(e);
         for(auto e : ev_w) if(e) (void)hipEventDestroy(e);
         for(auto e : ev_dm) if(e) (void)hipEventDestroy(e);
+        if(copy_stream) (void)hipStreamSynchronize(copy_stream);
+        for(int i = 0; i < 2; ++i) {
+            if(pre_d[i]) (void)hipFree(pre_d[i]);
+            if(pre_copied[i]) (void)hipEventDestroy(pre_copied[i]);
+            if(pre_consumed[i]) (void)hipEventDestroy(pre_consumed[i]);
+        }
+        if(copy_stream) (void)hipStreamDestroy(copy_stream);
         if(stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -395,25 +409,94 @@ extern "C" int sw_pipeline_stage_times(const sw_pipeline* p, float* ms, int n)
     return SW_OK;
 }
 
-extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t bytes, int on_device)
+namespace {
+
+int check_gulp(const sw_pipeline* p, size_t bytes)
 {
-    SW_CHECK_ARG(p && taftp, "null argument");
     SW_CHECK_ARG(p->inited, "pipeline not initialised (call sw_pipeline_init)");
     SW_CHECK_ARG(p->weights_set, "no beamforming weights set");
-    const auto& c      = p->cfg;
-    const int nant_in  = p->header.nantennas;
-    const size_t expect = (size_t)c.nsamples * nant_in * c.nchans * 2 * 2;
+    const auto& c       = p->cfg;
+    const size_t expect = (size_t)c.nsamples * p->header.nantennas * c.nchans * 2 * 2;
     if(bytes != expect) {
         sw::set_error("Unexpected buffer size, expected " + std::to_string(expect) + " but got " +
                       std::to_string(bytes));
         return SW_ERR_INVALID;
     }
-    hipStream_t st = p->stream;
+    return SW_OK;
+}
+
+// One gulp from device memory `in` on the pipeline stream; `consumed`, if
+// given, is recorded once the input has been read (after the transpose).
+int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed);
+
+} // namespace
+
+extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t bytes, int on_device)
+{
+    SW_CHECK_ARG(p && taftp, "null argument");
+    const int rc = check_gulp(p, bytes);
+    if(rc) return rc;
     const void* in = taftp;
     if(!on_device) {
-        SW_HIP_TRY(hipMemcpyAsync(p->taftp_d, taftp, bytes, hipMemcpyHostToDevice, st));
+        SW_HIP_TRY(hipMemcpyAsync(p->taftp_d, taftp, bytes, hipMemcpyHostToDevice, p->stream));
         in = p->taftp_d;
     }
+    return run_gulp(p, in, nullptr);
+}
+
+extern "C" int sw_pipeline_prefetch(sw_pipeline* p, const void* h_taftp, size_t bytes)
+{
+    SW_CHECK_ARG(p && h_taftp, "null argument");
+    const int rc = check_gulp(p, bytes);
+    if(rc) return rc;
+    SW_CHECK_ARG(p->pre_count < 2, "two gulps are already prefetched (process one first)");
+    if(!p->copy_stream) {
+        SW_HIP_TRY(hipStreamCreateWithFlags(&p->copy_stream, hipStreamNonBlocking));
+        for(int i = 0; i < 2; ++i) {
+            SW_HIP_TRY(hipEventCreateWithFlags(&p->pre_copied[i], hipEventDisableTiming));
+            SW_HIP_TRY(hipEventCreateWithFlags(&p->pre_consumed[i], hipEventDisableTiming));
+        }
+    }
+    const int slot = (p->pre_head + p->pre_count) % 2;
+    if(!p->pre_d[slot]) {
+        const auto& c = p->cfg;
+        SW_HIP_TRY(hipMalloc(&p->pre_d[slot], (size_t)c.nsamples * c.nantennas * c.nchans * 2 * 2));
+    }
+    // the slot's previous gulp must have been read by its transpose
+    if(p->pre_used[slot]) SW_HIP_TRY(hipStreamWaitEvent(p->copy_stream, p->pre_consumed[slot], 0));
+    SW_HIP_TRY(hipMemcpyAsync(p->pre_d[slot], h_taftp, bytes, hipMemcpyHostToDevice, p->copy_stream));
+    SW_HIP_TRY(hipEventRecord(p->pre_copied[slot], p->copy_stream));
+    ++p->pre_count;
+    return SW_OK;
+}
+
+extern "C" int sw_pipeline_process_prefetched(sw_pipeline* p)
+{
+    SW_CHECK_ARG(p, "null argument");
+    SW_CHECK_ARG(p->pre_count > 0, "no prefetched gulp (call sw_pipeline_prefetch)");
+    const int slot = p->pre_head;
+    p->pre_head    = (p->pre_head + 1) % 2;
+    --p->pre_count;
+    SW_HIP_TRY(hipStreamWaitEvent(p->stream, p->pre_copied[slot], 0));
+    p->pre_used[slot] = true;
+    return run_gulp(p, p->pre_d[slot], p->pre_consumed[slot]);
+}
+
+extern "C" int sw_pipeline_synchronize(sw_pipeline* p)
+{
+    SW_CHECK_ARG(p, "null argument");
+    if(p->copy_stream) SW_HIP_TRY(hipStreamSynchronize(p->copy_stream));
+    SW_HIP_TRY(hipStreamSynchronize(p->stream));
+    return SW_OK;
+}
+
+namespace {
+
+int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
+{
+    const auto& c      = p->cfg;
+    const int nant_in  = p->header.nantennas;
+    hipStream_t st     = p->stream;
     const sw_bf_dims d = bf_dims(p);
     const int F = c.nchans, A = c.nantennas, S = p->nbeamsets;
     const double utc_offset_gulp =
@@ -452,6 +535,7 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
     // stage is left with nothing to time)
     rc = sw_transpose_fpat_stats(in, p->ftpa, p->stats, c.nsamples / 256, nant_in, A, F, p->stats_ws, st);
     if(rc) return rc;
+    if(consumed) SW_HIP_TRY(hipEventRecord(consumed, st));
     SW_HIP_TRY(hipEventRecord(p->ev[1], st));
     SW_HIP_TRY(hipEventRecord(p->ev[2], st));
     if(p->call_count == 0) {
@@ -575,3 +659,5 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
     ++p->call_count;
     return SW_OK;
 }
+
+} // namespace

@@ -113,6 +113,51 @@ def test_pipeline_forms_identical_and_host_input(sw, dev, stokes, sigma):
             np.testing.assert_array_equal(x[1], y[1])
 
 
+def test_pipeline_prefetched_input(sw, dev):
+    """Gulps copied ahead from pinned host memory (sw_pipeline_prefetch, up to
+    two pending) and processed by sw_pipeline_process_prefetched give the same
+    blocks as the same gulps processed from device memory."""
+    from skyweaver_amd import DeviceBuffer as DB, HostBuffer
+    H = T // 256
+    gulps = [sw.synth_voltages((H, A, F, 256, 2, 2), seed=0x5EED0301 + i, sigma=16.0) for i in range(3)]
+    weights = sw.synth_weights((F, B, A, 2), seed=0x5EED1301)
+    dms, fft_len = [0.0, 50.0], [1024, 2048]
+    a = _run_pipeline(sw, [DB.from_host(g) for g in gulps], weights, dms, fft_len, "mfma", True)
+
+    hb = []
+    for g in gulps:
+        h = HostBuffer(g.nbytes)
+        h.array[:] = g.reshape(-1).view(np.uint8)
+        hb.append(h)
+    p = sw.BeamformerPipeline(A, F, B, T, TS, dms, CFREQ, BW, fft_len=fft_len)
+    p.set_weights(weights)
+    p.init(A)
+    got = {"cb": [], "ib": [], "stats": []}
+
+    def grab(key):
+        def f(blk):
+            got[key].append((blk.dm_idx, sw.download_block(blk), blk.utc_offset, blk.tsamp))
+        return f
+    p.set_sinks(cb=grab("cb"), ib=grab("ib"), stats=grab("stats"))
+    with pytest.raises(sw.SkyweaverError, match="no prefetched gulp"):
+        p.process_prefetched()
+    p.prefetch(hb[0])
+    p.prefetch(hb[1])
+    with pytest.raises(sw.SkyweaverError, match="already prefetched"):
+        p.prefetch(hb[2])
+    p.process_prefetched()
+    p.prefetch(hb[2])
+    p.process_prefetched()
+    p.process_prefetched()
+    p.synchronize()
+    p.close()
+    for key in ("cb", "ib", "stats"):
+        assert len(got[key]) == len(a[key]) > 0
+        for x, y in zip(a[key], got[key]):
+            assert x[0] == y[0] and x[2] == y[2]
+            np.testing.assert_array_equal(x[1], y[1])
+
+
 def test_pipeline_rejects_wrong_gulp_size(sw, dev):
     p = sw.BeamformerPipeline(A, F, B, T, TS, [0.0], CFREQ, BW)
     p.set_weights(np.zeros((F, B, A, 2), np.int8))

@@ -192,7 +192,10 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
 // the threads read it (index 32 R1 k3 + 32 k1 + k2).  The inverse runs the
 // same steps backwards with conjugate twiddles and ends with each thread
 // holding output samples 512 n1 + t of its columns.  Four LDS exchanges per
-// DM instead of a read-write round trip per radix pass.  (Keeping the
+// DM instead of a read-write round trip per radix pass.  The step-2
+// twiddles of both directions are powers of W_1024 (R1 n3 k2 and
+// 16 k1 n2 in units of 1/N) and come from an LDS table t1k[j] = W_1024^j;
+// the others are built by product trees from one table value.  (Keeping the
 // spectrum for a second DM would not fit the registers: one DM per launch.)
 constexpr int kEx1Row = 528;   // exchange 1/4 row (c, k1): 512 points + 16 pad
 constexpr int kEx2Row = 560;   // exchange 2/3 block (c, k1): 32 rows of 17
@@ -226,7 +229,7 @@ __device__ __forceinline__ void apply_powers(float2* v, float2 w1)
 // k3 = 0..15).  The caller has synchronised since the exchange buffer was
 // last read.
 template <int R1>
-__device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddles& tw, int t)
+__device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddles& tw, const float2* t1k, int t)
 {
     constexpr int C = 32 / R1;
     const int n3 = t & 15, j2 = t >> 4;
@@ -246,7 +249,19 @@ __device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddl
     for(int n2 = 0; n2 < 32; ++n2) v[n2] = lds[j2 * kEx1Row + n2 * 16 + n3];
     __syncthreads();
     dft32<false>(v);
-    apply_powers<32>(v, tw(R1 * n3));
+    // x W_N^(R1 n3 k2) = W_1024^(2 n3 k2), straight from the table (the
+    // opaque step keeps the compiler from holding 31 table addresses in
+    // registers across units)
+    {
+        int step = 2 * n3;
+        asm volatile("" : "+v"(step));
+        const float2* tk = t1k;
+#pragma unroll
+        for(int k2 = 1; k2 < 32; ++k2) {
+            tk += step;
+            v[k2] = cmul(v[k2], *tk);
+        }
+    }
 #pragma unroll
     for(int k2 = 0; k2 < 32; ++k2) lds[j2 * kEx2Row + k2 * kEx2Stride + n3] = v[k2];
     __syncthreads();
@@ -265,8 +280,8 @@ __device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddl
 // the inverse steps 3'-1': v ends holding y[512 n1 + t] (unnormalised) at
 // c R1 + n1.
 template <int R1>
-__device__ __forceinline__ void reg_chirp_inverse(float2* v, float2* lds, const Twiddles& tw, int t,
-                                                  const float2* __restrict__ H)
+__device__ __forceinline__ void reg_chirp_inverse(float2* v, float2* lds, const Twiddles& tw, const float2* t1k,
+                                                  int t, const float2* __restrict__ H)
 {
     constexpr int C = 32 / R1;
     const int n3 = t & 15, j2 = t >> 4;
@@ -278,9 +293,11 @@ __device__ __forceinline__ void reg_chirp_inverse(float2* v, float2* lds, const 
         float2* x = v + 16 * h;
 #pragma unroll
         for(int k3 = 0; k3 < 16; ++k3) x[k3] = cmul(x[k3], H[32 * R1 * k3 + 32 * k1 + k2]);
-        // step 3': IDFT16 -> n3, x W_N^(-R1 n3 k2)
+        // step 3': IDFT16 -> n3, x W_N^(-R1 n3 k2) and W_N^(-k1 n3): the
+        // second factor of step 2's twiddle W_N^(-k1 (16 n2 + n3)) does not
+        // depend on k2, so by linearity it is applied before the IDFT32
         dft16<true>(x);
-        apply_powers<16>(x, cconj(tw(R1 * k2)));
+        apply_powers<16>(x, cconj(tw(R1 * k2 + k1)));
     }
     __syncthreads();  // all readers of the exchange buffer are done
 #pragma unroll
@@ -295,11 +312,16 @@ __device__ __forceinline__ void reg_chirp_inverse(float2* v, float2* lds, const 
     for(int k2 = 0; k2 < 32; ++k2) v[k2] = lds[j2 * kEx2Row + k2 * kEx2Stride + n3];
     __syncthreads();
     dft32<true>(v);
+    // x W_N^(-16 k1 n2) = conj(W_1024^((32 / R1) k1 n2)), from the table
     {
-        const float2 b = cconj(tw(k1_2 * n3));
-        apply_powers<32>(v, cconj(tw(16 * k1_2)));
+        int step = (32 / R1) * k1_2;
+        asm volatile("" : "+v"(step));
+        const float2* tk = t1k;
 #pragma unroll
-        for(int n2 = 0; n2 < 32; ++n2) v[n2] = cmul(v[n2], b);
+        for(int n2 = 1; n2 < 32; ++n2) {
+            tk += step;
+            v[n2] = cmul(v[n2], cconj(*tk));
+        }
     }
 #pragma unroll
     for(int n2 = 0; n2 < 32; ++n2) lds[j2 * kEx1Row + n2 * 16 + n3] = v[n2];
@@ -345,16 +367,18 @@ __device__ __forceinline__ void load_column(const CdParams& p, int f, int col, l
 
 // Quantise y / N and write the kept positions [O/2, N - O/2) of a segment
 // into the output ring (ring index j0 + position, wrapped once).
+__device__ __forceinline__ void store_kept(const CdParams& p, int d, size_t colbase, int j0, int pos, float2 y)
+{
+    int jj = j0 + pos;
+    if(jj >= p.ylen) jj -= p.ylen;
+    store_c8(p.out[d] + (colbase + jj) * 2, y);
+    if(p.outf[d]) p.outf[d][colbase + jj] = y;
+}
 __device__ __forceinline__ void store_output(const CdParams& p, int d, size_t colbase, int j0, int pos, int N,
                                              float2 y)
 {
     const int half = p.O / 2;
-    if(pos >= half && pos < N - half) {
-        int jj = j0 + pos;
-        if(jj >= p.ylen) jj -= p.ylen;
-        store_c8(p.out[d] + (colbase + jj) * 2, y);
-        if(p.outf[d]) p.outf[d][colbase + jj] = y;
-    }
+    if(pos >= half && pos < N - half) store_kept(p, d, colbase, j0, pos, y);
 }
 
 // Persistent: a workgroup (one per CU, the exchange buffer fills the LDS)
@@ -366,9 +390,11 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
 {
     constexpr int N = 512 * R1;
     constexpr int C = 32 / R1;
-    __shared__ float2 lds[kRegLds + TwiddleLds<N>::SIZE];
+    __shared__ float2 lds[kRegLds + TwiddleLds<N>::SIZE + 1024];
     const Twiddles tw = load_twiddles<N, 512>(lds + kRegLds, p.tw);
+    float2* t1k = lds + kRegLds + TwiddleLds<N>::SIZE;  // W_1024^j = W_N^(j N / 1024)
     const int t = threadIdx.x;
+    for(int j = t; j < 1024; j += 512) t1k[j] = p.tw[j * (N / 1024)];
     const float inv_n = 1.0f / (float)N;
     const int ngrp = p.PA / C;
     uint32_t nxt[16];  // two raw samples per register: value i = c R1 + n1
@@ -391,15 +417,26 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
         }
         if(u + (int)gridDim.x < nunits) fetch(u + gridDim.x);  // in flight during the transforms
         __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
-        reg_forward<R1>(v, lds, tw, t);
-        reg_chirp_inverse<R1>(v, lds, tw, t, p.chirp[d] + (size_t)f * N);
+        reg_forward<R1>(v, lds, tw, t1k, t);
+        reg_chirp_inverse<R1>(v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N);
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
+        // positions 512 .. N - 513 are all kept when O / 2 <= 512; the
+        // opaque copy of t keeps the position tests of the first and last
+        // rows from being precomputed and held across units
+        const bool interior = p.O / 2 <= 512;
+        int tt = t;
+        asm volatile("" : "+v"(tt));
 #pragma unroll
         for(int c = 0; c < C; ++c) {
             const size_t cb = ((size_t)f * p.PA + cg * C + c) * p.ylen;
 #pragma unroll
-            for(int n1 = 0; n1 < R1; ++n1)
-                store_output(p, d, cb, j0, 512 * n1 + t, N, cscale(v[c * R1 + n1], inv_n));
+            for(int n1 = 0; n1 < R1; ++n1) {
+                const float2 y = cscale(v[c * R1 + n1], inv_n);
+                if(interior && n1 > 0 && n1 < R1 - 1)
+                    store_kept(p, d, cb, j0, 512 * n1 + tt, y);
+                else
+                    store_output(p, d, cb, j0, 512 * n1 + tt, N, y);
+            }
         }
     }
 }
@@ -418,9 +455,11 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
 __global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, int nunits, float2* scratch)
 {
     constexpr int L = 16384, N = 2 * L;
-    __shared__ float2 lds[kRegLds + TwiddleLds<L>::SIZE];
+    __shared__ float2 lds[kRegLds + TwiddleLds<L>::SIZE + 1024];
     const Twiddles tw = load_twiddles<L, 512>(lds + kRegLds, p.tw);
+    float2* t1k = lds + kRegLds + TwiddleLds<L>::SIZE;  // W_1024^j = W_L^(j L / 1024)
     const int t = threadIdx.x;
+    for(int j = t; j < 1024; j += 512) t1k[j] = p.tw[j * (L / 1024)];
     const float inv_n = 1.0f / (float)N;
     uint32_t raw[32];  // x[512 n1 + t] at 0..15, x[L + 512 n1 + t] at 16..31, two per register
     for(int u = blockIdx.x; u < nunits; u += gridDim.x) {
@@ -443,8 +482,8 @@ __global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, i
             v[2 * i + 1] = cadd(unpack_c8(raw[i] >> 16), unpack_c8(raw[16 + i] >> 16));
         }
         __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
-        reg_forward<32>(v, lds, tw, t);
-        reg_chirp_inverse<32>(v, lds, tw, t, H);
+        reg_forward<32>(v, lds, tw, t1k, t);
+        reg_chirp_inverse<32>(v, lds, tw, t1k, t, H);
 #pragma unroll
         for(int n1 = 0; n1 < 32; ++n1) mine[512 * n1] = v[n1];
         // ---- r = 1: Z_1 = W_N^n (x[n] - x[n + L])
@@ -457,17 +496,27 @@ __global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, i
 #pragma unroll
         for(int n1 = 0; n1 < 32; ++n1) v[n1] = cmul(v[n1], wn[512 * n1]);
         __syncthreads();
-        reg_forward<32>(v, lds, tw, t);
-        reg_chirp_inverse<32>(v, lds, tw, t, H + L);
+        reg_forward<32>(v, lds, tw, t1k, t);
+        reg_chirp_inverse<32>(v, lds, tw, t1k, t, H + L);
         // ---- y[n] = V_0 + W_N^(-n) V_1, y[n + L] = V_0 - W_N^(-n) V_1
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;
         const size_t cb = ((size_t)f * p.PA + col) * p.ylen;
+        const bool interior = p.O / 2 <= 512;  // positions 512 .. N - 513 all kept
+        int tt = t;
+        asm volatile("" : "+v"(tt));
 #pragma unroll
         for(int n1 = 0; n1 < 32; ++n1) {
             const float2 a = mine[512 * n1];
             const float2 b = cmul(v[n1], cconj(wn[512 * n1]));
-            store_output(p, d, cb, j0, 512 * n1 + t, N, cscale(cadd(a, b), inv_n));
-            store_output(p, d, cb, j0, L + 512 * n1 + t, N, cscale(csub(a, b), inv_n));
+            const float2 y0 = cscale(cadd(a, b), inv_n), y1 = cscale(csub(a, b), inv_n);
+            if(interior && n1 > 0)
+                store_kept(p, d, cb, j0, 512 * n1 + tt, y0);
+            else
+                store_output(p, d, cb, j0, 512 * n1 + tt, N, y0);
+            if(interior && n1 < 31)
+                store_kept(p, d, cb, j0, L + 512 * n1 + tt, y1);
+            else
+                store_output(p, d, cb, j0, L + 512 * n1 + tt, N, y1);
         }
     }
 }

@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
 // holding output samples 512 n1 + t of its columns.  Four LDS exchanges per
 // DM instead of a read-write round trip per radix pass.  The step-2
 // twiddles of both directions are powers of W_1024 (R1 n3 k2 and
-// 16 k1 n2 in units of 1/N) and come from an LDS table t1k[j] = W_1024^j;
+// 16 k1 n2 in units of 1/N) and come from LDS tables of W_1024^j and W_512^j;
 // the others are built by product trees from one table value.  (Keeping the
 // spectrum for a second DM would not fit the registers: one DM per launch.)
 constexpr int kEx1Row = 528;   // exchange 1/4 row (c, k1): 512 points + 16 pad
@@ -253,9 +253,12 @@ __device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddl
     // opaque step keeps the compiler from holding 31 table addresses in
     // registers across units)
     {
-        int step = 2 * n3;
+        // W_512^j sits at t1k[1024 + j]: with the 8-byte stride n3 the 16
+        // distinct addresses of a wave fall on fewer common banks than with
+        // W_1024's 16-byte stride
+        int step = n3;
         asm volatile("" : "+v"(step));
-        const float2* tk = t1k;
+        const float2* tk = t1k + 1024;
 #pragma unroll
         for(int k2 = 1; k2 < 32; ++k2) {
             tk += step;
@@ -390,11 +393,12 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
 {
     constexpr int N = 512 * R1;
     constexpr int C = 32 / R1;
-    __shared__ float2 lds[kRegLds + TwiddleLds<N>::SIZE + 1024];
+    __shared__ float2 lds[kRegLds + TwiddleLds<N>::SIZE + 1536];
     const Twiddles tw = load_twiddles<N, 512>(lds + kRegLds, p.tw);
-    float2* t1k = lds + kRegLds + TwiddleLds<N>::SIZE;  // W_1024^j = W_N^(j N / 1024)
+    float2* t1k = lds + kRegLds + TwiddleLds<N>::SIZE;  // W_1024^j = W_N^(j N / 1024), then W_512^j
     const int t = threadIdx.x;
     for(int j = t; j < 1024; j += 512) t1k[j] = p.tw[j * (N / 1024)];
+    t1k[1024 + t] = p.tw[t * (N / 512)];
     const float inv_n = 1.0f / (float)N;
     const int ngrp = p.PA / C;
     uint32_t nxt[16];  // two raw samples per register: value i = c R1 + n1
@@ -455,68 +459,68 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
 __global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, int nunits, float2* scratch)
 {
     constexpr int L = 16384, N = 2 * L;
-    __shared__ float2 lds[kRegLds + TwiddleLds<L>::SIZE + 1024];
+    __shared__ float2 lds[kRegLds + TwiddleLds<L>::SIZE + 1536];
     const Twiddles tw = load_twiddles<L, 512>(lds + kRegLds, p.tw);
-    float2* t1k = lds + kRegLds + TwiddleLds<L>::SIZE;  // W_1024^j = W_L^(j L / 1024)
+    float2* t1k = lds + kRegLds + TwiddleLds<L>::SIZE;  // W_1024^j = W_L^(j L / 1024), then W_512^j
     const int t = threadIdx.x;
     for(int j = t; j < 1024; j += 512) t1k[j] = p.tw[j * (L / 1024)];
+    t1k[1024 + t] = p.tw[t * (L / 512)];
     const float inv_n = 1.0f / (float)N;
     uint32_t raw[32];  // x[512 n1 + t] at 0..15, x[L + 512 n1 + t] at 16..31, two per register
     for(int u = blockIdx.x; u < nunits; u += gridDim.x) {
-        // the thread's slots, formed afresh per unit (an opaque offset keeps
-        // the compiler from holding 32 precomputed addresses in registers)
-        int off = t;
-        asm volatile("" : "+v"(off));
-        float2* mine = scratch + (size_t)blockIdx.x * L + off;  // [n1][512]
-        const float2* wn = p.twN + off;                         // W_N^(512 n1 + t) at wn[512 n1]
         const int col = u % p.PA, fs = u / p.PA;
         const int f = fs / p.nseg, s = fs % p.nseg;
         const long b = p.in0 + (long)s * p.hop;
         const float2* H = p.chirp[d] + (size_t)f * N;
         float2 v[32];
-        // ---- r = 0: Z_0 = x[n] + x[n + L]
-        load_column<32>(p, f, col, b, t, raw);
+        // the two halves share one copy of the transform code (a loop that
+        // is not unrolled): the kernel then fits the instruction cache
+#pragma unroll 1
+        for(int r = 0; r < 2; ++r) {
+            // the thread's slots, formed afresh per half (an opaque offset
+            // keeps the compiler from holding 32 precomputed addresses in
+            // registers across the transforms)
+            int off = t;
+            asm volatile("" : "+v"(off));
+            float2* mine = scratch + (size_t)blockIdx.x * L + off;  // [n1][512]
+            const float2* wn = p.twN + off;                         // W_N^(512 n1 + t) at wn[512 n1]
+            load_column<32>(p, f, col, b, off, raw);
+            if(r == 0) {  // Z_0 = x[n] + x[n + L]
 #pragma unroll
-        for(int i = 0; i < 16; ++i) {
-            v[2 * i]     = cadd(unpack_c8(raw[i]), unpack_c8(raw[16 + i]));
-            v[2 * i + 1] = cadd(unpack_c8(raw[i] >> 16), unpack_c8(raw[16 + i] >> 16));
+                for(int i = 0; i < 16; ++i) {
+                    v[2 * i]     = cadd(unpack_c8(raw[i]), unpack_c8(raw[16 + i]));
+                    v[2 * i + 1] = cadd(unpack_c8(raw[i] >> 16), unpack_c8(raw[16 + i] >> 16));
+                }
+            } else {  // Z_1 = W_N^n (x[n] - x[n + L])
+#pragma unroll
+                for(int i = 0; i < 16; ++i) {
+                    v[2 * i]     = csub(unpack_c8(raw[i]), unpack_c8(raw[16 + i]));
+                    v[2 * i + 1] = csub(unpack_c8(raw[i] >> 16), unpack_c8(raw[16 + i] >> 16));
+                }
+#pragma unroll
+                for(int n1 = 0; n1 < 32; ++n1) v[n1] = cmul(v[n1], wn[512 * n1]);
+            }
+            __syncthreads();  // twiddle tables; previous readers of the exchange buffer
+            reg_forward<32>(v, lds, tw, t1k, t);
+            reg_chirp_inverse<32>(v, lds, tw, t1k, t, H + r * L);
+            if(r == 0) {
+#pragma unroll
+                for(int n1 = 0; n1 < 32; ++n1) mine[512 * n1] = v[n1];
+            }
         }
-        __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
-        reg_forward<32>(v, lds, tw, t1k, t);
-        reg_chirp_inverse<32>(v, lds, tw, t1k, t, H);
-#pragma unroll
-        for(int n1 = 0; n1 < 32; ++n1) mine[512 * n1] = v[n1];
-        // ---- r = 1: Z_1 = W_N^n (x[n] - x[n + L])
-        load_column<32>(p, f, col, b, t, raw);
-#pragma unroll
-        for(int i = 0; i < 16; ++i) {
-            v[2 * i]     = csub(unpack_c8(raw[i]), unpack_c8(raw[16 + i]));
-            v[2 * i + 1] = csub(unpack_c8(raw[i] >> 16), unpack_c8(raw[16 + i] >> 16));
-        }
-#pragma unroll
-        for(int n1 = 0; n1 < 32; ++n1) v[n1] = cmul(v[n1], wn[512 * n1]);
-        __syncthreads();
-        reg_forward<32>(v, lds, tw, t1k, t);
-        reg_chirp_inverse<32>(v, lds, tw, t1k, t, H + L);
         // ---- y[n] = V_0 + W_N^(-n) V_1, y[n + L] = V_0 - W_N^(-n) V_1
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;
         const size_t cb = ((size_t)f * p.PA + col) * p.ylen;
-        const bool interior = p.O / 2 <= 512;  // positions 512 .. N - 513 all kept
-        int tt = t;
+        int tt = t;  // opaque: addresses and position tests formed here, not held across units
         asm volatile("" : "+v"(tt));
+        const float2* mine = scratch + (size_t)blockIdx.x * L + tt;
+        const float2* wn = p.twN + tt;
 #pragma unroll
         for(int n1 = 0; n1 < 32; ++n1) {
             const float2 a = mine[512 * n1];
             const float2 b = cmul(v[n1], cconj(wn[512 * n1]));
-            const float2 y0 = cscale(cadd(a, b), inv_n), y1 = cscale(csub(a, b), inv_n);
-            if(interior && n1 > 0)
-                store_kept(p, d, cb, j0, 512 * n1 + tt, y0);
-            else
-                store_output(p, d, cb, j0, 512 * n1 + tt, N, y0);
-            if(interior && n1 < 31)
-                store_kept(p, d, cb, j0, L + 512 * n1 + tt, y1);
-            else
-                store_output(p, d, cb, j0, L + 512 * n1 + tt, N, y1);
+            store_output(p, d, cb, j0, 512 * n1 + tt, N, cscale(cadd(a, b), inv_n));
+            store_output(p, d, cb, j0, L + 512 * n1 + tt, N, cscale(csub(a, b), inv_n));
         }
     }
 }

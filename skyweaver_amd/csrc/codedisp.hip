@@ -399,7 +399,6 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
     const int t = threadIdx.x;
     for(int j = t; j < 1024; j += 512) t1k[j] = p.tw[j * (N / 1024)];
     t1k[1024 + t] = p.tw[t * (N / 512)];
-    const float inv_n = 1.0f / (float)N;
     const int ngrp = p.PA / C;
     uint32_t nxt[16];  // two raw samples per register: value i = c R1 + n1
     auto fetch = [&](int u) {
@@ -435,7 +434,7 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
             const size_t cb = ((size_t)f * p.PA + cg * C + c) * p.ylen;
 #pragma unroll
             for(int n1 = 0; n1 < R1; ++n1) {
-                const float2 y = cscale(v[c * R1 + n1], inv_n);
+                const float2 y = v[c * R1 + n1];  // already / N (chirp)
                 if(interior && n1 > 0 && n1 < R1 - 1)
                     store_kept(p, d, cb, j0, 512 * n1 + tt, y);
                 else
@@ -465,7 +464,6 @@ __global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, i
     const int t = threadIdx.x;
     for(int j = t; j < 1024; j += 512) t1k[j] = p.tw[j * (L / 1024)];
     t1k[1024 + t] = p.tw[t * (L / 512)];
-    const float inv_n = 1.0f / (float)N;
     uint32_t raw[32];  // x[512 n1 + t] at 0..15, x[L + 512 n1 + t] at 16..31, two per register
     for(int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int col = u % p.PA, fs = u / p.PA;
@@ -519,8 +517,8 @@ __global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, i
         for(int n1 = 0; n1 < 32; ++n1) {
             const float2 a = mine[512 * n1];
             const float2 b = cmul(v[n1], cconj(wn[512 * n1]));
-            store_output(p, d, cb, j0, 512 * n1 + tt, N, cscale(cadd(a, b), inv_n));
-            store_output(p, d, cb, j0, L + 512 * n1 + tt, N, cscale(csub(a, b), inv_n));
+            store_output(p, d, cb, j0, 512 * n1 + tt, N, cadd(a, b));  // V_0, V_1 already / N (chirp)
+            store_output(p, d, cb, j0, L + 512 * n1 + tt, N, csub(a, b));
         }
     }
 }
@@ -610,11 +608,16 @@ __global__ void chirp_kernel(float2* out, int nchans, int N, int RP, int R1, dou
     sincos(phase, &s, &c);
     const int L = N / RP, m = k / RP, r = k % RP;
     int mp = m;
+    float scale = 1.0f;
     if(R1 > 0) {
         const int k1 = m % R1, k2 = (m / R1) % 32, k3 = m / (32 * R1);
         mp = 32 * R1 * k3 + 32 * k1 + k2;
+        // the register kernels' chirp carries the 1/N of the inverse
+        // transform: a power of two, so every value of the inverse is
+        // scaled exactly and the result is bit-identical to dividing at the end
+        scale = 1.0f / (float)N;
     }
-    out[((size_t)chan * RP + r) * L + mp] = make_float2((float)c, (float)s);
+    out[((size_t)chan * RP + r) * L + mp] = make_float2((float)c * scale, (float)s * scale);
 }
 
 // new history = the last Oh samples of (old history ++ gulp), per column

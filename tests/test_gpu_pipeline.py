@@ -250,3 +250,67 @@ def test_pipeline_incoherent_dedispersion(sw, orc, dev):
         assert utc == pytest.approx((k * 300 + mx) * tsamp_cb, rel=1e-9)
         assert tsb == pytest.approx(tsamp_cb * 2, rel=1e-12)
     assert times["incoherent_dedispersion"] >= 0.0
+
+
+def test_pipeline_full_size_beamsets_iquv(sw, orc, dev):
+    """A whole subband with uneven beamsets: 80 antennas, 128 channels, 65536
+    samples, 1024 beams in four beamsets over antennas 0-79, 0-63, 0-39 and
+    0-19 (0/1 antenna weights; the complex weights of excluded antennas are
+    zero, as delay models make them), full Stokes, tscrunch 16, DM 0.  The
+    chosen form (MFMA) and the plain form (v_dot4) agree bit for bit on every
+    coherent and incoherent beam and statistic; the first 256 output samples
+    of channels 0 and 127 are checked against the oracle chain."""
+    from skyweaver_amd import DeviceBuffer as DB
+    A4, F4, T4, B4, TS4 = 80, 128, 65536, 1024, 16
+    BW4 = 208984.375 * F4
+    bounds = [(0, 256, 80), (256, 640, 64), (640, 896, 40), (896, 1024, 20)]
+    S4 = len(bounds)
+    print("full-size IQUV beamsets: synthesising 2.7 GB", flush=True)
+    taftp = sw.synth_voltages((T4 // 256, A4, F4, 256, 2, 2), seed=0x5EED0004, sigma=16.0, nthreads=16)
+    weights = sw.synth_weights((F4, B4, A4, 2), seed=0x5EED1004)
+    bmap = np.zeros(B4, np.int32)
+    antw = np.zeros((S4, A4), np.float32)
+    for s, (b0, b1, na) in enumerate(bounds):
+        bmap[b0:b1] = s
+        antw[s, :na] = 1
+        weights[:, b0:b1, na:] = 0
+    d_in = DB.from_host(taftp)
+
+    def run(form, batched):
+        p = sw.BeamformerPipeline(A4, F4, B4, T4, TS4, [0.0], 1284e6, BW4, stokes="IQUV", form=form,
+                                  batched=batched)
+        p.set_weights(weights)
+        p.set_beamsets(bmap, antw)
+        p.init(A4)
+        got = {}
+
+        def grab(key):
+            def f(blk):
+                got[key] = sw.download_block(blk)
+            return f
+        p.set_sinks(cb=grab("cb"), ib=grab("ib"), stats=grab("stats"))
+        p(d_in)
+        p.close()
+        return got
+
+    print("chosen form", flush=True)
+    a = run("mfma", True)
+    print("plain form", flush=True)
+    b = run("plain", False)
+    for key in ("cb", "ib", "stats"):
+        np.testing.assert_array_equal(a[key], b[key])
+    cb = a["cb"].view(np.int8).reshape(T4 // TS4, F4, B4, 4)
+    ib = a["ib"].view(np.int8).reshape(S4, T4 // TS4, F4, 4)
+    assert np.unique(cb).size > 100
+
+    stats = a["stats"].view(np.float32).reshape(F4, 2, A4, 4)
+    scal = orc.update_scalings(stats[..., 1], antw, S4, TS4, TS4, 24.0)
+    nt = 4096
+    for f in (0, F4 - 1):
+        ftpa = orc.transpose(np.ascontiguousarray(taftp[:, :, f:f + 1]), A4)[:, :nt]
+        sel = np.arange(S4) * F4 + f
+        _, ib_q = orc.ib_beamform(ftpa, TS4, scal["ib_scaling"][sel], scal["ib_offsets"][sel], antw, S4, "IQUV")
+        cb_o = orc.cb_beamform(ftpa, weights[f:f + 1], TS4, scal["cb_scaling"][sel], scal["cb_offsets"][sel],
+                               antw, bmap, S4, "IQUV", True)
+        np.testing.assert_array_equal(cb[:nt // TS4, f:f + 1], cb_o)
+        np.testing.assert_array_equal(ib[:, :nt // TS4, f:f + 1], ib_q)

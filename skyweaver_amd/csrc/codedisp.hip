@@ -15,14 +15,15 @@
 //
 // Data are time-major ([F][P*A][T] complex int8, one column per (pol,
 // antenna)) so that every transform reads contiguous runs:
-//  * N = 4096, 8192, 16384 (codedisp_reg_kernel<N/512>): a column is held in
+//  * N = 1024 ... 16384 (codedisp_reg_kernel<N/512>): a column is held in
 //    the registers of 512 threads (32 points each) and transformed as
 //    DFT-R1 x DFT-32 x DFT-16 with two LDS exchanges per direction; the
 //    spectrum stays in registers in a permuted order that the chirp table
 //    shares.  Persistent workgroups walk (channel, segment, column group).
 //  * N = 32768 (codedisp_reg2_kernel): two 16384-point halves of a radix-2
 //    decimation in frequency, fused in one persistent kernel.
-//  * other lengths: one workgroup per (channel, segment, group of C columns)
+//  * other lengths (shorter, or column counts the register kernels do not
+//    divide into units): one workgroup per (channel, segment, group of C columns)
 //    keeps C x N points in LDS (Stockham passes); with two LDS buffers the
 //    forward FFT is shared by every DM of the launch.  N > 16384 there is a
 //    radix-RP decimation-in-frequency stage, the LDS kernel on each of the
@@ -180,7 +181,7 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
     }
 }
 
-// N = 512 R1 (R1 = 8, 16, 32: N = 4096, 8192, 16384) held in registers:
+// N = 512 R1 (R1 = 2 ... 32: N = 1024 ... 16384) held in registers:
 // 512 threads x 32 points per unit, a unit being C = 32 / R1 columns of one
 // (channel, segment).  With n = 512 n1 + 16 n2 + n3 and
 // k = k1 + R1 k2 + 32 R1 k3 (n1, k1 < R1; n2, k2 < 32; n3, k3 < 16) the
@@ -205,7 +206,7 @@ constexpr int kRegLds = 32 * kEx2Row;
 // 32 banks apart (528 and 560 float2 are 32 dwords mod 64), and the rows of
 // 17 points put 32 consecutive k2 on distinct banks.
 
-// v[k] *= w1^k for k = 1..R-1 (R = 8, 16 or 32), powers by the product tree
+// v[k] *= w1^k for k = 1..R-1 (R = 2 ... 32), powers by the product tree
 // of twiddle_powers, built 16 at a time to bound register use
 template <int R>
 __device__ __forceinline__ void apply_powers(float2* v, float2 w1)
@@ -681,13 +682,13 @@ int launch_lds(int L, const CdParams& p, int blocks, hipStream_t st)
     return SW_OK;
 }
 
-// the register kernels take N = 4096, 8192, 16384 (R1 = N / 512) when the
+// the register kernels take N = 1024 ... 16384 (R1 = N / 512) when the
 // columns split into units of 32 / R1, and N = 32768 as two halves of 16384
 // (RP = 2); 0: the LDS / global-memory kernels
 int reg_r1(int N, int RP, int PA)
 {
     if(RP == 2 && N == 32768) return 32;
-    if(RP != 1 || (N != 4096 && N != 8192 && N != 16384)) return 0;
+    if(RP != 1 || N < 1024 || N > 16384 || (N & (N - 1)) != 0) return 0;
     const int R1 = N / 512;
     return PA % (32 / R1) == 0 ? R1 : 0;
 }
@@ -710,7 +711,7 @@ int launch_reg(int R1, const CdParams& p, int nunits, float2* scratch, hipStream
         hipLaunchKernelGGL((codedisp_reg_kernel<RR>), grid, dim3(512), 0, st, p, d, nunits);  \
         break;
         switch(R1) {
-            SW_REG(8) SW_REG(16) SW_REG(32)
+            SW_REG(2) SW_REG(4) SW_REG(8) SW_REG(16) SW_REG(32)
         default: set_error("codedisp: no register kernel for this length"); return SW_ERR_INVALID;
         }
 #undef SW_REG

@@ -314,3 +314,48 @@ def test_pipeline_full_size_beamsets_iquv(sw, orc, dev):
                                antw, bmap, S4, "IQUV", True)
         np.testing.assert_array_equal(cb[:nt // TS4, f:f + 1], cb_o)
         np.testing.assert_array_equal(ib[:, :nt // TS4, f:f + 1], ib_q)
+
+
+def test_pipeline_full_size_8dm_forms_identical(sw, dev):
+    """64 antennas x 64 channels x 131072 samples, 512 beams, 8 coherent DMs
+    0-350 with their FFT lengths 4096-32768 (uneven), tscrunch 8, Stokes I,
+    two gulps (the second reads its dedispersion history from the first):
+    the chosen form (MFMA beams on the dedisperser's ring, batched DMs) and
+    the plain form (v_dot4 beams on corner-turned FTPA, one DM per launch)
+    give identical coherent beams, incoherent beams and statistics."""
+    from skyweaver_amd import DeviceBuffer as DB
+    A3, F3, T3, B3, TS3 = 64, 64, 131072, 512, 8
+    dms = [0.0, 5.0, 20.0, 50.0, 100.0, 180.0, 260.0, 350.0]
+    fft = [4096, 4096, 8192, 8192, 16384, 16384, 32768, 32768]
+    BW3 = 208984.375 * F3
+    print("full-size 8 DMs: synthesising 2 x 2.1 GB", flush=True)
+    gulps = [DB.from_host(sw.synth_voltages((T3 // 256, A3, F3, 256, 2, 2), seed=0x5EED0003 + i, sigma=16.0,
+                                            nthreads=16)) for i in range(2)]
+    weights = sw.synth_weights((F3, B3, A3, 2), seed=0x5EED1003)
+
+    def run(form, batched):
+        p = sw.BeamformerPipeline(A3, F3, B3, T3, TS3, dms, 1284e6, BW3, fft_len=fft, form=form, batched=batched)
+        p.set_weights(weights)
+        p.init(A3)
+        got = {"cb": [], "ib": [], "stats": []}
+
+        def grab(key):
+            def f(blk):
+                got[key].append((blk.dm_idx, sw.download_block(blk)))
+            return f
+        p.set_sinks(cb=grab("cb"), ib=grab("ib"), stats=grab("stats"))
+        for g in gulps:
+            p(g)
+        p.close()
+        return got
+
+    print("chosen form", flush=True)
+    a = run("mfma", True)
+    print("plain form", flush=True)
+    b = run("plain", False)
+    assert len(a["cb"]) == len(b["cb"]) == 16 and len(a["stats"]) == 2
+    for key in ("cb", "ib", "stats"):
+        for x, y in zip(a[key], b[key]):
+            assert x[0] == y[0]
+            np.testing.assert_array_equal(x[1], y[1])
+    assert np.unique(a["cb"][-1][1]).size > 100

@@ -27,6 +27,16 @@ def test_argument_errors_do_not_touch_the_gpu(sw):
     d = sw.dims(4, 64, 6, 32, 16)
     assert lib.sw_cb_beamform(16, 16, 16, 16, 16, 16, 16, ctypes.byref(d), 0, None, None) == -1
     assert "multiple of 4" in sw.last_error()
+    assert lib.sw_transpose_fpat_stats(16, 16, 16, 1, 8, 4, 1, None, None) == -1
+    assert "nantennas" in sw.last_error()
+    assert lib.sw_transpose_fpat_stats(8, 16, 16, 1, 4, 4, 1, None, None) == -1
+    assert "aligned" in sw.last_error()
+    assert lib.sw_transpose_fpat_stats(16, 16, 16, 1 << 14, 4, 4, 1, None, None) == -1
+    assert "exact sums" in sw.last_error()
+    assert lib.sw_pipeline_prefetch(None, None, 0) == -1
+    assert "null" in sw.last_error()
+    assert lib.sw_pipeline_process_prefetched(None) == -1
+    assert lib.sw_pipeline_synchronize(None) == -1
 
 
 def test_update_scalings_matches_oracle_bitwise(sw, orc):

@@ -188,12 +188,14 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
 // forward transform is
 //   DFT_R1 over n1 (thread t = 16 n2 + n3, each column), x W_N^(t k1)
 //   -> exchange -> DFT32 over n2 (thread (c, k1, n3)), x W_N^(R1 n3 k2)
-//   -> exchange -> DFT16 over n3 (thread (c, k1, k2), two per thread),
+//   -> exchange -> DFT16 over n3 (thread (c, k1, n3) takes k2 = n3, n3 + 16),
 // which leaves X[k] in registers; the chirp is stored permuted to the order
 // the threads read it (index 32 R1 k3 + 32 k1 + k2).  The inverse runs the
 // same steps backwards with conjugate twiddles and ends with each thread
 // holding output samples 512 n1 + t of its columns.  Four LDS exchanges per
-// DM instead of a read-write round trip per radix pass.  The step-2
+// DM instead of a read-write round trip per radix pass; the middle two (step
+// 2 <-> step 3) stay inside the 16 lanes of a row, i.e. inside one wave, and
+// need no workgroup barrier.  The step-2
 // twiddles of both directions are powers of W_1024 (R1 n3 k2 and
 // 16 k1 n2 in units of 1/N) and come from LDS tables of W_1024^j and W_512^j;
 // the others are built by product trees from one table value.  (Keeping the
@@ -205,6 +207,16 @@ constexpr int kRegLds = 32 * kEx2Row;
 // Lane groups of a wave that read or write one exchange row at a time start
 // 32 banks apart (528 and 560 float2 are 32 dwords mod 64), and the rows of
 // 17 points put 32 consecutive k2 on distinct banks.
+
+// An LDS exchange among the lanes of one wave: a wave's LDS operations are
+// performed in program order, so a wave-scope fence (which also keeps the
+// compiler from moving the accesses across it) is all the ordering needed.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // v[k] *= w1^k for k = 1..R-1 (R = 2 ... 32), powers by the product tree
 // of twiddle_powers, built 16 at a time to bound register use
@@ -268,14 +280,16 @@ __device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddl
     }
 #pragma unroll
     for(int k2 = 0; k2 < 32; ++k2) lds[j2 * kEx2Row + k2 * kEx2Stride + n3] = v[k2];
-    __syncthreads();
-    // step 3: pairs q = t, t + 512 (row q >> 5, k2 = q & 31): DFT16 over n3 -> k3
+    wave_lds_sync();
+    // step 3: thread (j2, n3) takes k2 = n3 and n3 + 16 of its own row j2,
+    // which the 16 lanes of that row (in this wave) have just written: DFT16
+    // over n3 -> k3
 #pragma unroll
     for(int h = 0; h < 2; ++h) {
-        const int q = t + 512 * h;
+        const int k2 = n3 + 16 * h;
         float2* x = v + 16 * h;
 #pragma unroll
-        for(int m = 0; m < 16; ++m) x[m] = lds[(q >> 5) * kEx2Row + (q & 31) * kEx2Stride + m];
+        for(int m = 0; m < 16; ++m) x[m] = lds[j2 * kEx2Row + k2 * kEx2Stride + m];
         dft16<false>(x);
     }
 }
@@ -292,8 +306,7 @@ __device__ __forceinline__ void reg_chirp_inverse(float2* v, float2* lds, const 
     const int k1_2 = j2 % R1;
 #pragma unroll
     for(int h = 0; h < 2; ++h) {
-        const int q = t + 512 * h;
-        const int k2 = q & 31, k1 = (q >> 5) % R1;
+        const int k2 = n3 + 16 * h, k1 = k1_2;
         float2* x = v + 16 * h;
 #pragma unroll
         for(int k3 = 0; k3 < 16; ++k3) x[k3] = cmul(x[k3], H[32 * R1 * k3 + 32 * k1 + k2]);
@@ -303,18 +316,20 @@ __device__ __forceinline__ void reg_chirp_inverse(float2* v, float2* lds, const 
         dft16<true>(x);
         apply_powers<16>(x, cconj(tw(R1 * k2 + k1)));
     }
-    __syncthreads();  // all readers of the exchange buffer are done
+    // row j2 of the exchange buffer is this wave's alone between the two
+    // workgroup barriers of a transform: written and read back by its own
+    // 16 lanes
 #pragma unroll
     for(int h = 0; h < 2; ++h) {
-        const int q = t + 512 * h;
+        const int k2 = n3 + 16 * h;
 #pragma unroll
-        for(int m = 0; m < 16; ++m) lds[(q >> 5) * kEx2Row + (q & 31) * kEx2Stride + m] = v[16 * h + m];
+        for(int m = 0; m < 16; ++m) lds[j2 * kEx2Row + k2 * kEx2Stride + m] = v[16 * h + m];
     }
-    __syncthreads();
+    wave_lds_sync();
     // step 2': thread (j2, n3) holds k2 = 0..31; IDFT32 -> n2, x W_N^(-k1 (16 n2 + n3))
 #pragma unroll
     for(int k2 = 0; k2 < 32; ++k2) v[k2] = lds[j2 * kEx2Row + k2 * kEx2Stride + n3];
-    __syncthreads();
+    __syncthreads();  // every wave is done with its rows before the buffer is reused
     dft32<true>(v);
     // x W_N^(-16 k1 n2) = conj(W_1024^((32 / R1) k1 n2)), from the table
     {

@@ -249,6 +249,18 @@ struct EpiParams {
     int s = 0;
 };
 
+// The incoherent beam computed by the coherent beamformer itself from the
+// time rows it has staged (time-major input, Stokes I, 0/1 antenna weights,
+// tscrunch 4 ... 32): the integer sums of ib_tm_kernel, with its float-chain
+// fallback at 2^24, so the results are identical and the ring is read once.
+struct FusedIb {
+    float* raw = nullptr;  // [S][T/ts][F]; null: not fused (ib_raw is an input)
+    int8_t* out = nullptr;
+    const float* scale = nullptr;   // [S][F]
+    const float* offset = nullptr;  // [S][F]
+    const float* antw = nullptr;    // [S][A], 0 or 1
+};
+
 // TMI: the voltages come time-major from the coherent dedisperser's ring
 // (sample t of column c = p A + a of channel f at ftpa + ((f 2A + c) ld + w0
 // + t) 2) and are corner-turned through LDS into the FTPA rows the operands
@@ -259,7 +271,7 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
                int8_t* __restrict__ out, const float* __restrict__ scale,
                const float* __restrict__ offset, const int* __restrict__ bmap,
                const float* __restrict__ ib_raw, int nchans, int nsamples, int nbeams,
-               int ntchunks, int ib_sub, int nbeamsets, long ld, long w0)
+               int ntchunks, int ib_sub, int nbeamsets, long ld, long w0, FusedIb fib)
 {
     constexpr int NV = nstokes(S);
     constexpr int A  = NSTEPS * 16;  // antennas (K' = 2A = 32 * NSTEPS)
@@ -327,11 +339,29 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
                 tb_off[b] = ib_sub ? 0.0f : offset[sb * nchans + f];
             }
             const int o0 = tc * kMfmaWaves * 32 / TS;
-            for(int i = tid; i < nbeamsets * NO_WG * NV; i += kMfmaWaves * 64) {
+            for(int i = tid; !(TMI && fib.raw) && i < nbeamsets * NO_WG * NV; i += kMfmaWaves * 64) {
                 const int k = i % NV, o = (i / NV) % NO_WG, sb = i / (NV * NO_WG);
                 float v = 0.0f;
                 if(ib_sub && o0 + o < nsamps_out) v = ib_raw[(((size_t)sb * nsamps_out + o0 + o) * nchans + f) * NV + k];
                 tb_ib[i] = v;
+            }
+        }
+    }
+    // fused incoherent beam: this lane's antenna weights (0/1) as bit masks,
+    // bit 2 sb (+1) = antenna 2 wd mod A (+1) in beamset sb, loaded while the
+    // rows are staged
+    constexpr int NWD = (A + 63) / 64;
+    unsigned ibm[NWD];
+    if constexpr(TMI && S == SW_STOKES_I && kPre) {
+#pragma unroll
+        for(int k = 0; k < NWD; ++k) {
+            ibm[k] = 0;
+            const int wd = lane + 64 * k;
+            if(fib.raw && wd < A) {
+                const int a = (2 * wd) % A;
+                for(int sb = 0; sb < nbeamsets; ++sb)
+                    ibm[k] |= (fib.antw[sb * A + a] != 0.0f ? 1u : 0u) << (2 * sb) |
+                              (fib.antw[sb * A + a + 1] != 0.0f ? 2u : 0u) << (2 * sb);
             }
         }
     }
@@ -351,6 +381,66 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
                 *reinterpret_cast<uint16_t*>(stg + (8 * q + k) * SPITCH + c * 2) = (uint16_t)(w[k >> 1] >> (16 * (k & 1)));
         }
         __syncthreads();
+        if constexpr(S == SW_STOKES_I && kPre) {
+            if(fib.raw) {  // uniform: the incoherent beam from the staged rows
+                // wave: outputs o = wave, wave + 4, ...; lane: words w = lane,
+                // lane + 64 of a row (word w = columns 2w, 2w + 1: one pol,
+                // antennas 2w mod A and the next)
+                const int o0 = tc * NO_WG;
+#pragma unroll
+                for(int oi = 0; oi < NO_WG / kMfmaWaves; ++oi) {
+                    const int o = wave + kMfmaWaves * oi;
+                    int lo[NWD], hi[NWD];
+#pragma unroll
+                    for(int k = 0; k < NWD; ++k) {
+                        lo[k] = 0;
+                        hi[k] = 0;
+                        const int wd = lane + 64 * k;
+                        if(wd < A) {
+                            for(int tt = 0; tt < TS; ++tt) {
+                                const int v = *reinterpret_cast<const int*>(stg + (o * TS + tt) * SPITCH + wd * 4);
+                                lo[k] = __builtin_amdgcn_sdot4(v, v & 0xFFFF, lo[k], false);
+                                hi[k] = __builtin_amdgcn_sdot4(v, (int)((unsigned)v & 0xFFFF0000u), hi[k], false);
+                            }
+                        }
+                    }
+                    for(int sb = 0; sb < nbeamsets; ++sb) {
+                        int part = 0;
+#pragma unroll
+                        for(int k = 0; k < NWD; ++k) {
+                            if(ibm[k] >> (2 * sb) & 1u) part += lo[k];
+                            if(ibm[k] >> (2 * sb) & 2u) part += hi[k];
+                        }
+#pragma unroll
+                        for(int m = 32; m > 0; m >>= 1) part += __shfl_xor(part, m, 64);
+                        float power = (float)part;  // the float chain's value while below 2^24
+                        if(part >= (1 << 24)) {     // the oracle's float chain: time, then antenna ascending
+                            power = 0.0f;
+                            for(int tt = 0; tt < TS; ++tt) {
+                                const uint8_t* r = stg + (o * TS + tt) * SPITCH;
+                                for(int a = 0; a < A; ++a) {
+                                    const uint16_t d0 = *reinterpret_cast<const uint16_t*>(r + a * 2);
+                                    const uint16_t d1 = *reinterpret_cast<const uint16_t*>(r + (A + a) * 2);
+                                    const float st =
+                                        stokes<SW_STOKES_I>((float)(int8_t)d0, (float)(int8_t)(d0 >> 8),
+                                                            (float)(int8_t)d1, (float)(int8_t)(d1 >> 8));
+                                    power += st * fib.antw[sb * A + a];
+                                }
+                            }
+                        }
+                        if(lane == 0) {
+                            if(o0 + o < nsamps_out) {
+                                const size_t oi = ((size_t)sb * nsamps_out + o0 + o) * nchans + f;
+                                fib.raw[oi] = power;
+                                fib.out[oi] = clamp_i8(rescale<SW_STOKES_I>(power, fib.offset[sb * nchans + f],
+                                                                            fib.scale[sb * nchans + f]));
+                            }
+                            if(tabs) tb_ib[sb * NO_WG + o] = ib_sub ? power : 0.0f;
+                        }
+                    }
+                }
+            }
+        }
     }
     v4i va[2][NSTEPS];
 #pragma unroll
@@ -571,7 +661,7 @@ int launch_plain(const uint8_t* ftpa, const uint8_t* w, int8_t* out, const float
 template <int S, int TS, bool TMI>
 int launch_mfma_ts(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, const float* sc,
                    const float* off, const int* bmap, const float* ib, const sw_bf_dims& d,
-                   hipStream_t st, long ld, long w0)
+                   hipStream_t st, long ld, long w0, const FusedIb& fib)
 {
     const int ntchunks = (d.nsamples + 32 * kMfmaWaves - 1) / (32 * kMfmaWaves);
     dim3 grid(ntchunks * d.nchans);
@@ -579,7 +669,7 @@ int launch_mfma_ts(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, cons
     case NS:                                                                               \
         hipLaunchKernelGGL((cb_mfma_kernel<S, TS, NS, TMI>), grid, dim3(kMfmaWaves * 64), 0, \
                            st, ftpa, packed, out, sc, off, bmap, ib, d.nchans,             \
-                           d.nsamples, d.nbeams, ntchunks, d.ib_subtract, d.nbeamsets, ld, w0); \
+                           d.nsamples, d.nbeams, ntchunks, d.ib_subtract, d.nbeamsets, ld, w0, fib); \
         break;
     switch(d.nantennas / 16) {
         SW_MFMA(1) SW_MFMA(2) SW_MFMA(3) SW_MFMA(4) SW_MFMA(5) SW_MFMA(6) SW_MFMA(7) SW_MFMA(8)
@@ -595,15 +685,15 @@ int launch_mfma_ts(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, cons
 template <int S, bool TMI>
 int launch_mfma(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, const float* sc,
                 const float* off, const int* bmap, const float* ib, const sw_bf_dims& d,
-                hipStream_t st, long ld = 0, long w0 = 0)
+                hipStream_t st, long ld = 0, long w0 = 0, const FusedIb& fib = FusedIb{})
 {
     switch(d.tscrunch) {
-    case 1: return launch_mfma_ts<S, 1, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0);
-    case 2: return launch_mfma_ts<S, 2, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0);
-    case 4: return launch_mfma_ts<S, 4, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0);
-    case 8: return launch_mfma_ts<S, 8, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0);
-    case 16: return launch_mfma_ts<S, 16, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0);
-    case 32: return launch_mfma_ts<S, 32, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0);
+    case 1: return launch_mfma_ts<S, 1, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib);
+    case 2: return launch_mfma_ts<S, 2, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib);
+    case 4: return launch_mfma_ts<S, 4, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib);
+    case 8: return launch_mfma_ts<S, 8, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib);
+    case 16: return launch_mfma_ts<S, 16, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib);
+    case 32: return launch_mfma_ts<S, 32, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib);
     default:
         set_error("sw_cb_beamform: MFMA form supports tscrunch in {1, 2, 4, 8, 16, 32}");
         return SW_ERR_UNSUPPORTED;
@@ -629,6 +719,14 @@ int cb_mfma_supported(const sw_bf_dims& d)
     return d.nantennas % 16 == 0 && d.nantennas <= 128 && d.nsamples % 32 == 0 &&
            (d.tscrunch == 1 || d.tscrunch == 2 || d.tscrunch == 4 || d.tscrunch == 8 ||
             d.tscrunch == 16 || d.tscrunch == 32);
+}
+
+// the incoherent beam can be fused into the time-major MFMA form: Stokes I,
+// epilogue tables in LDS (kBeamLds beams, 16 beamsets), tscrunch 4 ... 32
+int cb_fused_ib_supported(const sw_bf_dims& d)
+{
+    return cb_mfma_supported(d) && d.stokes == SW_STOKES_I && d.nbeams <= kBeamLds && d.nbeamsets <= 16 &&
+           d.tscrunch >= 4;
 }
 
 } // namespace sw
@@ -660,9 +758,13 @@ extern "C" int sw_cb_pack_weights(const void* d_weights, void* d_packed, int nch
 namespace sw {
 // The MFMA form on the coherent dedisperser's time-major ring (Stokes I):
 // sample t of column (f, p, a) at d_ring + ((f 2A + p A + a) ld + w0 + t) 2.
+// With d_ib_out_raw non-null the incoherent beam is computed here too (see
+// FusedIb; requires cb_fused_ib_supported) and written to d_ib_out_raw /
+// d_ib_out, and d_ib_raw is not read.
 int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_packed, const float* d_scale,
                           const float* d_offset, const int* d_beamset_map, const float* d_ib_raw, void* d_out,
-                          const sw_bf_dims& d, hipStream_t st)
+                          const sw_bf_dims& d, hipStream_t st, float* d_ib_out_raw, void* d_ib_out,
+                          const float* d_ib_scale, const float* d_ib_offset, const float* d_ant_weights)
 {
     int rc = check_dims(d);
     if(rc) return rc;
@@ -675,9 +777,15 @@ int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_pa
         set_error("cb_beamform_packed_tm: shape not supported by the MFMA form");
         return SW_ERR_UNSUPPORTED;
     }
+    FusedIb fib;
+    if(d_ib_out_raw) {
+        SW_CHECK_ARG(cb_fused_ib_supported(d), "fused incoherent beam: unsupported shape");
+        SW_CHECK_ARG(d_ib_out && d_ib_scale && d_ib_offset && d_ant_weights, "null buffer");
+        fib = FusedIb{d_ib_out_raw, static_cast<int8_t*>(d_ib_out), d_ib_scale, d_ib_offset, d_ant_weights};
+    }
     return launch_mfma<SW_STOKES_I, true>(static_cast<const uint8_t*>(d_ring), static_cast<const uint8_t*>(d_packed),
                                           static_cast<int8_t*>(d_out), d_scale, d_offset, d_beamset_map, d_ib_raw, d,
-                                          st, ld, w0);
+                                          st, ld, w0, fib);
 }
 } // namespace sw
 

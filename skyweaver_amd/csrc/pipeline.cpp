@@ -32,7 +32,9 @@ int ib_beamform_tm(const void* d_ring, long ld, long w0, float* d_raw, void* d_o
                    const float* d_offset, const float* d_ant_weights, const sw_bf_dims& d, hipStream_t st);
 int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_packed, const float* d_scale,
                           const float* d_offset, const int* d_beamset_map, const float* d_ib_raw, void* d_out,
-                          const sw_bf_dims& d, hipStream_t st);
+                          const sw_bf_dims& d, hipStream_t st, float* d_ib_out_raw, void* d_ib_out,
+                          const float* d_ib_scale, const float* d_ib_offset, const float* d_ant_weights);
+int cb_fused_ib_supported(const sw_bf_dims& d);
 }
 
 struct sw_pipeline {
@@ -560,6 +562,7 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
     // output ring directly, with no corner turn to FTPA
     bool ring_direct = p->use_mfma && c.stokes == SW_STOKES_I && 2 * A <= 256 && c.nsamples % 8 == 0;
     for(float w : p->h_antw) ring_direct = ring_direct && (w == 0.0f || w == 1.0f);
+    const bool fuse_ib = ring_direct && sw::cb_fused_ib_supported(d);
     std::vector<void*> outs(p->ftpa_dd.begin(), p->ftpa_dd.end());
     if(ring_direct) std::fill(outs.begin(), outs.end(), nullptr);
     rc = sw_codedisp_run(p->cd, p->ftpa, outs.data(), nullptr, 0, c.ndms, c.batched_dms, st);
@@ -580,7 +583,9 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
         if(ring_direct) {
             rc = sw_codedisp_window(p->cd, dm, &ring, &ld, &w0);
             if(rc) return rc;
-            rc = sw::ib_beamform_tm(ring, ld, w0, p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, d, st);
+            // with the incoherent beam fused into the coherent beamformer
+            // (which then reads the ring once) there is nothing to do here
+            if(!fuse_ib) rc = sw::ib_beamform_tm(ring, ld, w0, p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, d, st);
         } else {
             rc = sw_ib_beamform(p->ftpa_dd[dm], p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, &d, st);
         }
@@ -588,7 +593,8 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
         SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm + 1], st));
         if(ring_direct)
             rc = sw::cb_beamform_packed_tm(ring, ld, w0, p->packed, cb_scaling, cb_offsets, p->bmap, p->ib_raw,
-                                           p->cb_out, d, st);
+                                           p->cb_out, d, st, fuse_ib ? p->ib_raw : nullptr, p->ib_q, ib_scaling,
+                                           ib_offsets, p->antw);
         else if(p->use_mfma)
             rc = sw_cb_beamform_packed(p->ftpa_dd[dm], p->packed, cb_scaling, cb_offsets, p->bmap,
                                        p->ib_raw, p->cb_out, &d, st);

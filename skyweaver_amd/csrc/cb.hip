@@ -347,20 +347,21 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
             }
         }
     }
-    // fused incoherent beam: this lane's antenna weights (0/1) as bit masks,
-    // bit 2 sb (+1) = antenna 2 wd mod A (+1) in beamset sb, loaded while the
-    // rows are staged
-    constexpr int NWD = (A + 63) / 64;
-    unsigned ibm[NWD];
-    if constexpr(TMI && S == SW_STOKES_I && kPre) {
+    // fused incoherent beam: lane l reads 16-byte runs (words 4 g .. 4 g + 3,
+    // g = l mod LPR, word w = columns 2w, 2w + 1: one pol, antennas 2w mod A
+    // and the next) of rows l / LPR + RPI i; its antenna weights (0/1) as bit
+    // masks, bit 2 sb (+1) of ibm[j] = word 4 g + j's first (second) antenna
+    // in beamset sb, loaded while the rows are staged
+    constexpr int LPR = A / 4 > 0 ? A / 4 : 1;  // lanes per row
+    constexpr int RPI = 64 / LPR;               // rows per read
+    unsigned ibm[4] = {0, 0, 0, 0};
+    if constexpr(TMI && S == SW_STOKES_I && kPre && 256 % A == 0 && TS >= RPI) {
+        if(fib.raw) {
 #pragma unroll
-        for(int k = 0; k < NWD; ++k) {
-            ibm[k] = 0;
-            const int wd = lane + 64 * k;
-            if(fib.raw && wd < A) {
-                const int a = (2 * wd) % A;
+            for(int j = 0; j < 4; ++j) {
+                const int a = (2 * (4 * (lane % LPR) + j)) % A;
                 for(int sb = 0; sb < nbeamsets; ++sb)
-                    ibm[k] |= (fib.antw[sb * A + a] != 0.0f ? 1u : 0u) << (2 * sb) |
+                    ibm[j] |= (fib.antw[sb * A + a] != 0.0f ? 1u : 0u) << (2 * sb) |
                               (fib.antw[sb * A + a + 1] != 0.0f ? 2u : 0u) << (2 * sb);
             }
         }
@@ -381,35 +382,30 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
                 *reinterpret_cast<uint16_t*>(stg + (8 * q + k) * SPITCH + c * 2) = (uint16_t)(w[k >> 1] >> (16 * (k & 1)));
         }
         __syncthreads();
-        if constexpr(S == SW_STOKES_I && kPre) {
+        if constexpr(S == SW_STOKES_I && kPre && 256 % A == 0 && TS >= RPI) {
             if(fib.raw) {  // uniform: the incoherent beam from the staged rows
-                // wave: outputs o = wave, wave + 4, ...; lane: words w = lane,
-                // lane + 64 of a row (word w = columns 2w, 2w + 1: one pol,
-                // antennas 2w mod A and the next)
                 const int o0 = tc * NO_WG;
+                const int r0 = lane / LPR, g = lane % LPR;
 #pragma unroll
                 for(int oi = 0; oi < NO_WG / kMfmaWaves; ++oi) {
-                    const int o = wave + kMfmaWaves * oi;
-                    int lo[NWD], hi[NWD];
+                    const int o = wave * (NO_WG / kMfmaWaves) + oi;  // this wave's own rows
+                    int lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
 #pragma unroll
-                    for(int k = 0; k < NWD; ++k) {
-                        lo[k] = 0;
-                        hi[k] = 0;
-                        const int wd = lane + 64 * k;
-                        if(wd < A) {
-                            for(int tt = 0; tt < TS; ++tt) {
-                                const int v = *reinterpret_cast<const int*>(stg + (o * TS + tt) * SPITCH + wd * 4);
-                                lo[k] = __builtin_amdgcn_sdot4(v, v & 0xFFFF, lo[k], false);
-                                hi[k] = __builtin_amdgcn_sdot4(v, (int)((unsigned)v & 0xFFFF0000u), hi[k], false);
-                            }
+                    for(int i = 0; i < TS / RPI; ++i) {
+                        const uint4 v = *reinterpret_cast<const uint4*>(stg + (o * TS + RPI * i + r0) * SPITCH + 16 * g);
+                        const int w[4] = {(int)v.x, (int)v.y, (int)v.z, (int)v.w};
+#pragma unroll
+                        for(int j = 0; j < 4; ++j) {
+                            lo[j] = __builtin_amdgcn_sdot4(w[j], w[j] & 0xFFFF, lo[j], false);
+                            hi[j] = __builtin_amdgcn_sdot4(w[j], (int)((unsigned)w[j] & 0xFFFF0000u), hi[j], false);
                         }
                     }
                     for(int sb = 0; sb < nbeamsets; ++sb) {
                         int part = 0;
 #pragma unroll
-                        for(int k = 0; k < NWD; ++k) {
-                            if(ibm[k] >> (2 * sb) & 1u) part += lo[k];
-                            if(ibm[k] >> (2 * sb) & 2u) part += hi[k];
+                        for(int j = 0; j < 4; ++j) {
+                            if(ibm[j] >> (2 * sb) & 1u) part += lo[j];
+                            if(ibm[j] >> (2 * sb) & 2u) part += hi[j];
                         }
 #pragma unroll
                         for(int m = 32; m > 0; m >>= 1) part += __shfl_xor(part, m, 64);
@@ -430,10 +426,10 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
                         }
                         if(lane == 0) {
                             if(o0 + o < nsamps_out) {
-                                const size_t oi = ((size_t)sb * nsamps_out + o0 + o) * nchans + f;
-                                fib.raw[oi] = power;
-                                fib.out[oi] = clamp_i8(rescale<SW_STOKES_I>(power, fib.offset[sb * nchans + f],
-                                                                            fib.scale[sb * nchans + f]));
+                                const size_t oi2 = ((size_t)sb * nsamps_out + o0 + o) * nchans + f;
+                                fib.raw[oi2] = power;
+                                fib.out[oi2] = clamp_i8(rescale<SW_STOKES_I>(power, fib.offset[sb * nchans + f],
+                                                                             fib.scale[sb * nchans + f]));
                             }
                             if(tabs) tb_ib[sb * NO_WG + o] = ib_sub ? power : 0.0f;
                         }
@@ -722,11 +718,13 @@ int cb_mfma_supported(const sw_bf_dims& d)
 }
 
 // the incoherent beam can be fused into the time-major MFMA form: Stokes I,
-// epilogue tables in LDS (kBeamLds beams, 16 beamsets), tscrunch 4 ... 32
+// epilogue tables in LDS (kBeamLds beams, 16 beamsets), tscrunch 4 ... 32,
+// 16, 32, 64 or 128 antennas
 int cb_fused_ib_supported(const sw_bf_dims& d)
 {
+    // 16-byte row reads: A divides 256 and a read's 256 / A rows fit an output
     return cb_mfma_supported(d) && d.stokes == SW_STOKES_I && d.nbeams <= kBeamLds && d.nbeamsets <= 16 &&
-           d.tscrunch >= 4;
+           d.tscrunch >= 4 && 256 % d.nantennas == 0 && d.tscrunch * d.nantennas >= 256;
 }
 
 } // namespace sw

@@ -43,7 +43,7 @@ VALU_ADD16_PEAK = 78.6e12   # 16-bit lane adds/s: 256 CUs x 64 lanes x 2 lanes p
 # HBM bytes per cb_mfma_kernel launch at this workload, from rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE calibrated on a 1 GiB copy kernel
 # (profiles/hbm_traffic_pipeline_64ant_64ch_65536samp_800beam_dm50.md)
-CB_TRAFFIC_BYTES = 1.331e9
+CB_TRAFFIC_BYTES = 1.314e9   # HBM bytes per cb_mfma_kernel launch at this workload (FETCH_SIZE + WRITE_SIZE counters)
 
 # incoherent dedispersion workload: 256 channels (4 x 64 over 53.5 MHz at
 # 1284 MHz), 512 beams, CB tscrunch 8, 1024 DM trials 0:1000:0.9765625, one

@@ -269,8 +269,8 @@ __device__ __forceinline__ P zero_piece()
 // time blocks; DM tiles of neighbouring delays), which its L2 then serves.
 constexpr int kIdTbGroup = 8, kIdTileGroup = 4;
 
-template <int NW, int DPW, int NTT, int RCAP, bool VEC16>
-__global__ void __launch_bounds__(NW * 64)
+template <int NW, int DPW, int NTT, int RCAP, bool VEC16, bool PF>
+__global__ void __launch_bounds__(NW * 64, PF ? 1 : 2)
 idisp_mfma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const int* __restrict__ delays,
                   const int2* __restrict__ span, const int2* __restrict__ tiles, int ntiles, int nchunks,
                   int ntb, long w0, long nwork, int nchans, int W, int ndms, int ts, int max_delay,
@@ -370,6 +370,20 @@ idisp_mfma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const
         }
     };
 
+    if constexpr(!PF) {
+        // no register prefetch (two workgroups share a CU and hide each
+        // other's loads): stage quad q, then its products
+        for(int q = 0; q < nq; ++q) {
+            __syncthreads();  // the previous quad's reads are done
+            fetch(q, preA);
+            stash(preA);
+            int2 spc;
+            int dc[DPW];
+            lane_delays(q, spc, dc);
+            __syncthreads();
+            products(spc, dc);
+        }
+    } else {
     // quads alternate between two register sets: A holds even quads, B odd
     int2 spA, spB;
     int dA[DPW], dB[DPW];
@@ -408,6 +422,7 @@ idisp_mfma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const
             }
             products(spc, dc);
         }
+    }
     }
 
     // C lane (n, g), register r: beam b0 + 16 bg + 4 g + r at sample t0 + 16 tt + n
@@ -557,7 +572,9 @@ extern "C" int sw_idisp_create(const sw_idisp_config* cfg, sw_idisp** out)
     int cap = sw::kRowsCap, dmax_tile = 8;
     p->mfma = cfg->kernel == 0 && p->nchans % 4 == 0 && 16 * p->tscrunch <= sw::kMfRows;
     if(p->mfma) {
-        p->tt = 32 * p->tscrunch <= sw::kMfRows / 2 ? 32 : 16;
+        // 16 output samples per workgroup, no register prefetch: at most
+        // 128 VGPRs, so two workgroups share a CU and hide each other's loads
+        p->tt = 16;
         cap = sw::kMfRows;
         dmax_tile = sw::kMfWaves * sw::kMfDpw;
     } else {
@@ -687,7 +704,7 @@ int sw::idisp_launch(sw_idisp* p, const void* d_tfb, size_t nsamples, int nbeams
             const long nw = std::min(max_launch, nwork - w0);
             const unsigned nblk = (unsigned)(8 * ((nw + 7) / 8));
 #define SW_IDM_LAUNCH(NTT, V16)                                                                              \
-    hipLaunchKernelGGL((sw::idisp_mfma_kernel<sw::kMfWaves, sw::kMfDpw, NTT, sw::kMfRows, V16>), dim3(nblk), \
+    hipLaunchKernelGGL((sw::idisp_mfma_kernel<sw::kMfWaves, sw::kMfDpw, NTT, sw::kMfRows, V16, NTT == 2>), dim3(nblk), \
                        dim3(sw::kMfWaves * 64), 0, st, static_cast<const int8_t*>(d_tfb),                   \
                        static_cast<int8_t*>(d_tdb), p->d_delays, p->d_span, p->d_tiles, (int)ntiles,          \
                        (int)nchunks128, (int)ntb128, w0, nw, p->nchans, (int)nbv, p->ndms, p->tscrunch, p->max_delay,      \

@@ -493,7 +493,10 @@ int sw_pipeline_synchronize(sw_pipeline* pipeline);
 /* Per-stage device times of the last gulp (ms), in the order transpose,
  * statistics, dedispersion, incoherent beam, coherent beam, incoherent
  * dedispersion (with its aggregation), weights (delay-model path), other;
- * n <= SW_NSTAGES entries are written. */
+ * n <= SW_NSTAGES entries are written.  Fused work is timed with the kernel
+ * that does it: the statistics' power sums with the transpose and, for
+ * Stokes I with 0/1 antenna weights, the incoherent beam with the coherent
+ * beams (the statistics and incoherent-beam stages then read about 0). */
 #define SW_NSTAGES 8
 int sw_pipeline_stage_times(const sw_pipeline* pipeline, float* ms, int n);
 /* The processing stream (hipStream_t) the pipeline enqueues work on. */

@@ -15,6 +15,12 @@
 // stages are enqueued back to back on one stream with no intermediate host
 // synchronisation (only when a sink must read a block, and at the end of
 // the gulp), and the scalings are indexed per beamset (beamset*F + f).
+// Kernels are fused where the data allow: the statistics' power sums are
+// taken by the transpose, and for Stokes I with 0/1 antenna weights the
+// beamformers read the dedisperser's time-major ring directly, the
+// incoherent beam being computed inside the coherent beamformer.  A host
+// gulp can be copied on a second stream while the previous one is processed
+// (sw_pipeline_prefetch / sw_pipeline_process_prefetched).
 #include "sw_common.hpp"
 
 #include <algorithm>
@@ -558,8 +564,9 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
     float* ib_scaling = p->scl + 3 * S * F;
 
     // Stokes I with 0/1 antenna weights (what delay models give): the MFMA
-    // beamformer and the incoherent beam read the dedisperser's time-major
-    // output ring directly, with no corner turn to FTPA
+    // beamformer (and the incoherent beam, inside it where supported, else
+    // its own kernel) reads the dedisperser's time-major output ring
+    // directly, with no corner turn to FTPA
     bool ring_direct = p->use_mfma && c.stokes == SW_STOKES_I && 2 * A <= 256 && c.nsamples % 8 == 0;
     for(float w : p->h_antw) ring_direct = ring_direct && (w == 0.0f || w == 1.0f);
     const bool fuse_ib = ring_direct && sw::cb_fused_ib_supported(d);

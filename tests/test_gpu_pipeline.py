@@ -359,3 +359,47 @@ def test_pipeline_full_size_8dm_forms_identical(sw, dev):
             assert x[0] == y[0]
             np.testing.assert_array_equal(x[1], y[1])
     assert np.unique(a["cb"][-1][1]).size > 100
+
+
+@pytest.mark.parametrize("nant,ts", [(64, 4), (32, 8), (128, 32), (16, 16), (48, 16), (64, 2)])
+def test_pipeline_fused_incoherent_beam_shapes(sw, dev, nant, ts):
+    """The incoherent beam computed inside the coherent beamformer (antenna
+    counts dividing 256, tscrunch * antennas >= 256) and, for the other
+    shapes here (48 antennas; tscrunch 2), by its own kernel on the ring:
+    both equal the plain form's incoherent beam on FTPA, for two beamsets
+    over uneven antenna subsets and a DM with latency."""
+    from skyweaver_amd import DeviceBuffer as DB
+    Fq, Tq, Bq = 4, 4096, 96
+    bw = 13.375e6 * Fq / 64
+    H = Tq // 256
+    g = [DB.from_host(sw.synth_voltages((H, nant, Fq, 256, 2, 2), seed=100 + nant + ts + i, sigma=24.0))
+         for i in range(2)]
+    weights = sw.synth_weights((Fq, Bq, nant, 2), seed=200 + nant)
+    bmap = (np.arange(Bq) % 2).astype(np.int32)
+    antw = np.ones((2, nant), np.float32)
+    antw[1, nant // 3:] = 0
+    weights[:, 1::2, nant // 3:] = 0
+
+    def run(form, batched):
+        p = sw.BeamformerPipeline(nant, Fq, Bq, Tq, ts, [0.0, 30.0], 1284e6, bw, fft_len=[1024, 2048], form=form,
+                                  batched=batched)
+        p.set_weights(weights)
+        p.set_beamsets(bmap, antw)
+        p.init(nant)
+        got = {"cb": [], "ib": []}
+
+        def grab(key):
+            def f(blk):
+                got[key].append(sw.download_block(blk))
+            return f
+        p.set_sinks(cb=grab("cb"), ib=grab("ib"))
+        for x in g:
+            p(x)
+        p.close()
+        return got
+
+    a, b = run("mfma", True), run("plain", False)
+    for key in ("cb", "ib"):
+        assert len(a[key]) == len(b[key]) == 4
+        for x, y in zip(a[key], b[key]):
+            np.testing.assert_array_equal(x, y)

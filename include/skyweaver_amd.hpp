@@ -730,13 +730,34 @@ class MultiFileWriter {
   public:
     MultiFileWriter(PipelineConfig const& config, std::string tag = "") : _config(config), _tag(std::move(tag)) {}
     void init(ObservationHeader const& header) { _header = header; }
+    MultiFileWriter(MultiFileWriter const&) = delete;
+    MultiFileWriter& operator=(MultiFileWriter const&) = delete;
+    ~MultiFileWriter()
+    {
+        if(_pinned) sw_host_free(_pinned);
+    }
     bool operator()(BlockView const& block, std::size_t stream_idx)
     {
         auto it = _streams.find(stream_idx);
         if(it == _streams.end()) it = _streams.emplace(stream_idx, create_stream(block, stream_idx)).first;
-        _buf.resize(block.size());
-        block.copy_to_host(_buf.data());
-        it->second->write(reinterpret_cast<char const*>(_buf.data()), _buf.size());
+        const std::size_t n = block.size();
+        if(block.on_device()) {
+            // device blocks come out through one reusable pinned buffer: the
+            // copy runs at DMA rate instead of through a pageable staging copy
+            if(n > _pinned_size) {
+                if(_pinned) sw_host_free(_pinned);
+                _pinned = nullptr;
+                _pinned_size = 0;
+                check(sw_host_alloc(&_pinned, n));
+                _pinned_size = n;
+            }
+            block.copy_to_host(_pinned);
+            it->second->write(static_cast<char const*>(_pinned), n);
+        } else {
+            _buf.resize(n);
+            block.copy_to_host(_buf.data());
+            it->second->write(reinterpret_cast<char const*>(_buf.data()), n);
+        }
         return false;
     }
     FileStream const& stream(std::size_t idx) const { return *_streams.at(idx); }
@@ -813,6 +834,8 @@ class MultiFileWriter {
     ObservationHeader _header;
     std::map<std::size_t, std::unique_ptr<FileStream>> _streams;
     std::vector<int8_t> _buf;
+    void* _pinned = nullptr;
+    std::size_t _pinned_size = 0;
 };
 
 // A handler that accepts and discards blocks (for unused outputs).

@@ -332,11 +332,30 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     const bool tabs = kPre && nbeams <= kBeamLds && nbeamsets <= 16;  // uniform
     if constexpr(kPre) {
         if(tabs) {
-            for(int b = tid; b < nbeams; b += kMfmaWaves * 64) {
-                const int sb = bmap[b];
-                tb_s[b]   = sb;
-                tb_sc[b]  = scale[sb * nchans + f];
-                tb_off[b] = ib_sub ? 0.0f : offset[sb * nchans + f];
+            // all of a thread's loads in flight together (beam map first,
+            // then the scales and offsets it selects)
+            constexpr int NB = kBeamLds / (kMfmaWaves * 64);
+            int sbv[NB];
+#pragma unroll
+            for(int k = 0; k < NB; ++k) {
+                const int b = tid + k * kMfmaWaves * 64;
+                sbv[k] = b < nbeams ? bmap[b] : 0;
+            }
+            float scv[NB], ofv[NB];
+#pragma unroll
+            for(int k = 0; k < NB; ++k) {
+                const int b = tid + k * kMfmaWaves * 64;
+                scv[k] = b < nbeams ? scale[sbv[k] * nchans + f] : 0.0f;
+                ofv[k] = (b < nbeams && !ib_sub) ? offset[sbv[k] * nchans + f] : 0.0f;
+            }
+#pragma unroll
+            for(int k = 0; k < NB; ++k) {
+                const int b = tid + k * kMfmaWaves * 64;
+                if(b < nbeams) {
+                    tb_s[b]   = sbv[k];
+                    tb_sc[b]  = scv[k];
+                    tb_off[b] = ofv[k];
+                }
             }
             const int o0 = tc * kMfmaWaves * 32 / TS;
             for(int i = tid; !(TMI && fib.raw) && i < nbeamsets * NO_WG * NV; i += kMfmaWaves * 64) {
@@ -368,18 +387,28 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     }
     // ---- A operands: 32 time rows x K' for both polarisations
     if constexpr(TMI) {
-        // corner turn: 16-byte runs of 8 samples of each column into rows
+        // corner turn: 16-byte runs of 8 samples of each column into rows;
+        // a thread's loads are all issued before its LDS writes
         const int t0w = tc * kMfmaWaves * 32;
         uint8_t* stg = reinterpret_cast<uint8_t*>(smem);
-        for(int i = tid; i < 2 * A * 4 * kMfmaWaves; i += kMfmaWaves * 64) {
-            const int c = i % (2 * A), q = i / (2 * A);
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if(t0w + 8 * q < nsamples)
-                v = *reinterpret_cast<const uint4*>(ftpa + (((size_t)f * 2 * A + c) * ld + w0 + t0w + 8 * q) * 2);
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        constexpr int NST = 2 * A * 4 * kMfmaWaves / (kMfmaWaves * 64);  // runs per thread
+        uint4 run[NST];
 #pragma unroll
-            for(int k = 0; k < 8; ++k)
-                *reinterpret_cast<uint16_t*>(stg + (8 * q + k) * SPITCH + c * 2) = (uint16_t)(w[k >> 1] >> (16 * (k & 1)));
+        for(int k = 0; k < NST; ++k) {
+            const int i = tid + k * kMfmaWaves * 64;
+            const int c = i % (2 * A), q = i / (2 * A);
+            run[k] = make_uint4(0, 0, 0, 0);
+            if(t0w + 8 * q < nsamples)
+                run[k] = *reinterpret_cast<const uint4*>(ftpa + (((size_t)f * 2 * A + c) * ld + w0 + t0w + 8 * q) * 2);
+        }
+#pragma unroll
+        for(int k = 0; k < NST; ++k) {
+            const int i = tid + k * kMfmaWaves * 64;
+            const int c = i % (2 * A), q = i / (2 * A);
+            const uint32_t w[4] = {run[k].x, run[k].y, run[k].z, run[k].w};
+#pragma unroll
+            for(int j = 0; j < 8; ++j)
+                *reinterpret_cast<uint16_t*>(stg + (8 * q + j) * SPITCH + c * 2) = (uint16_t)(w[j >> 1] >> (16 * (j & 1)));
         }
         __syncthreads();
         if constexpr(S == SW_STOKES_I && kPre && 256 % A == 0 && TS >= RPI) {

@@ -374,12 +374,12 @@ idisp_mfma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const
         // no register prefetch (two workgroups share a CU and hide each
         // other's loads): stage quad q, then its products
         for(int q = 0; q < nq; ++q) {
-            __syncthreads();  // the previous quad's reads are done
-            fetch(q, preA);
-            stash(preA);
             int2 spc;
             int dc[DPW];
-            lane_delays(q, spc, dc);
+            lane_delays(q, spc, dc);  // issued with the row loads, before any wait
+            fetch(q, preA);
+            __syncthreads();  // the previous quad's reads are done
+            stash(preA);
             __syncthreads();
             products(spc, dc);
         }

@@ -330,6 +330,21 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
 
     const int nsamps_out = nsamples / TS;
     const bool tabs = kPre && nbeams <= kBeamLds && nbeamsets <= 16;  // uniform
+    // TMI: the workgroup's time rows, 16-byte runs of 8 samples of each
+    // column, issued first so that their latency overlaps the table loads
+    constexpr int NST = TMI ? 2 * A * 4 * kMfmaWaves / (kMfmaWaves * 64) : 1;  // runs per thread
+    uint4 run[NST];
+    if constexpr(TMI) {
+        const int t0w = tc * kMfmaWaves * 32;
+#pragma unroll
+        for(int k = 0; k < NST; ++k) {
+            const int i = tid + k * kMfmaWaves * 64;
+            const int c = i % (2 * A), q = i / (2 * A);
+            run[k] = make_uint4(0, 0, 0, 0);
+            if(t0w + 8 * q < nsamples)
+                run[k] = *reinterpret_cast<const uint4*>(ftpa + (((size_t)f * 2 * A + c) * ld + w0 + t0w + 8 * q) * 2);
+        }
+    }
     if constexpr(kPre) {
         if(tabs) {
             // all of a thread's loads in flight together (beam map first,
@@ -387,20 +402,9 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     }
     // ---- A operands: 32 time rows x K' for both polarisations
     if constexpr(TMI) {
-        // corner turn: 16-byte runs of 8 samples of each column into rows;
-        // a thread's loads are all issued before its LDS writes
+        // corner turn of the runs loaded above into rows
         const int t0w = tc * kMfmaWaves * 32;
         uint8_t* stg = reinterpret_cast<uint8_t*>(smem);
-        constexpr int NST = 2 * A * 4 * kMfmaWaves / (kMfmaWaves * 64);  // runs per thread
-        uint4 run[NST];
-#pragma unroll
-        for(int k = 0; k < NST; ++k) {
-            const int i = tid + k * kMfmaWaves * 64;
-            const int c = i % (2 * A), q = i / (2 * A);
-            run[k] = make_uint4(0, 0, 0, 0);
-            if(t0w + 8 * q < nsamples)
-                run[k] = *reinterpret_cast<const uint4*>(ftpa + (((size_t)f * 2 * A + c) * ld + w0 + t0w + 8 * q) * 2);
-        }
 #pragma unroll
         for(int k = 0; k < NST; ++k) {
             const int i = tid + k * kMfmaWaves * 64;

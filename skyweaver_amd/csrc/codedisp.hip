@@ -83,9 +83,14 @@ __device__ __forceinline__ float2 load_c8(const int8_t* p)
 
 __device__ __forceinline__ void store_c8(int8_t* p, float2 y)
 {
-    // __float2int_rn then static_cast<char>: round half even, wrap modulo 2^8
-    const int re = (int)rintf(y.x), im = (int)rintf(y.y);
-    *reinterpret_cast<uint16_t*>(p) = (uint16_t)((re & 0xff) | ((im & 0xff) << 8));
+    // __float2int_rn then static_cast<char>: round half even, wrap modulo 2^8.
+    // Adding 1.5 * 2^23 rounds to an integer (the FP32 add rounds to nearest
+    // even; 1.5 * 2^23 is even) and leaves it, two's complement, in the low
+    // mantissa bits, exactly for |y| < 2^22.  A dedispersed sample is a
+    // unit-gain filter of char2 input: |y| <= 128 * sqrt(2 N), < 2^22 for any
+    // N below 2^29.
+    const uint32_t re = __float_as_uint(y.x + 12582912.0f), im = __float_as_uint(y.y + 12582912.0f);
+    *reinterpret_cast<uint16_t*>(p) = (uint16_t)((re & 0xffu) | ((im & 0xffu) << 8));
 }
 
 // input sample tau (relative to the gulp start) of column col of channel f

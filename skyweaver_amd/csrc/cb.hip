@@ -403,7 +403,6 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     // ---- A operands: 32 time rows x K' for both polarisations
     if constexpr(TMI) {
         // corner turn of the runs loaded above into rows
-        const int t0w = tc * kMfmaWaves * 32;
         uint8_t* stg = reinterpret_cast<uint8_t*>(smem);
 #pragma unroll
         for(int k = 0; k < NST; ++k) {

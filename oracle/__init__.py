@@ -25,8 +25,9 @@ STOKES = {"I": 0, "Q": 1, "U": 2, "V": 3, "IQUV": 4}
 
 
 def build() -> str:
-    """Compile the oracle shared library (gcc, -ffp-contract=off)."""
-    subprocess.run(["make", "-s", "-C", _HERE, "CC=gcc"], check=True)
+    """Compile the oracle shared library (gcc, -ffp-contract=off), always from
+    the sources."""
+    subprocess.run(["make", "-B", "-s", "-C", _HERE, "CC=gcc"], check=True)
     return _LIB_PATH
 
 

@@ -31,9 +31,11 @@ class SkyweaverError(RuntimeError):
         self.code = code
 
 
-def build(jobs: int = 8) -> str:
-    """Compile the HIP library for gfx950 (hipcc, in-tree)."""
-    subprocess.run(["make", "-s", "-C", _HERE, f"-j{jobs}"], check=True)
+def build(jobs: int = 8, force: bool = True) -> str:
+    """Compile the HIP library for gfx950 (hipcc, in-tree); by default every
+    source is compiled again, also where a built library exists."""
+    cmd = ["make", "-s", "-C", _HERE, f"-j{jobs}"] + (["-B"] if force else [])
+    subprocess.run(cmd, check=True)
     return LIB_PATH
 
 

@@ -39,6 +39,7 @@ TSAMP = 4096 / 856e6
 SEED_V, SEED_W = 0x5EED0002, 0x5EED1002
 INT8_PEAK_TOPS = 5000.0     # MI355X dense int8 MFMA
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 (packed); the unpacked VALU rate is half of it
 VALU_ADD16_PEAK = 78.6e12   # 16-bit lane adds/s: 256 CUs x 64 lanes x 2 lanes per 32-bit add x 2.4 GHz
 # HBM bytes per cb_mfma_kernel launch at this workload, from rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE calibrated on a 1 GiB copy kernel
@@ -296,6 +297,24 @@ def main():
                 "traffic": CB_TRAFFIC_BYTES, "ops_per_launch": ops, "avg_launch_ms": round(cb_ms, 4),
                 "note": "int8 operations (1 MAC = 2 ops); peak is the dense int8 MFMA rate; the launch also "
                         "computes the incoherent beam"}
+    # the other large kernel, the coherent dedisperser (FP32 VALU): per column
+    # and segment a forward and an inverse N-point transform and the chirp,
+    # 10 N log2 N + 6 N flops; segments per gulp T / (N - O) on the stream's
+    # segment grid.  The overlap O comes from a one-channel plan of the band.
+    N = FFT_LEN[0]
+    cd_plan = sw.CoherentDedisperser(1, 1, T, CFREQ - BW / 2, BW / F, TSAMP, DMS, fft_len=FFT_LEN)
+    try:
+        _, O = cd_plan.info(0)
+    finally:
+        cd_plan.close()
+    cd_flops = F * 2 * A * (T / (N - O)) * (10 * N * np.log2(N) + 6 * N)
+    cd_ms = stage["dedispersion"]
+    cd_tf = cd_flops / (cd_ms * 1e-3) / 1e12
+    roofline_dd = {"kernel": "codedisp_reg_kernel<32>", "bound": "valu", "achieved": round(cd_tf, 1),
+                   "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(cd_tf / FP32_PEAK_TFLOPS, 4),
+                   "flops_per_launch": cd_flops, "avg_launch_ms": round(cd_ms, 4), "fft_len": N, "overlap": O,
+                   "note": "FP32 FFT flops (10 N log2 N + 6 N per column and segment); the peak counts packed "
+                           "FP32, which this build does not use (unpacked rate 78.6 TFLOP/s)"}
     hbm = {}
     # the transpose stage is taftp_to_fpat_stats_kernel: the statistics' power
     # sums are taken in the same pass (one read of the gulp, one write), and
@@ -369,6 +388,7 @@ def main():
         "wall_ms_per_step": round(wall, 4),
         "stage_ms": {k: round(v, 4) for k, v in stage.items()},
         "roofline": roofline,
+        "roofline_dedispersion": roofline_dd,
         "hbm_kernels": hbm,
         "plain": plain,
         "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,

@@ -41,10 +41,12 @@ INT8_PEAK_TOPS = 5000.0     # MI355X dense int8 MFMA
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 (packed); the unpacked VALU rate is half of it
 VALU_ADD16_PEAK = 78.6e12   # 16-bit lane adds/s: 256 CUs x 64 lanes x 2 lanes per 32-bit add x 2.4 GHz
-# HBM bytes per cb_mfma_kernel launch at this workload, from rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE calibrated on a 1 GiB copy kernel
+# HBM bytes per launch of the two large kernels at this workload, from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE calibrated on a 1 GiB copy kernel
 # (profiles/hbm_traffic_pipeline_64ant_64ch_65536samp_800beam_dm50.md)
-CB_TRAFFIC_BYTES = 1.314e9   # HBM bytes per cb_mfma_kernel launch at this workload (FETCH_SIZE + WRITE_SIZE counters)
+TRAFFIC_BYTES = {"coherent_beam": 1.314e9, "dedispersion": 2.217e9}
+INT8_LDS_BYTES_PER_CLK = 128   # LDS bytes per clock per CU
+NUM_CU, PEAK_CLK_GHZ = 256, 2.4
 
 # incoherent dedispersion workload: 256 channels (4 x 64 over 53.5 MHz at
 # 1284 MHz), 512 beams, CB tscrunch 8, 1024 DM trials 0:1000:0.9765625, one
@@ -61,13 +63,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(sw, d_taftp, weights, form, batched, steps, warmup, capture=False, dms=None, fft_len=None):
+def run_form(sw, d_taftp, weights, form, batched, steps, warmup, capture=False, dms=None, fft_len=None,
+             share_forward=False):
     """Times `steps` gulps after `warmup`; with capture, returns the TFB
     output of every coherent DM for one more gulp."""
     dms = DMS if dms is None else dms
     fft_len = FFT_LEN if fft_len is None else fft_len
     p = sw.BeamformerPipeline(A, F, B, T, TS, dms, CFREQ, BW, stokes="I", fft_len=fft_len, form=form,
-                              batched=batched)
+                              batched=batched, share_forward=share_forward)
     p.set_weights(weights)
     p.init(A)
     captured = {}
@@ -140,21 +143,25 @@ def _same_outputs(a, b):
                 all(np.array_equal(a[k], b[k]) for k in a))
 
 
-def bench_target_8dm(sw, d_taftp, weights, steps, warmup, fft_len=None, label=""):
-    """The same gulp with 8 coherent DMs: chosen form vs plain form (the
-    same FFT lengths for both), outputs compared on the same gulp of the
-    stream."""
+def bench_target_8dm(sw, d_taftp, weights, steps, warmup, fft_len=None, share_forward=True, label=""):
+    """The same gulp with 8 coherent DMs: chosen form (batched dedispersion
+    launches, MFMA beams) vs plain form (one DM per dedispersion launch,
+    v_dot4 beams), with the same FFT lengths and overlaps for both (with
+    share_forward, DMs of equal length share the larger overlap, so the
+    chosen form's batched launches transform each segment forward once for
+    all of them); outputs compared on the same gulp of the stream."""
     fft_len = FFT_LEN8 if fft_len is None else fft_len
     log(f"8 coherent DMs{label}: chosen form")
     ms_c, _, st_c, tfb_c = run_form(sw, d_taftp, weights, "mfma", True, steps, warmup, capture=True,
-                                    dms=DMS8, fft_len=fft_len)
+                                    dms=DMS8, fft_len=fft_len, share_forward=share_forward)
     log(f"8 coherent DMs{label}: plain form")
     ms_p, _, st_p, tfb_p = run_form(sw, d_taftp, weights, "plain", False, steps, warmup, capture=True,
-                                    dms=DMS8, fft_len=fft_len)
+                                    dms=DMS8, fft_len=fft_len, share_forward=share_forward)
     v_c = B * F * T * len(DMS8) / (ms_c * 1e-3)
     v_p = B * F * T * len(DMS8) / (ms_p * 1e-3)
     return {"workload": "64 antennas x 64 channels x 65536 samples, 800 beams, 8 coherent DMs "
                         f"{DMS8} (FFT {list(fft_len)}), tscrunch 16, Stokes I",
+            "share_forward": bool(share_forward),
             "steps": steps, "warmup": warmup,
             "chosen": {"ms_per_step": round(ms_c, 4), "value": v_c, "xrealtime": round(T * TSAMP / (ms_c * 1e-3), 2),
                        "stage_ms": {k: round(v, 4) for k, v in st_c.items()}},
@@ -162,6 +169,21 @@ def bench_target_8dm(sw, d_taftp, weights, steps, warmup, fft_len=None, label=""
                       "stage_ms": {k: round(v, 4) for k, v in st_p.items()}},
             "speedup_vs_plain": round(v_c / v_p, 3), "target_speedup": 4.0,
             "outputs_bit_identical": _same_outputs(tfb_c, tfb_p)}
+
+
+def host_cores():
+    """Host CPU counts as seen by this process, and the threads the CPU
+    baseline uses: the affinity set, capped at the CPU share a one-GPU job
+    gets on the GPU box (OMP_NUM_THREADS, 16 there)."""
+    import subprocess
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        nproc = int(subprocess.run(["nproc", "--all"], capture_output=True, text=True).stdout.strip())
+    except (OSError, ValueError):
+        nproc = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return {"nproc_all": nproc, "os_cpu_count": os.cpu_count(), "affinity": aff, "share": share,
+            "threads": max(1, min(aff, share))}
 
 
 def cpu_baseline(sw, orc, taftp, weights, nchan_sample, nthreads):
@@ -227,13 +249,80 @@ def bench_idisp(sw, steps):
     adds = float(nout) * len(ID_DMS) * ID_B * ID_F
     nbytes = d_in.nbytes + d_out.nbytes
     tsamp_cb = ID_CBTS * ID_F / ID_BW
+    # the matrix-core form issues one 16x16x64 int8 product (16 useful adds
+    # of 16384 MACs) per 16 beams x 16 samples x 4 channels, i.e. 64 int8
+    # operations (MAC = 2 ops) per useful add
+    mfma_ops = adds * 64 if idd.kernel == "idisp_mfma_kernel" else 0.0
+    lds_bound_ms = adds / (INT8_LDS_BYTES_PER_CLK * NUM_CU * PEAK_CLK_GHZ * 1e9) * 1e3
     return {"kernel": idd.kernel, "layout": "FTB (channel-major aggregation ring)", "ms_per_dispatch": round(ms, 3), "steps": steps,
             "dispatch_samples": disp, "max_delay": idd.max_delay, "ndms": len(ID_DMS), "nchans": ID_F,
             "nbeams": ID_B, "adds_per_dispatch": adds,
             "adds_per_s": adds / (ms * 1e-3), "valu_add16_peak": VALU_ADD16_PEAK,
             "frac_valu": round(adds / (ms * 1e-3) / VALU_ADD16_PEAK, 4),
+            "int8_mfma_ops_per_dispatch": mfma_ops,
+            "frac_int8_mfma_peak": round(mfma_ops / (ms * 1e-3) / (INT8_PEAK_TOPS * 1e12), 4),
+            "lds_bound_ms": round(lds_bound_ms, 1), "frac_lds_bound": round(lds_bound_ms / ms, 4),
             "hbm_GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1),
             "xrealtime": round(disp * tsamp_cb / (ms * 1e-3), 2)}
+
+
+def bench_config5(sw, steps, warmup):
+    """The full pipeline with incoherent dedispersion at its full size: 64
+    antennas x 256 channels (53.5 MHz at 1284 MHz) x 262144-sample gulps,
+    512 beams, tscrunch 8, coherent DMs {0, 100}, each feeding 1024
+    incoherent DMs 0:1000:0.9765625 through the aggregation ring (dispatch
+    max(10 max_delay, 8192) samples).  The 17 GB gulp is a 1 GiB seeded
+    block tiled in device memory (the kernels' cost does not depend on the
+    values).  Device time per gulp over `steps` gulps, incoherent
+    dedispersion dispatches included as they fall."""
+    from skyweaver_amd import DeviceBuffer
+    A5, F5, T5, B5, TS5 = 64, 256, 262144, 512, 8
+    BW5 = 208984.375 * F5
+    cdms = [0.0, 100.0]
+    nbytes = T5 * A5 * F5 * 4
+    free0, total = sw.mem_info()
+    tile = sw.synth_voltages((1 << 29, 2), seed=0x5EED0005, sigma=16.0, nthreads=16).reshape(-1)
+    d_in = DeviceBuffer(nbytes)
+    d_tile = DeviceBuffer.from_host(tile)
+    for off in range(0, nbytes, tile.nbytes):
+        sw.check(sw.lib().sw_memcpy_dtod(d_in.ptr + off, d_tile.ptr, min(tile.nbytes, nbytes - off), None))
+    sw.synchronize()
+    d_tile.close()
+    del tile
+    weights = sw.synth_weights((F5, B5, A5, 2), seed=0x5EED1005)
+    free1, _ = sw.mem_info()
+    p = sw.BeamformerPipeline(A5, F5, B5, T5, TS5, cdms, CFREQ, BW5, id_plan=[(1, ID_DMS), (1, ID_DMS)])
+    p.set_weights(weights)
+    p.init(A5)
+    ndisp = [0]
+    p.set_tdb_sink(lambda blk: ndisp.__setitem__(0, ndisp[0] + 1))
+    for _ in range(warmup):
+        p(d_in)
+    free2, _ = sw.mem_info()
+    ndisp[0] = 0
+    stage = {k: 0.0 for k in sw.STAGES}
+    e0, e1 = sw.Event(), sw.Event()
+    e0.record(p.stream)
+    for _ in range(steps):
+        p(d_in)
+        for k, v in p.stage_times().items():
+            stage[k] += v / steps
+    e1.record(p.stream)
+    ms = e0.elapsed_ms(e1) / steps
+    p.close()
+    d_in.close()
+    tsamp = F5 / BW5
+    id_adds = (T5 / TS5) * len(ID_DMS) * B5 * F5 * len(cdms)
+    return {"workload": "64 antennas x 256 channels (53.5 MHz) x 2 pols x 262144 samples, 512 beams, tscrunch 8, "
+                        "coherent DMs [0, 100], each with 1024 incoherent DMs 0:1000:0.9765625 (TDB)",
+            "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 3),
+            "value": B5 * F5 * T5 * len(cdms) / (ms * 1e-3), "unit": "beam-samples/s",
+            "xrealtime": round(T5 * tsamp / (ms * 1e-3), 3),
+            "stage_ms": {k: round(v, 3) for k, v in stage.items()},
+            "tdb_dispatches_in_timed_gulps": ndisp[0],
+            "incoherent_adds_per_gulp": id_adds,
+            "hbm_bytes": {"input_gulp": nbytes, "pipeline": int(free1 - free2), "device_total": int(total)},
+            "data": "1 GiB seeded Gaussian block (sigma 16, seed 0x5EED0005) tiled over the gulp"}
 
 
 def main():
@@ -246,6 +335,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-idisp", action="store_true", help="skip the incoherent dedispersion measurement")
     ap.add_argument("--no-8dm", action="store_true", help="skip the 8-DM chosen vs plain measurement")
+    ap.add_argument("--no-config5", action="store_true", help="skip the full-size 256-channel pipeline with "
+                    "incoherent dedispersion")
     ap.add_argument("--only-8dm", action="store_true",
                     help="time only the chosen form on the 8-DM workload (for kernel traces) and print its line")
     args = ap.parse_args()
@@ -292,11 +383,11 @@ def main():
     cb_ms = stage["coherent_beam"]
     ops = 8.0 * F * B * T * 2 * A           # 4 int8 MACs per complex MAC, 2 ops per MAC
     achieved = ops / (cb_ms * 1e-3) / 1e12
-    roofline = {"kernel": "cb_mfma_kernel", "bound": "mfma", "achieved": round(achieved, 1),
-                "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4),
-                "traffic": CB_TRAFFIC_BYTES, "ops_per_launch": ops, "avg_launch_ms": round(cb_ms, 4),
-                "note": "int8 operations (1 MAC = 2 ops); peak is the dense int8 MFMA rate; the launch also "
-                        "computes the incoherent beam"}
+    roofline_cb = {"kernel": "cb_mfma_kernel", "bound": "mfma", "achieved": round(achieved, 1),
+                   "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4),
+                   "traffic": TRAFFIC_BYTES["coherent_beam"], "ops_per_launch": ops, "avg_launch_ms": round(cb_ms, 4),
+                   "note": "int8 operations (1 MAC = 2 ops); peak is the dense int8 MFMA rate; the launch also "
+                           "computes the incoherent beam"}
     # the other large kernel, the coherent dedisperser (FP32 VALU): per column
     # and segment a forward and an inverse N-point transform and the chirp,
     # 10 N log2 N + 6 N flops; segments per gulp T / (N - O) on the stream's
@@ -312,9 +403,13 @@ def main():
     cd_tf = cd_flops / (cd_ms * 1e-3) / 1e12
     roofline_dd = {"kernel": "codedisp_reg_kernel<32>", "bound": "valu", "achieved": round(cd_tf, 1),
                    "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(cd_tf / FP32_PEAK_TFLOPS, 4),
+                   "traffic": TRAFFIC_BYTES["dedispersion"],
                    "flops_per_launch": cd_flops, "avg_launch_ms": round(cd_ms, 4), "fft_len": N, "overlap": O,
                    "note": "FP32 FFT flops (10 N log2 N + 6 N per column and segment); the peak counts packed "
                            "FP32, which this build does not use (unpacked rate 78.6 TFLOP/s)"}
+    # the roofline line is for the kernel that takes the most time per gulp
+    roofline = dict(roofline_dd if cd_ms >= cb_ms else roofline_cb)
+    roofline["dominant_of"] = {"dedispersion_ms": round(cd_ms, 4), "coherent_beam_ms": round(cb_ms, 4)}
     hbm = {}
     # the transpose stage is taftp_to_fpat_stats_kernel: the statistics' power
     # sums are taken in the same pass (one read of the gulp, one write), and
@@ -335,12 +430,14 @@ def main():
         target = bench_target_8dm(sw, d_taftp, weights, max(2, args.steps // 5), 2)
         # the same 8 DMs with FFT lengths picked by the autotuner (one set for both forms)
         log("autotuning the 8 DMs' FFT lengths")
-        tuned, tms = sw.CoherentDedisperser.autotune(F, A, T, CFREQ - BW / 2, BW / F, TSAMP, DMS8, tune_chans=8)
+        tuned, share, tms = sw.CoherentDedisperser.autotune(F, A, T, CFREQ - BW / 2, BW / F, TSAMP, DMS8,
+                                                            tune_chans=8)
         target_tuned = bench_target_8dm(sw, d_taftp, weights, max(2, args.steps // 5), 2, fft_len=tuned,
-                                        label=" (autotuned FFT lengths)")
+                                        share_forward=share, label=" (autotuned FFT lengths)")
         target_tuned["autotune_ms_per_gulp"] = {
             "fft_len": [1024, 2048, 4096, 8192, 16384, 32768],
-            "per_dm": {str(dm): [round(float(x), 3) for x in tms[d]] for d, dm in enumerate(DMS8)}}
+            "per_dm": {str(dm): [round(float(x), 3) for x in tms[d]] for d, dm in enumerate(DMS8)},
+            "batched_all_that_fit": [round(float(x), 3) for x in tms[len(DMS8)]]}
 
     host_input = None
     if not args.no_plain:
@@ -352,19 +449,31 @@ def main():
         log("incoherent dedispersion (1024 DMs x 256 channels x 512 beams)")
         idisp = bench_idisp(sw, max(1, args.steps // 5))
 
+    config5 = None
+    if not args.no_config5:
+        log("full pipeline, 256 channels x 262144 samples, 2 coherent DMs x 1024 incoherent DMs")
+        config5 = bench_config5(sw, max(3, args.steps // 5), 2)
+
     cpu = None
     if not args.no_cpu:
         import oracle
-        nthreads = min(16, os.cpu_count() or 1)
+        cores = host_cores()
+        nthreads = cores["threads"]
         # three channels per thread: about 13 s of CPU work on 16 threads
         nchan = args.cpu_channels or max(1, min(F, 3 * nthreads))
         log(f"CPU oracle sample: {nchan} channels on {nthreads} threads")
         secs = cpu_baseline(sw, oracle, taftp, weights, nchan, nthreads)
         cpu_value = B * nchan * T * len(DMS) / secs
+        log("CPU oracle sample: 2 channels on 1 thread")
+        secs1 = cpu_baseline(sw, oracle, taftp, weights, 2, 1)
         cpu = {"value": cpu_value, "unit": "beam-samples/s", "cores": nthreads, "kind": "port",
                "sample": f"{nchan} of {F} channels of one gulp (transpose, statistics, FP64 dedispersion, "
-                         f"incoherent and coherent beams), {secs:.1f} s; work is linear in channels",
-               "seconds": round(secs, 2), "xrealtime": T * TSAMP / (secs * F / nchan)}
+                         f"incoherent and coherent beams), {secs:.1f} s on {nthreads} threads; work is linear in "
+                         "channels",
+               "seconds": round(secs, 2), "xrealtime": T * TSAMP / (secs * F / nchan),
+               "host": cores,
+               "one_core": {"value": B * 2 * T * len(DMS) / secs1, "seconds": round(secs1, 2),
+                            "sample": f"2 of {F} channels on 1 thread"}}
 
     out = {
         "metric": "beam-samples/s",
@@ -389,12 +498,14 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stage.items()},
         "roofline": roofline,
         "roofline_dedispersion": roofline_dd,
+        "roofline_coherent_beam": roofline_cb,
         "hbm_kernels": hbm,
         "plain": plain,
         "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
         "target_8dm": target,
         "target_8dm_autotuned_fft": target_tuned,
         "incoherent_dedispersion": idisp,
+        "config5_full_pipeline": config5,
         "host_input": host_input,
         "cpu_baseline": cpu,
     }

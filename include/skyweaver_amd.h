@@ -62,6 +62,7 @@ const char* sw_version(void);
  * second GPU runtime in its process.  No reference counterpart: the
  * reference allocates through thrust containers (cpp/skyweaver/DescribedVector.hpp). */
 int sw_device_count(int* count);
+int sw_mem_info(size_t* free_bytes, size_t* total_bytes); /* current device */
 int sw_malloc(void** d_ptr, size_t bytes);
 int sw_free(void* d_ptr);
 int sw_host_alloc(void** h_ptr, size_t bytes); /* pinned host memory */
@@ -253,7 +254,8 @@ int sw_idisp_dedisperse_ftb(sw_idisp* plan, const void* d_ftb, size_t nsamples, 
  * latency L = 0 when the hop divides the gulp length T and L = N - O
  * otherwise (sw_codedisp_info).  fft_len / overlap entries <= 0 (or NULL
  * arrays) select O = even-ceil of the intra-channel smear at the bottom of
- * the band and N = the smallest power of two >= max(1024, 16 O). */
+ * the band and N = the smallest power of two >= max(1024, 16 O).
+ * With share_forward, DMs of equal N get the largest O among them. */
 typedef struct sw_codedisp_config {
     int nchans;        /* F */
     int nantennas;     /* A (FTPA padded antenna count) */
@@ -271,6 +273,10 @@ typedef struct sw_codedisp_config {
                              unchanged until the next sw_codedisp_run; no
                              history copy.  Ignored when the history spans
                              more than one gulp. */
+    int share_forward;    /* 1: DMs of equal FFT length take one overlap, the
+                             largest of their own, so that they lie on one
+                             segment grid and a batched run computes each
+                             segment's forward transform once for all of them */
 } sw_codedisp_config;
 typedef struct sw_codedisp sw_codedisp;
 
@@ -280,11 +286,16 @@ int sw_codedisp_destroy(sw_codedisp* plan);
  * length for the whole gulp).  For every DM of cfg, with the overlap the
  * plan would choose, times one gulp of synthetic input through the
  * dedisperser on min(tune_chans, nchans) channels for each length in
- * {1024, ..., 32768} of at least 4 O, and writes the fastest to
- * fft_len_out[ndms]; ms_out (NULL allowed) receives [ndms][6] times per
- * gulp scaled to nchans (-1: length not tried). */
+ * {1024, ..., 32768} of at least 4 O, one DM per launch; then, for each
+ * length, all DMs it fits (4 O <= N) as one batched launch sharing the
+ * forward transforms (share_forward), with the other DMs at their own best
+ * length.  The faster plan is written out: fft_len_out[ndms] and
+ * *share_forward_out (NULL allowed; 1 when the batched plan won).  ms_out
+ * (NULL allowed) receives [ndms + 1][6] times per gulp scaled to nchans:
+ * rows 0 .. ndms-1 the single-DM times, row ndms the batched time of each
+ * length (-1: not tried). */
 int sw_codedisp_autotune(const sw_codedisp_config* cfg, int tune_chans, int* fft_len_out,
-                         float* ms_out, void* stream);
+                         int* share_forward_out, float* ms_out, void* stream);
 int sw_codedisp_info(const sw_codedisp* plan, int dm_idx, int* fft_len, int* overlap,
                      int* latency);
 /* Dedisperse the gulp d_ftpa (FTPA [F][T][2][A]) for the DMs
@@ -425,7 +436,9 @@ typedef struct sw_pipeline_config {
     const int* fft_len;     /* per DM or NULL (see sw_codedisp_config) */
     const int* overlap;     /* per DM or NULL */
     int cb_form;            /* enum sw_cb_form */
-    int batched_dms;        /* 1: share forward FFTs across DMs */
+    int batched_dms;        /* 1: one launch per group of DMs with equal FFT length
+                               and overlap, sharing each segment's forward FFT;
+                               0: one DM per launch (identical results) */
     /* Incoherent dedispersion of the coherent beams (PipelineConfig::
      * enable_incoherent_dedispersion and the DedispersionPlan blocks): when
      * id_enable is set, every coherent DM's TFB output is also pushed into an
@@ -437,9 +450,11 @@ typedef struct sw_pipeline_config {
     const float* const* id_dms;
     const int* id_ndms;
     int id_dispatch_size;   /* 0: max(10 max_delay, 8192) */
-    /* 1: when fft_len is NULL, pick each coherent DM's FFT length with
-     * sw_codedisp_autotune (on 8 channels) at create time */
+    /* 1: when fft_len is NULL, pick each coherent DM's FFT length (and
+     * share_forward) with sw_codedisp_autotune (on 8 channels) at create time */
     int autotune_fft;
+    /* sw_codedisp_config::share_forward for the pipeline's dedisperser */
+    int share_forward;
 } sw_pipeline_config;
 
 typedef struct sw_obs_header {

@@ -66,7 +66,7 @@ class CodedispConfig(ctypes.Structure):
                 ("low_freq", ctypes.c_double), ("chan_bw", ctypes.c_double), ("tsamp", ctypes.c_double),
                 ("ndms", ctypes.c_int), ("dms", ctypes.POINTER(ctypes.c_float)),
                 ("fft_len", ctypes.POINTER(ctypes.c_int)), ("overlap", ctypes.POINTER(ctypes.c_int)),
-                ("input_fpat", ctypes.c_int), ("history_in_place", ctypes.c_int)]
+                ("input_fpat", ctypes.c_int), ("history_in_place", ctypes.c_int), ("share_forward", ctypes.c_int)]
 
 
 _vp = ctypes.c_void_p
@@ -101,11 +101,12 @@ def _declare(L):
     L.sw_codedisp_create.argtypes = [ctypes.POINTER(CodedispConfig), _vp, _pvp]
     L.sw_codedisp_destroy.argtypes = [_vp]
     L.sw_codedisp_info.argtypes = [_vp, _c, ctypes.POINTER(_c), ctypes.POINTER(_c), ctypes.POINTER(_c)]
-    L.sw_codedisp_autotune.argtypes = [ctypes.POINTER(CodedispConfig), _c, _vp, _vp, _vp]
+    L.sw_codedisp_autotune.argtypes = [ctypes.POINTER(CodedispConfig), _c, _vp, _vp, _vp, _vp]
     L.sw_codedisp_run.argtypes = [_vp, _vp, _pvp, _pvp, _c, _c, _c, _vp]
     L.sw_codedisp_push_history.argtypes = [_vp, _vp, _vp]
     L.sw_codedisp_set_history.argtypes = [_vp, _vp, _vp]
     L.sw_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    L.sw_mem_info.argtypes = [ctypes.POINTER(_sz), ctypes.POINTER(_sz)]
     L.sw_malloc.argtypes = [_pvp, _sz]
     L.sw_free.argtypes = [_vp]
     L.sw_host_alloc.argtypes = [_pvp, _sz]
@@ -163,6 +164,13 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     check(lib().sw_device_count(ctypes.byref(n)))
     return n.value
+
+
+def mem_info():
+    """(free, total) bytes of device memory on the current device."""
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    check(lib().sw_mem_info(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
 
 
 class DeviceBuffer:
@@ -521,10 +529,12 @@ class CoherentDedisperser:
     """Overlap-save coherent dedispersion plan (sw_codedisp_* in the C ABI)."""
 
     def __init__(self, nchans, nantennas, nsamples, low_freq, chan_bw, tsamp, dms, fft_len=None,
-                 overlap=None, stream=None, input_fpat=False, history_in_place=False):
+                 overlap=None, stream=None, input_fpat=False, history_in_place=False, share_forward=False):
         """history_in_place (with input_fpat): the history is read from the
         gulp last given to push_history, which must stay unchanged until the
-        next run."""
+        next run.  share_forward: DMs of equal FFT length take the largest of
+        their overlaps, so a batched run transforms each segment once for
+        all of them."""
         import numpy as np
         self._dms = np.ascontiguousarray(dms, np.float32)
         n = len(self._dms)
@@ -534,7 +544,7 @@ class CoherentDedisperser:
                              self._dms.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                              self._fft.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                              self._ov.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 1 if input_fpat else 0,
-                             1 if history_in_place else 0)
+                             1 if history_in_place else 0, 1 if share_forward else 0)
         h = ctypes.c_void_p()
         check(lib().sw_codedisp_create(ctypes.byref(cfg), _stream(stream), ctypes.byref(h)))
         self.handle = h.value
@@ -549,8 +559,9 @@ class CoherentDedisperser:
     @staticmethod
     def autotune(nchans, nantennas, nsamples, low_freq, chan_bw, tsamp, dms, overlap=None, tune_chans=8,
                  input_fpat=True, stream=None):
-        """sw_codedisp_autotune: (fastest FFT length per DM, times [ndms][6] in ms per gulp for
-        lengths 1024..32768, -1 where not tried)."""
+        """sw_codedisp_autotune: (FFT length per DM, share_forward, times [ndms + 1][6] in ms
+        per gulp for lengths 1024..32768: one row per DM run alone, then the batched run of
+        every DM a length fits; -1 where not tried)."""
         import numpy as np
         dm = np.ascontiguousarray(dms, np.float32)
         n = len(dm)
@@ -559,10 +570,11 @@ class CoherentDedisperser:
                              dm.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), None,
                              ov.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 1 if input_fpat else 0, 0)
         out = np.zeros(n, np.int32)
-        ms = np.zeros((n, 6), np.float32)
-        check(lib().sw_codedisp_autotune(ctypes.byref(cfg), tune_chans, out.ctypes.data, ms.ctypes.data,
-                                         _stream(stream)))
-        return [int(x) for x in out], ms
+        share = ctypes.c_int(0)
+        ms = np.zeros((n + 1, 6), np.float32)
+        check(lib().sw_codedisp_autotune(ctypes.byref(cfg), tune_chans, out.ctypes.data, ctypes.byref(share),
+                                         ms.ctypes.data, _stream(stream)))
+        return [int(x) for x in out], bool(share.value), ms
 
     def latency(self, d):
         """Output latency of DM d in samples: a gulp g emits stream samples [gT - L, (g+1)T - L)."""
@@ -607,7 +619,8 @@ class PipelineConfigC(ctypes.Structure):
                 ("coherent_dms", ctypes.POINTER(ctypes.c_float)), ("fft_len", ctypes.POINTER(ctypes.c_int)),
                 ("overlap", ctypes.POINTER(ctypes.c_int)), ("cb_form", ctypes.c_int), ("batched_dms", ctypes.c_int),
                 ("id_enable", ctypes.c_int), ("id_tscrunch", ctypes.c_void_p), ("id_dms", ctypes.c_void_p),
-                ("id_ndms", ctypes.c_void_p), ("id_dispatch_size", ctypes.c_int), ("autotune_fft", ctypes.c_int)]
+                ("id_ndms", ctypes.c_void_p), ("id_dispatch_size", ctypes.c_int), ("autotune_fft", ctypes.c_int),
+                ("share_forward", ctypes.c_int)]
 
 
 class ObsHeaderC(ctypes.Structure):
@@ -624,6 +637,36 @@ class BlockC(ctypes.Structure):
 
 
 SINK_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(BlockC))
+
+
+class _SinkErrors:
+    """Exceptions raised by Python sinks inside C callbacks.  A callback that
+    raises returns 1, so the library stops the gulp with SW_ERR_IO; the
+    recorded exception is then re-raised by the call that ran the gulp."""
+
+    def __init__(self):
+        self.exc = None
+
+    def wrap(self, fn):
+        if fn is None:
+            return SINK_FN()
+
+        def _f(user, blk):
+            try:
+                fn(blk.contents)
+                return 0
+            except BaseException as e:  # noqa: BLE001 -- re-raised by check()
+                if self.exc is None:
+                    self.exc = e
+                return 1
+        return SINK_FN(_f)
+
+    def check(self, rc):
+        """check(rc), but a sink's own exception takes precedence."""
+        exc, self.exc = self.exc, None
+        if exc is not None:
+            raise exc
+        check(rc)
 
 
 def _declare_pipeline(L):
@@ -654,9 +697,11 @@ class BeamformerPipeline:
 
     def __init__(self, nantennas, nchans, nbeams, nsamples, tscrunch, dms, centre_frequency, bandwidth,
                  stokes="I", ib_subtract=True, output_level=24.0, fft_len=None, overlap=None, form="mfma",
-                 batched=True, total_nchans=4096, id_plan=None, id_dispatch_size=0, autotune_fft=False):
+                 batched=True, total_nchans=4096, id_plan=None, id_dispatch_size=0, autotune_fft=False,
+                 share_forward=False):
         """id_plan: None (no incoherent dedispersion) or, per coherent DM, (tscrunch, incoherent DMs).
-        autotune_fft (with fft_len None): FFT lengths chosen by sw_codedisp_autotune."""
+        autotune_fft (with fft_len None): FFT lengths and share_forward chosen by sw_codedisp_autotune.
+        share_forward: DMs of equal FFT length share one overlap (and, batched, their forward FFTs)."""
         import numpy as np
         L = lib()
         if not hasattr(L, "_pipeline_declared"):
@@ -674,6 +719,7 @@ class BeamformerPipeline:
                               self._ov.ctypes.data_as(ip) if self._ov is not None else None,
                               FORMS[form], 1 if batched else 0)
         cfg.autotune_fft = 1 if autotune_fft else 0
+        cfg.share_forward = 1 if share_forward else 0
         if id_plan is not None:
             assert len(id_plan) == n
             self._id_ts = np.array([b[0] for b in id_plan], np.int32)
@@ -688,6 +734,7 @@ class BeamformerPipeline:
         h = ctypes.c_void_p()
         check(L.sw_pipeline_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value
+        self._errors = _SinkErrors()
         self._sinks = []
         self._tdb_sink = SINK_FN()
 
@@ -703,14 +750,10 @@ class BeamformerPipeline:
         check(lib().sw_pipeline_set_beamsets(self.handle, w.shape[0], m.ctypes.data, w.ctypes.data))
 
     def set_sinks(self, cb=None, ib=None, stats=None):
-        """Python callables f(block: BlockC) -> None, invoked synchronously."""
-        def wrap(fn):
-            if fn is None:
-                return SINK_FN()
-            def _f(user, blk):
-                fn(blk.contents)
-                return 0
-            return SINK_FN(_f)
+        """Python callables f(block: BlockC) -> None, invoked synchronously.
+        An exception raised by a sink ends the gulp and is re-raised by the
+        call that processed it."""
+        wrap = self._errors.wrap
         self._sinks = [wrap(cb), wrap(ib), wrap(stats)]
         check(lib().sw_pipeline_set_sinks(self.handle, self._sinks[0], None, self._sinks[1], None,
                                           self._sinks[2], None))
@@ -720,10 +763,7 @@ class BeamformerPipeline:
         check(lib().sw_pipeline_set_delay_file(self.handle, str(path).encode()))
 
     def set_tdb_sink(self, fn):
-        def _f(user, blk):
-            fn(blk.contents)
-            return 0
-        self._tdb_sink = SINK_FN(_f)
+        self._tdb_sink = self._errors.wrap(fn)
         check(lib().sw_pipeline_set_tdb_sink(self.handle, self._tdb_sink, None))
 
     def init(self, nantennas, utc_start=1708082229.0, sample_clock=1712e6, utc_offset=0.0):
@@ -733,11 +773,11 @@ class BeamformerPipeline:
     def __call__(self, taftp, nbytes=None):
         """Process one gulp: a DeviceBuffer (device-resident) or a numpy array (host)."""
         if isinstance(taftp, DeviceBuffer):
-            check(lib().sw_pipeline_process(self.handle, taftp.ptr, nbytes or taftp.nbytes, 1))
+            self._errors.check(lib().sw_pipeline_process(self.handle, taftp.ptr, nbytes or taftp.nbytes, 1))
         else:
             import numpy as np
             a = np.ascontiguousarray(taftp)
-            check(lib().sw_pipeline_process(self.handle, a.ctypes.data, a.nbytes, 0))
+            self._errors.check(lib().sw_pipeline_process(self.handle, a.ctypes.data, a.nbytes, 0))
         return False
 
     def prefetch(self, taftp):
@@ -750,7 +790,7 @@ class BeamformerPipeline:
 
     def process_prefetched(self):
         """Process the oldest prefetched gulp."""
-        check(lib().sw_pipeline_process_prefetched(self.handle))
+        self._errors.check(lib().sw_pipeline_process_prefetched(self.handle))
         return False
 
     def synchronize(self):
@@ -822,6 +862,7 @@ class IncoherentDedispersionPipeline:
         h = ctypes.c_void_p()
         check(L.sw_idpipe_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h
+        self._errors = _SinkErrors()
         self._sink = SINK_FN()
 
     def info(self, block):
@@ -830,22 +871,20 @@ class IncoherentDedispersionPipeline:
         return {"max_delay": md.value, "dispatch_size": ds.value, "filled": fl.value}
 
     def set_sink(self, fn):
-        """fn(block: BlockC) -> None, invoked synchronously for every TDB block."""
-        def _f(user, blk):
-            fn(blk.contents)
-            return 0
-        self._sink = SINK_FN(_f)
+        """fn(block: BlockC) -> None, invoked synchronously for every TDB block;
+        an exception it raises is re-raised by the push that produced the block."""
+        self._sink = self._errors.wrap(fn)
         check(lib().sw_idpipe_set_sink(self.handle, self._sink, None))
 
     def push(self, block, tfb, nsamples, utc_offset, tsamp, stream=None):
         import numpy as np
         if isinstance(tfb, np.ndarray):
             arr = np.ascontiguousarray(tfb)
-            check(lib().sw_idpipe_push(self.handle, block, arr.ctypes.data, nsamples, 0, float(utc_offset),
-                                       float(tsamp), _stream(stream)))
+            self._errors.check(lib().sw_idpipe_push(self.handle, block, arr.ctypes.data, nsamples, 0,
+                                                    float(utc_offset), float(tsamp), _stream(stream)))
         else:
-            check(lib().sw_idpipe_push(self.handle, block, _ptr(tfb), nsamples, 1, float(utc_offset), float(tsamp),
-                                       _stream(stream)))
+            self._errors.check(lib().sw_idpipe_push(self.handle, block, _ptr(tfb), nsamples, 1, float(utc_offset),
+                                                    float(tsamp), _stream(stream)))
 
     def close(self):
         if getattr(self, "handle", None):

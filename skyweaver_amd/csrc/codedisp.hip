@@ -301,10 +301,11 @@ __device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddl
 
 // Y = X H with the chirp stored permuted (index 32 R1 k3 + 32 k1 + k2), then
 // the inverse steps 3'-1': v ends holding y[512 n1 + t] (unnormalised) at
-// c R1 + n1.
+// c R1 + n1.  X (the spectrum reg_forward left) may be v itself, or be kept
+// unchanged for further DMs.
 template <int R1>
-__device__ __forceinline__ void reg_chirp_inverse(float2* v, float2* lds, const Twiddles& tw, const float2* t1k,
-                                                  int t, const float2* __restrict__ H)
+__device__ __forceinline__ void reg_chirp_inverse(const float2* X, float2* v, float2* lds, const Twiddles& tw,
+                                                  const float2* t1k, int t, const float2* __restrict__ H)
 {
     constexpr int C = 32 / R1;
     const int n3 = t & 15, j2 = t >> 4;
@@ -314,7 +315,7 @@ __device__ __forceinline__ void reg_chirp_inverse(float2* v, float2* lds, const 
         const int k2 = n3 + 16 * h, k1 = k1_2;
         float2* x = v + 16 * h;
 #pragma unroll
-        for(int k3 = 0; k3 < 16; ++k3) x[k3] = cmul(x[k3], H[32 * R1 * k3 + 32 * k1 + k2]);
+        for(int k3 = 0; k3 < 16; ++k3) x[k3] = cmul(X[16 * h + k3], H[32 * R1 * k3 + 32 * k1 + k2]);
         // step 3': IDFT16 -> n3, x W_N^(-R1 n3 k2) and W_N^(-k1 n3): the
         // second factor of step 2's twiddle W_N^(-k1 (16 n2 + n3)) does not
         // depend on k2, so by linearity it is applied before the IDFT32
@@ -442,7 +443,7 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
         if(u + (int)gridDim.x < nunits) fetch(u + gridDim.x);  // in flight during the transforms
         __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
         reg_forward<R1>(v, lds, tw, t1k, t);
-        reg_chirp_inverse<R1>(v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N);
+        reg_chirp_inverse<R1>(v, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N);
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
         // positions 512 .. N - 513 are all kept when O / 2 <= 512; the
         // opaque copy of t keeps the position tests of the first and last
@@ -460,6 +461,68 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
                     store_kept(p, d, cb, j0, 512 * n1 + tt, y);
                 else
                     store_output(p, d, cb, j0, 512 * n1 + tt, N, y);
+            }
+        }
+    }
+}
+
+// The DMs of one launch (p.ndm, sharing N and the overlap) from one forward
+// transform: per unit the spectrum X stays in registers while each DM's
+// chirp, inverse transform and stores run in turn, so a unit costs one
+// forward and p.ndm inverse transforms instead of p.ndm of each.  The next
+// unit's input is fetched during the last DM, once X is no longer needed.
+template <int R1>
+__global__ void __launch_bounds__(512) codedisp_regm_kernel(CdParams p, int nunits)
+{
+    constexpr int N = 512 * R1;
+    constexpr int C = 32 / R1;
+    __shared__ float2 lds[kRegLds + TwiddleLds<N>::SIZE + 1536];
+    const Twiddles tw = load_twiddles<N, 512>(lds + kRegLds, p.tw);
+    float2* t1k = lds + kRegLds + TwiddleLds<N>::SIZE;  // W_1024^j = W_N^(j N / 1024), then W_512^j
+    const int t = threadIdx.x;
+    for(int j = t; j < 1024; j += 512) t1k[j] = p.tw[j * (N / 1024)];
+    t1k[1024 + t] = p.tw[t * (N / 512)];
+    const int ngrp = p.PA / C;
+    uint32_t nxt[16];  // two raw samples per register: value i = c R1 + n1
+    auto fetch = [&](int u) {
+        const int cg = u % ngrp, fs = u / ngrp;
+        const int f = fs / p.nseg, s = fs % p.nseg;
+        const long b = p.in0 + (long)s * p.hop;
+#pragma unroll
+        for(int c = 0; c < C; ++c) load_column<R1 / 2>(p, f, cg * C + c, b, t, nxt + c * (R1 / 2));
+    };
+    if(blockIdx.x < nunits) fetch(blockIdx.x);
+    for(int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int cg = u % ngrp, fs = u / ngrp;
+        const int f = fs / p.nseg, s = fs % p.nseg;
+        float2 X[32];
+#pragma unroll
+        for(int i = 0; i < 16; ++i) {
+            X[2 * i]     = unpack_c8(nxt[i]);
+            X[2 * i + 1] = unpack_c8(nxt[i] >> 16);
+        }
+        __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
+        reg_forward<R1>(X, lds, tw, t1k, t);
+        const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
+        const bool interior = p.O / 2 <= 512;
+        for(int d = 0; d < p.ndm; ++d) {
+            if(d > 0) __syncthreads();  // the previous DM's readers of the exchange buffer
+            if(d == p.ndm - 1 && u + (int)gridDim.x < nunits) fetch(u + gridDim.x);
+            float2 v[32];
+            reg_chirp_inverse<R1>(X, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N);
+            int tt = t;
+            asm volatile("" : "+v"(tt));
+#pragma unroll
+            for(int c = 0; c < C; ++c) {
+                const size_t cb = ((size_t)f * p.PA + cg * C + c) * p.ylen;
+#pragma unroll
+                for(int n1 = 0; n1 < R1; ++n1) {
+                    const float2 y = v[c * R1 + n1];  // already / N (chirp)
+                    if(interior && n1 > 0 && n1 < R1 - 1)
+                        store_kept(p, d, cb, j0, 512 * n1 + tt, y);
+                    else
+                        store_output(p, d, cb, j0, 512 * n1 + tt, N, y);
+                }
             }
         }
     }
@@ -521,7 +584,7 @@ __global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, i
             }
             __syncthreads();  // twiddle tables; previous readers of the exchange buffer
             reg_forward<32>(v, lds, tw, t1k, t);
-            reg_chirp_inverse<32>(v, lds, tw, t1k, t, H + r * L);
+            reg_chirp_inverse<32>(v, v, lds, tw, t1k, t, H + r * L);
             if(r == 0) {
 #pragma unroll
                 for(int n1 = 0; n1 < 32; ++n1) mine[512 * n1] = v[n1];
@@ -714,7 +777,9 @@ int reg_r1(int N, int RP, int PA)
 }
 
 // units = (channel, segment, group of 32 / R1 columns), or (channel,
-// segment, column) for N = 32768; one DM per launch
+// segment, column) for N = 32768.  N <= 16384: several DMs (p.ndm > 1) share
+// one forward transform in codedisp_regm_kernel, one DM alone runs in
+// codedisp_reg_kernel; N = 32768: one DM per launch.
 int launch_reg(int R1, const CdParams& p, int nunits, float2* scratch, hipStream_t st)
 {
     const dim3 grid((unsigned)std::min(nunits, persistent_blocks()));
@@ -723,6 +788,19 @@ int launch_reg(int R1, const CdParams& p, int nunits, float2* scratch, hipStream
             hipLaunchKernelGGL(codedisp_reg2_kernel, grid, dim3(512), 0, st, p, d, nunits, scratch);
             SW_LAUNCH_CHECK();
         }
+        return SW_OK;
+    }
+    if(p.ndm > 1) {
+#define SW_REGM(RR)                                                                             \
+    case RR:                                                                                    \
+        hipLaunchKernelGGL((codedisp_regm_kernel<RR>), grid, dim3(512), 0, st, p, nunits);     \
+        break;
+        switch(R1) {
+            SW_REGM(2) SW_REGM(4) SW_REGM(8) SW_REGM(16) SW_REGM(32)
+        default: set_error("codedisp: no register kernel for this length"); return SW_ERR_INVALID;
+        }
+#undef SW_REGM
+        SW_LAUNCH_CHECK();
         return SW_OK;
     }
     for(int d = 0; d < p.ndm; ++d) {
@@ -859,8 +937,13 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
         p->dms.push_back(dm);
         p->N.push_back(N);
         p->O.push_back(O);
-        p->Omax = std::max(p->Omax, O);
     }
+    if(cfg->share_forward) {  // one overlap per FFT length: the largest
+        std::map<int, int> omax;
+        for(size_t d = 0; d < p->dms.size(); ++d) omax[p->N[d]] = std::max(omax[p->N[d]], p->O[d]);
+        for(size_t d = 0; d < p->dms.size(); ++d) p->O[d] = omax[p->N[d]];
+    }
+    for(int O : p->O) p->Omax = std::max(p->Omax, O);
     // group DMs sharing (N, O): one forward transform serves the whole group
     std::map<std::pair<int, int>, int> gid;
     for(int d = 0; d < (int)p->dms.size(); ++d) {
@@ -973,8 +1056,37 @@ __global__ void tune_fill_kernel(uint32_t* x, size_t n)
 } // namespace
 } // namespace sw
 
+namespace sw {
+namespace {
+// Device time (ms) of one gulp of plan DMs [0, ndm) run together (batched) or
+// one at a time: two warm-up gulps, three timed.
+int time_gulps(sw_codedisp* plan, const void* d_in, int ndm, int dm_begin, int batched, hipStream_t st,
+               hipEvent_t e0, hipEvent_t e1, float* ms_out)
+{
+    std::vector<void*> outs(plan->dms.size(), nullptr);  // outputs stay in the ring, as in the pipeline
+    float acc = 0.0f;
+    for(int it = 0; it < 5; ++it) {
+        if(it >= 2) SW_HIP_TRY(hipEventRecord(e0, st));
+        int rc = sw_codedisp_run(plan, d_in, outs.data(), nullptr, dm_begin, ndm, batched, st);
+        if(rc) return rc;
+        if(it >= 2) {
+            SW_HIP_TRY(hipEventRecord(e1, st));
+            SW_HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            SW_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            acc += ms;
+        }
+        rc = sw_codedisp_push_history(plan, d_in, st);
+        if(rc) return rc;
+    }
+    *ms_out = acc / 3.0f;
+    return SW_OK;
+}
+} // namespace
+} // namespace sw
+
 extern "C" int sw_codedisp_autotune(const sw_codedisp_config* cfg, int tune_chans, int* fft_len_out,
-                                    float* ms_out, void* stream)
+                                    int* share_forward_out, float* ms_out, void* stream)
 {
     SW_CHECK_ARG(cfg && fft_len_out && cfg->ndms > 0 && cfg->dms && tune_chans > 0, "bad argument");
     hipStream_t st = sw::as_stream(stream);
@@ -991,6 +1103,7 @@ extern "C" int sw_codedisp_autotune(const sw_codedisp_config* cfg, int tune_chan
     sub.nchans = std::min(tune_chans, cfg->nchans);
     sub.overlap = O.data();
     sub.history_in_place = 0;
+    sub.share_forward = 0;
     const size_t words = (size_t)sub.nchans * 2 * cfg->nantennas * cfg->nsamples / 2;
     uint32_t* d_in = nullptr;
     SW_HIP_TRY(hipMalloc(&d_in, words * 4));
@@ -998,46 +1111,72 @@ extern "C" int sw_codedisp_autotune(const sw_codedisp_config* cfg, int tune_chan
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    std::vector<float> best(nd, 1e30f), times((size_t)nd * kNc, -1.0f);
+    const float scale = (float)cfg->nchans / (float)sub.nchans;
+    std::vector<float> best(nd, 1e30f), times((size_t)(nd + 1) * kNc, -1.0f);
     std::vector<int> N(nd, 0);
-    std::vector<void*> outs(nd, nullptr);  // outputs stay in the ring, as in the pipeline
+    std::vector<std::vector<int>> fits(kNc);
     int rc = SW_OK;
+    // one DM per launch, every length
     for(int c = 0; c < kNc && rc == SW_OK; ++c) {
         // DMs for which this length keeps the overlap waste below a quarter
-        std::vector<int> fl(nd), use;
+        std::vector<int> fl(nd);
         for(int d = 0; d < nd; ++d) {
             const bool ok = kCand[c] >= 4 * O[d];
             fl[d] = ok ? kCand[c] : 0;
-            if(ok) use.push_back(d);
+            if(ok) fits[c].push_back(d);
         }
-        if(use.empty()) continue;
+        if(fits[c].empty()) continue;
         sub.fft_len = fl.data();
         sw_codedisp* plan = nullptr;
         rc = sw_codedisp_create(&sub, stream, &plan);
         if(rc) break;
-        for(int d : use) {
-            float acc = 0.0f;
-            for(int it = 0; it < 5 && rc == SW_OK; ++it) {  // two warm-up gulps, three timed
-                if(it >= 2) (void)hipEventRecord(e0, st);
-                rc = sw_codedisp_run(plan, d_in, outs.data(), nullptr, d, 1, 0, stream);
-                if(rc) break;
-                if(it >= 2) {
-                    (void)hipEventRecord(e1, st);
-                    (void)hipEventSynchronize(e1);
-                    float ms = 0.0f;
-                    (void)hipEventElapsedTime(&ms, e0, e1);
-                    acc += ms;
-                }
-                rc = sw_codedisp_push_history(plan, d_in, stream);
-            }
-            const float ms = acc / 3.0f * (float)cfg->nchans / (float)sub.nchans;
+        for(int d : fits[c]) {
+            float ms = 0.0f;
+            rc = sw::time_gulps(plan, d_in, 1, d, 0, st, e0, e1, &ms);
+            if(rc) break;
+            ms *= scale;
             times[(size_t)d * kNc + c] = ms;
-            if(rc == SW_OK && ms < best[d]) {
+            if(ms < best[d]) {
                 best[d] = ms;
                 N[d] = kCand[c];
             }
         }
         sw_codedisp_destroy(plan);
+    }
+    // every DM a length fits, batched at that length with one overlap
+    float best_total = 0.0f;
+    for(int d = 0; d < nd; ++d) best_total += best[d];
+    int best_c = -1;
+    for(int c = 0; c < kNc && rc == SW_OK; ++c) {
+        if(fits[c].size() < 2) continue;
+        std::vector<float> dm(fits[c].size());
+        std::vector<int> fl(fits[c].size(), kCand[c]), ov(fits[c].size());
+        for(size_t k = 0; k < fits[c].size(); ++k) {
+            dm[k] = cfg->dms[fits[c][k]];
+            ov[k] = O[fits[c][k]];
+        }
+        sw_codedisp_config bat = sub;
+        bat.ndms = (int)dm.size();
+        bat.dms = dm.data();
+        bat.fft_len = fl.data();
+        bat.overlap = ov.data();
+        bat.share_forward = 1;
+        sw_codedisp* plan = nullptr;
+        rc = sw_codedisp_create(&bat, stream, &plan);
+        if(rc) break;
+        float ms = 0.0f;
+        rc = sw::time_gulps(plan, d_in, bat.ndms, 0, 1, st, e0, e1, &ms);
+        sw_codedisp_destroy(plan);
+        if(rc) break;
+        ms *= scale;
+        times[(size_t)nd * kNc + c] = ms;
+        float total = ms;
+        for(int d = 0; d < nd; ++d)
+            if(kCand[c] < 4 * O[d]) total += best[d];
+        if(total < best_total) {
+            best_total = total;
+            best_c = c;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1045,8 +1184,9 @@ extern "C" int sw_codedisp_autotune(const sw_codedisp_config* cfg, int tune_chan
     if(rc) return rc;
     for(int d = 0; d < nd; ++d) {
         SW_CHECK_ARG(N[d] > 0, "autotune: no FFT length fits the overlap");
-        fft_len_out[d] = N[d];
+        fft_len_out[d] = (best_c >= 0 && kCand[best_c] >= 4 * O[d]) ? kCand[best_c] : N[d];
     }
+    if(share_forward_out) *share_forward_out = best_c >= 0 ? 1 : 0;
     if(ms_out) std::memcpy(ms_out, times.data(), sizeof(float) * times.size());
     return SW_OK;
 }

@@ -117,15 +117,19 @@ int dispatch(sw_idpipe* p, int bi, hipStream_t st)
         }
     }
     ++b.ndispatched;
-    // keep the last max_delay slots (AggregationBuffer::dispatch); the
-    // source [dispatch, dispatch + max_delay) and destination [0, max_delay)
-    // of every channel do not overlap (dispatch >= max_delay)
+    // keep the last max_delay slots (AggregationBuffer::dispatch): slots
+    // [dispatch, dispatch + max_delay) move to [0, max_delay), a left shift
+    // by dispatch, done as ordered launches of at most dispatch rows each,
+    // so that no launch reads a row another of its threads writes
     b.filled = 0;
     if(b.max_delay > 0) {
         const int nw = (int)(p->slot_bytes / p->nchans / 4);
         const long chan = (long)b.slots * nw;
-        rc = rows_copy(b.ring, b.ring, p->nchans, b.max_delay, nw, chan, nw, (long)b.dispatch, chan, nw, 0, st);
-        if(rc) return rc;
+        for(long r0 = 0; r0 < b.max_delay; r0 += (long)b.dispatch) {
+            const long k = std::min<long>((long)b.dispatch, b.max_delay - r0);
+            rc = rows_copy(b.ring, b.ring, p->nchans, k, nw, chan, nw, (long)b.dispatch + r0, chan, nw, r0, st);
+            if(rc) return rc;
+        }
         b.filled = b.max_delay;
     }
     b.ring_utc += b.dispatch * b.tsamp;
@@ -172,9 +176,9 @@ extern "C" int sw_idpipe_create(const sw_idpipe_config* cfg, sw_idpipe** out)
         size_t dispatch = cfg->dispatch_size > 0 ? (size_t)cfg->dispatch_size
                                                  : (size_t)std::max(b.max_delay * 10, 8192);
         dispatch -= dispatch % b.tscrunch;
-        if(dispatch == 0 || dispatch < (size_t)b.max_delay) {
+        if(dispatch == 0) {
             release(p.get());
-            sw::set_error("dispatch size must be at least tscrunch and the maximum DM delay");
+            sw::set_error("dispatch size must be at least tscrunch");
             return SW_ERR_INVALID;
         }
         b.dispatch = dispatch;

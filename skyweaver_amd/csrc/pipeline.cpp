@@ -70,6 +70,7 @@ struct sw_pipeline {
     std::vector<float> h_stats;
     bool weights_set   = false;
     bool use_mfma      = false;
+    bool scalings_dirty = true;    // recompute the scalings from this gulp's statistics
     // sinks
     sw_sink_fn cb_sink = nullptr, ib_sink = nullptr, st_sink = nullptr;
     void *cb_user = nullptr, *ib_user = nullptr, *st_user = nullptr;
@@ -160,8 +161,10 @@ int upload_beamsets(sw_pipeline* p)
     SW_HIP_TRY(hipMemcpyAsync(p->antw, p->h_antw.data(), sizeof(float) * p->nbeamsets * A,
                               hipMemcpyHostToDevice, p->stream));
     SW_HIP_TRY(hipStreamSynchronize(p->stream));
-    // scalings depend on the beamset weights: recompute on the next gulp
-    p->call_count = 0;
+    // the scalings depend on the beamset weights: recompute them from the
+    // next gulp's statistics (the gulp count, the pipeline's time base, is
+    // left alone)
+    p->scalings_dirty = true;
     return SW_OK;
 }
 
@@ -234,25 +237,29 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
     cd.dms       = p->dms.data();
     cd.fft_len   = p->cfg.fft_len;
     cd.overlap   = p->cfg.overlap;
+    cd.share_forward = c.share_forward;
     std::vector<int> tuned(c.ndms, 0);
     if(c.autotune_fft && !c.fft_len) {
         cd.input_fpat = 1;
-        int rc = sw_codedisp_autotune(&cd, 8, tuned.data(), nullptr, p->stream);
+        int share = 0;
+        int rc = sw_codedisp_autotune(&cd, 8, tuned.data(), &share, nullptr, p->stream);
         if(rc) return rc;
         cd.fft_len = tuned.data();
+        cd.share_forward = share;
+        p->cfg.share_forward = share;
     }
     cd.input_fpat = 1;
     cd.history_in_place = 1;
     int rc = sw_codedisp_create(&cd, p->stream, &p->cd);
     if(rc) return rc;
 
+    // the two FPAT gulp buffers; the host-input staging buffer (taftp_d) and
+    // the per-DM FTPA copies of the dedispersed voltages (ftpa_dd, read only
+    // by the beamformers' FTPA forms) are allocated on first use
     const size_t fb = p->ftpa_bytes();
-    SW_HIP_TRY(hipMalloc(&p->taftp_d, fb));
     SW_HIP_TRY(hipMalloc(&p->ftpa, fb));
     SW_HIP_TRY(hipMalloc(&p->ftpa_prev, fb));
-
-    p->ftpa_dd.resize(c.ndms, nullptr);
-    for(auto& b : p->ftpa_dd) SW_HIP_TRY(hipMalloc(&b, fb));
+    p->ftpa_dd.assign(c.ndms, nullptr);
     SW_HIP_TRY(hipMalloc(&p->stats, sizeof(float) * 4 * c.nchans * 2 * c.nantennas));
     SW_HIP_TRY(hipMalloc(&p->stats_ws, sw::fpat_stats_workspace(c.nchans, 2, c.nantennas, c.nsamples)));
     p->h_stats.resize((size_t)4 * c.nchans * 2 * c.nantennas);
@@ -446,6 +453,7 @@ extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t byt
     if(rc) return rc;
     const void* in = taftp;
     if(!on_device) {
+        if(!p->taftp_d) SW_HIP_TRY(hipMalloc(&p->taftp_d, p->ftpa_bytes()));
         SW_HIP_TRY(hipMemcpyAsync(p->taftp_d, taftp, bytes, hipMemcpyHostToDevice, p->stream));
         in = p->taftp_d;
     }
@@ -546,8 +554,9 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
     if(consumed) SW_HIP_TRY(hipEventRecord(consumed, st));
     SW_HIP_TRY(hipEventRecord(p->ev[1], st));
     SW_HIP_TRY(hipEventRecord(p->ev[2], st));
-    if(p->call_count == 0) {
+    if(p->scalings_dirty) {
         // update_scalings on the first gulp (BeamformerPipeline.cu:171-176)
+        // and on the first gulp after the beamsets change
         SW_HIP_TRY(hipMemcpyAsync(p->h_stats.data(), p->stats, sizeof(float) * p->h_stats.size(),
                                   hipMemcpyDeviceToHost, st));
         SW_HIP_TRY(hipStreamSynchronize(st));
@@ -557,6 +566,8 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
                                 h.data() + 3 * S * F);
         if(rc) return rc;
         SW_HIP_TRY(hipMemcpyAsync(p->scl, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, st));
+        SW_HIP_TRY(hipStreamSynchronize(st));  // h is freed at the end of this scope
+        p->scalings_dirty = false;
     }
     float* cb_offsets = p->scl;
     float* cb_scaling = p->scl + S * F;
@@ -570,6 +581,9 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
     bool ring_direct = p->use_mfma && c.stokes == SW_STOKES_I && 2 * A <= 256 && c.nsamples % 8 == 0;
     for(float w : p->h_antw) ring_direct = ring_direct && (w == 0.0f || w == 1.0f);
     const bool fuse_ib = ring_direct && sw::cb_fused_ib_supported(d);
+    if(!ring_direct)
+        for(auto& b : p->ftpa_dd)
+            if(!b) SW_HIP_TRY(hipMalloc(&b, p->ftpa_bytes()));
     std::vector<void*> outs(p->ftpa_dd.begin(), p->ftpa_dd.end());
     if(ring_direct) std::fill(outs.begin(), outs.end(), nullptr);
     rc = sw_codedisp_run(p->cd, p->ftpa, outs.data(), nullptr, 0, c.ndms, c.batched_dms, st);

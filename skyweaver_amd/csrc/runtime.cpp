@@ -238,6 +238,13 @@ extern "C" int sw_device_count(int* count)
     return SW_OK;
 }
 
+extern "C" int sw_mem_info(size_t* free_bytes, size_t* total_bytes)
+{
+    SW_CHECK_ARG(free_bytes && total_bytes, "null pointer");
+    SW_HIP_TRY(hipMemGetInfo(free_bytes, total_bytes));
+    return SW_OK;
+}
+
 extern "C" int sw_malloc(void** d_ptr, size_t bytes)
 {
     SW_CHECK_ARG(d_ptr, "null pointer");

@@ -122,21 +122,27 @@ def test_codedisp_dm0_is_identity(sw, dev):
         np.testing.assert_array_equal(outs[k].download(), exp)
 
 
-@pytest.mark.parametrize("paired", [False, True], ids=["own_overlap", "paired_overlap"])
-def test_codedisp_uneven_batch_and_history(sw, orc, dev, paired):
+@pytest.mark.parametrize("mode", ["own_overlap", "paired_overlap", "share_forward"])
+def test_codedisp_uneven_batch_and_history(sw, orc, dev, mode):
     """Eight DMs with uneven FFT lengths over two gulps: batched == one DM per
     launch bit for bit, and both match the oracle on the continuous stream.
-    paired: the DMs of each FFT length share one overlap (the larger), so the
-    batched launches share each forward transform between two DMs."""
+    paired_overlap: the DMs of each FFT length are given one overlap (the
+    larger); share_forward: the plan chooses that overlap itself.  Then the
+    batched launch of each pair of length 4096 ... 16384 transforms every
+    segment forward once for both DMs (codedisp_regm_kernel); the 32768-point
+    pair runs one DM per launch.  own_overlap: every DM has its own overlap,
+    hence its own segment grid, and runs alone."""
     from skyweaver_amd import DeviceBuffer as DB
     F, A, T = 2, 64, 24576
     dms = [0.0, 5.0, 20.0, 50.0, 100.0, 180.0, 260.0, 350.0]
     Ns = [4096, 4096, 8192, 8192, 16384, 16384, 32768, 32768]
-    ov = [2, 2, 10, 10, 32, 32, 62, 62] if paired else None
-    plan_b = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, fft_len=Ns, overlap=ov)
-    plan_u = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, fft_len=Ns, overlap=ov)
+    paired = [2, 2, 10, 10, 32, 32, 62, 62]
+    ov = paired if mode == "paired_overlap" else None
+    share = mode == "share_forward"
+    plan_b = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, fft_len=Ns, overlap=ov, share_forward=share)
+    plan_u = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, fft_len=Ns, overlap=ov, share_forward=share)
     Os = [plan_b.info(d)[1] for d in range(8)]
-    assert Os == (ov if paired else [0, 2, 4, 10, 18, 32, 46, 62])
+    assert Os == (paired if mode != "own_overlap" else [0, 2, 4, 10, 18, 32, 46, 62])
     lats = [plan_b.latency(d) for d in range(8)]
     assert lats == [0 if T % (Ns[d] - Os[d]) == 0 else Ns[d] - Os[d] for d in range(8)]
     g1 = sw.synth_voltages((F, T, 2, A, 2), seed=0x5EED0003, sigma=16.0)
@@ -205,14 +211,35 @@ def test_codedisp_autotune(sw, dev):
     with autotune_fft runs."""
     F, A, T = 4, 16, 32768
     dms = [0.0, 50.0, 350.0]
-    lens, ms = sw.CoherentDedisperser.autotune(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, tune_chans=2)
+    nd = len(dms)
+    lens, share, ms = sw.CoherentDedisperser.autotune(F, A, T, F_LOW, CHAN_BW, TSAMP, dms, tune_chans=2)
     cands = [1024, 2048, 4096, 8192, 16384, 32768]
     plan = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, dms)
-    for d in range(len(dms)):
-        O = plan.info(d)[1]
-        assert lens[d] in cands and lens[d] >= 4 * O
+    Os = [plan.info(d)[1] for d in range(nd)]
+    single = []
+    for d in range(nd):
+        assert lens[d] in cands and lens[d] >= 4 * Os[d]
         tried = [c for c in range(6) if ms[d][c] >= 0]
-        assert all(cands[c] >= 4 * O for c in tried) and ms[d][cands.index(lens[d])] == min(ms[d][c] for c in tried)
+        assert all(cands[c] >= 4 * Os[d] for c in tried)
+        single.append(min(tried, key=lambda c: ms[d][c]))
+    best_single = sum(ms[d][single[d]] for d in range(nd))
+    # the batched row: every length that fits at least two DMs was timed
+    for c in range(6):
+        fit = [d for d in range(nd) if cands[c] >= 4 * Os[d]]
+        assert (ms[nd][c] >= 0) == (len(fit) >= 2)
+    if not share:
+        assert [cands.index(x) for x in lens] == single
+    else:
+        # one length c for every DM it fits, the others at their own best,
+        # and that plan is faster than every DM at its own best
+        def plan_of(c):
+            fit = [d for d in range(nd) if cands[c] >= 4 * Os[d]]
+            return [cands[c] if d in fit else cands[single[d]] for d in range(nd)], fit
+        won = [c for c in range(6) if ms[nd][c] >= 0 and plan_of(c)[0] == lens]
+        assert won
+        fit = plan_of(won[0])[1]
+        total = ms[nd][won[0]] + sum(ms[d][single[d]] for d in range(nd) if d not in fit)
+        assert total < best_single
     from skyweaver_amd import DeviceBuffer as DB
     p = sw.BeamformerPipeline(A, F, 32, 4096, 16, dms, 1284e6, CHAN_BW * F, autotune_fft=True)
     p.set_weights(sw.synth_weights((F, 32, A, 2), seed=1))

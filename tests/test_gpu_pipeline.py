@@ -403,3 +403,65 @@ def test_pipeline_fused_incoherent_beam_shapes(sw, dev, nant, ts):
         assert len(a[key]) == len(b[key]) == 4
         for x, y in zip(a[key], b[key]):
             np.testing.assert_array_equal(x, y)
+
+
+def test_pipeline_set_beamsets_between_gulps_keeps_time_base(sw, orc, dev):
+    """Changing the beamsets after a gulp recomputes the scalings from the
+    next gulp's statistics and leaves the gulp count, the pipeline's time
+    base, alone: the second gulp's blocks carry its own utc_offset and its
+    beams equal the oracle's with scalings from its own statistics."""
+    from skyweaver_amd import DeviceBuffer as DB
+    H = T // 256
+    gulps = [sw.synth_voltages((H, A, F, 256, 2, 2), seed=0x5EED0501 + i, sigma=16.0 + 8 * i) for i in range(2)]
+    weights = sw.synth_weights((F, B, A, 2), seed=0x5EED1501)
+    utc_start, clock, total = 1708082300.0, 1712e6, 4096
+    p = sw.BeamformerPipeline(A, F, B, T, TS, [0.0], CFREQ, BW, fft_len=[1024], total_nchans=total)
+    p.set_weights(weights)
+    p.init(A, utc_start=utc_start, sample_clock=clock)
+    got = {"cb": [], "ib": [], "stats": []}
+
+    def grab(key):
+        def f(blk):
+            got[key].append((sw.download_block(blk), blk.utc_offset))
+        return f
+    p.set_sinks(cb=grab("cb"), ib=grab("ib"), stats=grab("stats"))
+    p(DB.from_host(gulps[0]))
+    bmap = (np.arange(B) % 2).astype(np.int32)
+    antw = np.ones((2, A), np.float32)
+    antw[1, 40:] = 0
+    w2 = weights.copy()
+    w2[:, 1::2, 40:] = 0
+    p.set_beamsets(bmap, antw)
+    p.set_weights(w2)
+    p(DB.from_host(gulps[1]))
+    p.close()
+    step = 2.0 * total * T / clock
+    for key in ("cb", "ib", "stats"):
+        assert [u for _, u in got[key]] == pytest.approx([0.0, step], abs=1e-12)
+    ftpa = orc.transpose(gulps[1], A)
+    st = got["stats"][1][0].view(np.float32).reshape(F, 2, A, 4)
+    scal = orc.update_scalings(st[..., 1], antw, 2, TS, TS, 24.0)
+    _, ib_q = orc.ib_beamform(ftpa, TS, scal["ib_scaling"], scal["ib_offsets"], antw, 2, "I")
+    cb = orc.cb_beamform(ftpa, w2, TS, scal["cb_scaling"], scal["cb_offsets"], antw, bmap, 2, "I", True)
+    np.testing.assert_array_equal(got["cb"][1][0].view(np.int8).reshape(cb.shape), cb)
+    np.testing.assert_array_equal(got["ib"][1][0].view(np.int8).reshape(ib_q.shape), ib_q)
+
+
+def test_pipeline_sink_exception_propagates(sw, dev):
+    """An exception raised by a Python sink stops the gulp (the C side sees a
+    non-zero return) and is re-raised by the call that processed it."""
+    from skyweaver_amd import DeviceBuffer as DB
+    H = T // 256
+    p = sw.BeamformerPipeline(A, F, B, T, TS, [0.0], CFREQ, BW, fft_len=[1024])
+    p.set_weights(sw.synth_weights((F, B, A, 2), seed=3))
+    p.init(A)
+
+    class Boom(Exception):
+        pass
+
+    def bad(blk):
+        raise Boom("sink failed")
+    p.set_sinks(cb=bad)
+    with pytest.raises(Boom, match="sink failed"):
+        p(DB.from_host(sw.synth_voltages((H, A, F, 256, 2, 2), seed=4, sigma=16.0)))
+    p.close()

@@ -69,6 +69,22 @@ def test_delay_fixture(sw, orc, dev, golden_dir):
     assert not w[:, 67:].any() and not w[:, :, 57:].any()
 
 
+def test_phase_closed_forms(sw, orc, dev):
+    """Offsets and rates that put the phase on eighths of a turn give the
+    weights the closed forms fix (test_oracle.PHASE_CASES): the GPU and the
+    oracle agree with them."""
+    from test_oracle import PHASE_CASES
+    nu, t0, dt = 1.2841e9, 1708082168.0, 16.0
+    d = np.zeros((len(PHASE_CASES), 4, 3), np.float32)
+    d[..., 0] = 1.0
+    for b, (off, rate, _) in enumerate(PHASE_CASES):
+        d[b, :, 1] = np.float32(off / nu)
+        d[b, :, 2] = np.float32(rate / (nu * dt))
+    w = check(sw, orc, d, np.array([nu]), cur=t0 + dt, ep=t0)
+    for b, (_, _, expect) in enumerate(PHASE_CASES):
+        assert (w[0, b, :, 0] == expect[0]).all() and (w[0, b, :, 1] == expect[1]).all(), (b, w[0, b, 0])
+
+
 @pytest.mark.parametrize("scale", [1.0, 0.5, -1.0])
 def test_random_delays(sw, orc, dev, scale):
     rng = np.random.default_rng(17)

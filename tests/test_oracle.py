@@ -367,6 +367,38 @@ def test_weights_zero_delays(orc):
     assert (w[..., 0] == 127).all() and (w[..., 1] == 0).all()
 
 
+# phase phi = ((t - t0) rate + offset) nu in turns; w = (rint(127 cos 2 pi phi),
+# rint(-127 sin 2 pi phi)) (WeightsManager.cu:70-79).  Offsets and rates are
+# chosen to put phi on eighths of a turn, where the weight is known without
+# computing a cosine: the cases separate the sign of the imaginary part, the
+# factor 2 pi, and the sign and scaling of the rate term.
+PHASE_CASES = [
+    # (offset turns, rate turns over dt, expected (re, im))
+    (0.0, 0.0, (127, 0)),
+    (0.25, 0.0, (0, -127)),
+    (0.5, 0.0, (-127, 0)),
+    (0.75, 0.0, (0, 127)),
+    (0.125, 0.0, (90, -90)),      # 127 / sqrt(2) = 89.80
+    (0.375, 0.0, (-90, -90)),
+    (0.0, 0.25, (0, -127)),       # rate only
+    (0.0, -0.25, (0, 127)),
+    (0.125, 0.125, (0, -127)),    # offset and rate add
+    (0.375, -0.125, (0, -127)),
+    (1.25, 0.0, (0, -127)),       # whole turns drop out
+]
+
+
+@pytest.mark.parametrize("off_turns,rate_turns,expect", PHASE_CASES)
+def test_weights_phase_closed_forms(orc, off_turns, rate_turns, expect):
+    nu, t0, dt = 1.2841e9, 1708082168.0, 16.0
+    d = np.zeros((2, 3, 3), np.float32)
+    d[..., 0] = 1.0
+    d[..., 1] = np.float32(off_turns / nu)
+    d[..., 2] = np.float32(rate_turns / (nu * dt))
+    w = orc.weights(d, np.array([nu]), t0 + dt, t0)
+    assert (w[..., 0] == expect[0]).all() and (w[..., 1] == expect[1]).all(), w[0, 0]
+
+
 def test_weights_vs_numpy(orc):
     B, A = 6, 5
     d = np.zeros((B, A, 3), np.float32)

@@ -345,6 +345,19 @@ int sw_statistics_fpat(const void* d_fpat, float* d_stats, int nchans, int nsamp
  * max(8, ceil(nheap_groups/16)) * nchans * 2 * nant_out * 32 bytes. */
 int sw_transpose_fpat_stats(const void* d_taftp, void* d_fpat, float* d_stats, int nheap_groups,
                             int nant_in, int nant_out, int nchans, void* d_workspace, void* stream);
+/* sw_transpose_fpat_stats that allocates nothing: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The power-sum partials live in a fixed 16 MiB device array that each GPU
+ * holds from library load until the process ends; results are identical to
+ * sw_transpose_fpat_stats.  The partials take 32 bytes per column
+ * (nchans * 2 * nant_out) and chunk of up to 64 heaps, i.e. at most
+ * 524288 / ceil(nheap_groups / 64) columns (32768 columns at 1024 heaps);
+ * larger shapes are rejected with SW_ERR_INVALID.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race and must be serialised by the caller. */
+int sw_transpose_fpat_stats_noalloc(const void* d_taftp, void* d_fpat, float* d_stats, int nheap_groups,
+                                    int nant_in, int nant_out, int nchans, void* stream);
 
 /* ---------------------------------------------------- BeamformerPipeline
  * BeamformerPipeline<CBH, IBH, StatsH, Traits>: ctor(PipelineConfig const&,

@@ -97,6 +97,7 @@ def _declare(L):
     L.sw_corner_turn.argtypes = [_vp, _vp, _c, _c, _c, _vp]
     L.sw_statistics_fpat.argtypes = [_vp, _vp, _c, _c, _c, _c, _vp, _vp]
     L.sw_transpose_fpat_stats.argtypes = [_vp, _vp, _vp, _c, _c, _c, _c, _vp, _vp]
+    L.sw_transpose_fpat_stats_noalloc.argtypes = [_vp, _vp, _vp, _c, _c, _c, _c, _vp]
     _pvp = ctypes.POINTER(ctypes.c_void_p)
     L.sw_codedisp_create.argtypes = [ctypes.POINTER(CodedispConfig), _vp, _pvp]
     L.sw_codedisp_destroy.argtypes = [_vp]
@@ -336,6 +337,13 @@ def transpose_fpat_stats(taftp, fpat, stats, nheaps, nant_in, nant_out, nchans, 
     """TAFTP -> FPAT fused with the statistics (one pass over the gulp)."""
     check(lib().sw_transpose_fpat_stats(_ptr(taftp), _ptr(fpat), _ptr(stats), nheaps, nant_in, nant_out, nchans,
                                         _ptr(workspace), _stream(stream)))
+
+
+def transpose_fpat_stats_noalloc(taftp, fpat, stats, nheaps, nant_in, nant_out, nchans, stream=None):
+    """sw_transpose_fpat_stats with its scratch in the library's fixed device
+    array: no device allocation."""
+    check(lib().sw_transpose_fpat_stats_noalloc(_ptr(taftp), _ptr(fpat), _ptr(stats), nheaps, nant_in, nant_out,
+                                                nchans, _stream(stream)))
 
 
 def corner_turn(src, dst, nchans, rows, cols, stream=None):

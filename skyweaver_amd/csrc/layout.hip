@@ -14,6 +14,9 @@
 //     integer sums and FP64 finalisation as statistics.hip.
 #include "sw_common.hpp"
 
+#include <mutex>
+#include <string>
+
 namespace sw {
 namespace {
 
@@ -169,7 +172,11 @@ fpat_stats_partial(const uint32_t* __restrict__ tm, long long* __restrict__ part
 // a heap's x^4 sum <= 8 x 2^28 = 2^31 (unsigned), so the 32-bit partials are
 // exact; the wave reduction and the x^4 total run in 64 bits.  Partials go
 // to part[chunk][col][4] (col = (f*2 + p)*A + a) for fpat_stats_finalize.
+// A chunk may also span up to kFusedHeapsMax heaps (|sum x^3| <= 2^30 per
+// lane and pol), which the no-allocation entry point uses to fit its fixed
+// scratch array.
 constexpr int kFusedHeaps = 16;
+constexpr int kFusedHeapsMax = 64;
 
 typedef short sw_short2 __attribute__((ext_vector_type(2)));
 typedef unsigned short sw_ushort2 __attribute__((ext_vector_type(2)));
@@ -198,7 +205,7 @@ __device__ __forceinline__ long long wave_sum(long long v)
 
 __global__ void __launch_bounds__(256)
 taftp_to_fpat_stats_kernel(const uint4* __restrict__ in, uint2* __restrict__ out, long long* __restrict__ part,
-                           int nant_in, int nant_out, int nchans, int nheaps, int nchunks)
+                           int nant_in, int nant_out, int nchans, int nheaps, int nchunks, int chunk_heaps)
 {
     const int lane    = threadIdx.x & 63;
     const size_t wave = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -207,7 +214,7 @@ taftp_to_fpat_stats_kernel(const uint4* __restrict__ in, uint2* __restrict__ out
     const size_t r  = wave / nchans;
     const int a     = (int)(r % nant_out);
     const int chunk = (int)(r / nant_out);
-    const int h0 = chunk * kFusedHeaps, h1 = min(nheaps, h0 + kFusedHeaps);
+    const int h0 = chunk * chunk_heaps, h1 = min(nheaps, h0 + chunk_heaps);
     const size_t col_words = (size_t)nheaps * 64;  // one column: T samples = T/4 uint2
     uint2* o0 = out + (((size_t)f * 2 + 0) * nant_out + a) * col_words + lane;
     uint2* o1 = out + (((size_t)f * 2 + 1) * nant_out + a) * col_words + lane;
@@ -343,8 +350,11 @@ extern "C" int sw_transpose_fpat(const void* d_taftp, void* d_fpat, int nheap_gr
     return SW_OK;
 }
 
-extern "C" int sw_transpose_fpat_stats(const void* d_taftp, void* d_fpat, float* d_stats, int nheap_groups,
-                                      int nant_in, int nant_out, int nchans, void* d_workspace, void* stream)
+namespace sw {
+namespace {
+
+int check_transpose_fpat_stats(const void* d_taftp, const void* d_fpat, const float* d_stats, int nheap_groups,
+                               int nant_in, int nant_out, int nchans)
 {
     SW_CHECK_ARG(d_taftp && d_fpat && d_stats, "null buffer");
     SW_CHECK_ARG(nheap_groups > 0 && nchans > 0, "empty input");
@@ -353,22 +363,100 @@ extern "C" int sw_transpose_fpat_stats(const void* d_taftp, void* d_fpat, float*
     SW_CHECK_ARG((reinterpret_cast<uintptr_t>(d_taftp) & 15) == 0 &&
                      (reinterpret_cast<uintptr_t>(d_fpat) & 7) == 0,
                  "input must be 16-byte and output 8-byte aligned");
-    const int nsamples = nheap_groups * 256;
-    SW_CHECK_ARG((size_t)nsamples * 2 < (1ull << 22), "too many samples per call for exact sums");
-    void* ws = d_workspace ? d_workspace : sw::scratch(sw::fpat_stats_workspace(nchans, 2, nant_out, nsamples), 2);
-    if(!ws) return SW_ERR_ALLOC;
-    const int nchunks = (nheap_groups + sw::kFusedHeaps - 1) / sw::kFusedHeaps;
+    SW_CHECK_ARG((size_t)nheap_groups * 256 * 2 < (1ull << 22), "too many samples per call for exact sums");
+    return SW_OK;
+}
+
+// the fused pass with power-sum partials in ws, chunks of chunk_heaps heaps
+int launch_transpose_fpat_stats(const void* d_taftp, void* d_fpat, float* d_stats, int nheap_groups, int nant_in,
+                                int nant_out, int nchans, void* ws, int chunk_heaps, hipStream_t st)
+{
+    const int nchunks = (nheap_groups + chunk_heaps - 1) / chunk_heaps;
     const size_t nwaves = (size_t)nchunks * nant_out * nchans;
-    hipLaunchKernelGGL(sw::taftp_to_fpat_stats_kernel, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0,
-                       sw::as_stream(stream), static_cast<const uint4*>(d_taftp), static_cast<uint2*>(d_fpat),
-                       static_cast<long long*>(ws), nant_in, nant_out, nchans, nheap_groups, nchunks);
+    hipLaunchKernelGGL(taftp_to_fpat_stats_kernel, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, st,
+                       static_cast<const uint4*>(d_taftp), static_cast<uint2*>(d_fpat), static_cast<long long*>(ws),
+                       nant_in, nant_out, nchans, nheap_groups, nchunks, chunk_heaps);
     SW_LAUNCH_CHECK();
     const int ncols = nchans * 2 * nant_out;
-    hipLaunchKernelGGL(sw::fpat_stats_finalize, dim3((ncols + 255) / 256), dim3(256), 0, sw::as_stream(stream),
+    hipLaunchKernelGGL(fpat_stats_finalize, dim3((ncols + 255) / 256), dim3(256), 0, st,
                        static_cast<const long long*>(ws), reinterpret_cast<float4*>(d_stats), nchunks, ncols,
-                       (long long)nsamples * 2);
+                       (long long)nheap_groups * 256 * 2);
     SW_LAUNCH_CHECK();
     return SW_OK;
+}
+
+// Scratch of sw_transpose_fpat_stats_noalloc: the power-sum partials
+// [chunk][col][4] int64.  One copy per GPU, held until the process ends;
+// kernels get its address from hipGetSymbolAddress (noalloc_scratch_ptr),
+// never from the host-side name.
+constexpr size_t kNoallocScratchBytes = (size_t)16 << 20;
+__device__ __align__(256) long long noalloc_scratch[kNoallocScratchBytes / sizeof(long long)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+// the scratch array's device address on the current device (looked up once per device)
+int noalloc_scratch_ptr(void** out)
+{
+    int dev = 0;
+    SW_HIP_TRY(hipGetDevice(&dev));
+    SW_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if(!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        SW_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+        SW_CHECK_ARG(p != nullptr, "hipGetSymbolAddress returned a null address");
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return SW_OK;
+}
+
+} // namespace
+} // namespace sw
+
+extern "C" int sw_transpose_fpat_stats(const void* d_taftp, void* d_fpat, float* d_stats, int nheap_groups,
+                                      int nant_in, int nant_out, int nchans, void* d_workspace, void* stream)
+{
+    int rc = sw::check_transpose_fpat_stats(d_taftp, d_fpat, d_stats, nheap_groups, nant_in, nant_out, nchans);
+    if(rc) return rc;
+    const int nsamples = nheap_groups * 256;
+    void* ws = d_workspace ? d_workspace : sw::scratch(sw::fpat_stats_workspace(nchans, 2, nant_out, nsamples), 2);
+    if(!ws) return SW_ERR_ALLOC;
+    return sw::launch_transpose_fpat_stats(d_taftp, d_fpat, d_stats, nheap_groups, nant_in, nant_out, nchans, ws,
+                                           sw::kFusedHeaps, sw::as_stream(stream));
+}
+
+extern "C" int sw_transpose_fpat_stats_noalloc(const void* d_taftp, void* d_fpat, float* d_stats, int nheap_groups,
+                                              int nant_in, int nant_out, int nchans, void* stream)
+{
+    int rc = sw::check_transpose_fpat_stats(d_taftp, d_fpat, d_stats, nheap_groups, nant_in, nant_out, nchans);
+    if(rc) return rc;
+    // the shortest chunks (most parallel waves) whose partials fit the scratch
+    const size_t ncols = (size_t)nchans * 2 * nant_out;
+    int chunk_heaps = 0;
+    for(int h = sw::kFusedHeaps; h <= sw::kFusedHeapsMax; h *= 2) {
+        const size_t nchunks = (size_t)(nheap_groups + h - 1) / h;
+        if(nchunks * ncols * 4 * sizeof(long long) <= sw::kNoallocScratchBytes) {
+            chunk_heaps = h;
+            break;
+        }
+    }
+    if(chunk_heaps == 0) {
+        const size_t most = sw::kNoallocScratchBytes /
+                            ((size_t)(nheap_groups + sw::kFusedHeapsMax - 1) / sw::kFusedHeapsMax * 4 * sizeof(long long));
+        sw::set_error("sw_transpose_fpat_stats_noalloc: " + std::to_string(ncols) + " columns (nchans * 2 * nant_out) of " +
+                      std::to_string(nheap_groups) + " heaps exceed its " +
+                      std::to_string(sw::kNoallocScratchBytes >> 20) + " MiB scratch, which holds at most " +
+                      std::to_string(most) + " columns at this gulp length");
+        return SW_ERR_INVALID;
+    }
+    void* ws = nullptr;
+    rc = sw::noalloc_scratch_ptr(&ws);
+    if(rc) return rc;
+    return sw::launch_transpose_fpat_stats(d_taftp, d_fpat, d_stats, nheap_groups, nant_in, nant_out, nchans, ws,
+                                           chunk_heaps, sw::as_stream(stream));
 }
 
 extern "C" int sw_corner_turn(const void* d_src, void* d_dst, int nchans, int rows, int cols,

@@ -468,6 +468,18 @@ typedef struct sw_pipeline_config {
     int autotune_fft;
     /* sw_codedisp_config::share_forward for the pipeline's dedisperser */
     int share_forward;
+    /* 1: the CB, IB and statistics sinks are called by a library thread, in
+     * the same order as synchronous sinks, once the blocks are written, while
+     * the pipeline goes on with the next DMs and gulps; each DM's outputs and
+     * the statistics alternate between two buffers (a block stays valid until
+     * its sink returns; the pipeline waits before reusing a buffer whose block
+     * is not yet delivered).  A sink's failure is returned by the next
+     * sw_pipeline_process / process_prefetched / synchronize call, which
+     * also drops the blocks queued after it.  sw_pipeline_synchronize waits
+     * until every queued block has been delivered.  0: the sinks are called
+     * synchronously within sw_pipeline_process, as the reference's handlers
+     * (cpp/skyweaver/detail/BeamformerPipeline.cu:240-257). */
+    int async_sinks;
 } sw_pipeline_config;
 
 typedef struct sw_obs_header {

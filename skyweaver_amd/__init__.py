@@ -628,7 +628,7 @@ class PipelineConfigC(ctypes.Structure):
                 ("overlap", ctypes.POINTER(ctypes.c_int)), ("cb_form", ctypes.c_int), ("batched_dms", ctypes.c_int),
                 ("id_enable", ctypes.c_int), ("id_tscrunch", ctypes.c_void_p), ("id_dms", ctypes.c_void_p),
                 ("id_ndms", ctypes.c_void_p), ("id_dispatch_size", ctypes.c_int), ("autotune_fft", ctypes.c_int),
-                ("share_forward", ctypes.c_int)]
+                ("share_forward", ctypes.c_int), ("async_sinks", ctypes.c_int)]
 
 
 class ObsHeaderC(ctypes.Structure):
@@ -706,10 +706,12 @@ class BeamformerPipeline:
     def __init__(self, nantennas, nchans, nbeams, nsamples, tscrunch, dms, centre_frequency, bandwidth,
                  stokes="I", ib_subtract=True, output_level=24.0, fft_len=None, overlap=None, form="mfma",
                  batched=True, total_nchans=4096, id_plan=None, id_dispatch_size=0, autotune_fft=False,
-                 share_forward=False):
+                 share_forward=False, async_sinks=False):
         """id_plan: None (no incoherent dedispersion) or, per coherent DM, (tscrunch, incoherent DMs).
         autotune_fft (with fft_len None): FFT lengths and share_forward chosen by sw_codedisp_autotune.
-        share_forward: DMs of equal FFT length share one overlap (and, batched, their forward FFTs)."""
+        share_forward: DMs of equal FFT length share one overlap (and, batched, their forward FFTs).
+        async_sinks: sinks are called by a library thread while the pipeline goes on (blocks of each
+        DM alternate between two buffers); a sink's exception is raised by the next call."""
         import numpy as np
         L = lib()
         if not hasattr(L, "_pipeline_declared"):
@@ -728,6 +730,7 @@ class BeamformerPipeline:
                               FORMS[form], 1 if batched else 0)
         cfg.autotune_fft = 1 if autotune_fft else 0
         cfg.share_forward = 1 if share_forward else 0
+        cfg.async_sinks = 1 if async_sinks else 0
         if id_plan is not None:
             assert len(id_plan) == n
             self._id_ts = np.array([b[0] for b in id_plan], np.int32)
@@ -802,7 +805,9 @@ class BeamformerPipeline:
         return False
 
     def synchronize(self):
-        check(lib().sw_pipeline_synchronize(self.handle))
+        """Wait for every copy and kernel enqueued and, with asynchronous sinks,
+        every block's delivery."""
+        self._errors.check(lib().sw_pipeline_synchronize(self.handle))
 
     def stage_times(self):
         buf = (ctypes.c_float * len(STAGES))()

@@ -25,9 +25,14 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -42,6 +47,106 @@ int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_pa
                           const float* d_ib_scale, const float* d_ib_offset, const float* d_ant_weights);
 int cb_fused_ib_supported(const sw_bf_dims& d);
 }
+
+namespace {
+
+// Asynchronous sink delivery: blocks are handed to the sinks by one library
+// thread, in the order the synchronous pipeline would call them, once the
+// event recorded after the kernels that wrote them has completed.  A buffer
+// counts its undelivered blocks; the pipeline waits for that count to reach
+// zero before it writes the buffer again.
+struct SinkWorker {
+    struct Job {
+        sw_sink_fn fn;
+        void* user;
+        sw_block blk;
+        hipEvent_t ready;
+        const void* buf;
+    };
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv_job, cv_done;
+    std::deque<Job> q;
+    std::map<const void*, int> pending;  // buffer -> blocks not yet delivered
+    bool stop = false;
+    int rc = SW_OK;  // first failure (later blocks are dropped)
+    std::string msg;
+
+    void start() { th = std::thread([this] { run(); }); }
+    void run()
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        for(;;) {
+            cv_job.wait(lk, [this] { return stop || !q.empty(); });
+            if(q.empty()) return;  // stop requested and drained
+            Job j = q.front();
+            q.pop_front();
+            const bool failed = rc != SW_OK;
+            lk.unlock();
+            int r = SW_OK;
+            std::string m;
+            if(!failed) {
+                const hipError_t e = hipEventSynchronize(j.ready);
+                if(e != hipSuccess) {
+                    r = SW_ERR_HIP;
+                    m = std::string("sink worker: hipEventSynchronize failed: ") + hipGetErrorString(e);
+                } else if(const int h = j.fn(j.user, &j.blk); h != 0) {
+                    r = SW_ERR_IO;
+                    m = "sw_pipeline_process: handler returned " + std::to_string(h);
+                }
+            }
+            lk.lock();
+            if(r != SW_OK && rc == SW_OK) {
+                rc = r;
+                msg = m;
+            }
+            if(--pending[j.buf] == 0) pending.erase(j.buf);
+            cv_done.notify_all();
+        }
+    }
+    void push(const Job& j)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        q.push_back(j);
+        ++pending[j.buf];
+        cv_job.notify_one();
+    }
+    // wait until no block of `buf` is waiting for its sink
+    void wait_free(const void* buf)
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_done.wait(lk, [&] { return pending.find(buf) == pending.end(); });
+    }
+    void drain()
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_done.wait(lk, [&] { return q.empty() && pending.empty(); });
+    }
+    // the first failure, reported once on the calling thread
+    int take_error()
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        const int r = rc;
+        if(r != SW_OK) {
+            sw::set_error(msg);
+            rc = SW_OK;
+        }
+        return r;
+    }
+    ~SinkWorker()
+    {
+        if(th.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                stop = true;
+            }
+            cv_job.notify_all();
+            th.join();
+        }
+    }
+};
+
+} // namespace
 
 struct sw_pipeline {
     sw_pipeline_config cfg{};
@@ -93,6 +198,13 @@ struct sw_pipeline {
     hipEvent_t ev_w[2]{};            // weights generation
     std::vector<hipEvent_t> ev_dm;  // per DM: ib start, cb start, cb end, id end
     float stage_ms[SW_NSTAGES]{};
+    // asynchronous sinks (cfg.async_sinks): the coherent and incoherent beam
+    // outputs of each DM and the statistics have two buffers, used by
+    // alternate gulps, and an event each recorded once they are written
+    std::unique_ptr<SinkWorker> worker;
+    std::vector<int8_t*> cb_bufs, ib_bufs;  // [dm][2]
+    float* stats_bufs[2]{};
+    std::vector<hipEvent_t> ev_ready;       // [dm][2], then the statistics' [2]
     // prefetched input (sw_pipeline_prefetch): two device gulp buffers filled
     // on a copy stream while the pipeline stream works on the previous gulp
     hipStream_t copy_stream = nullptr;
@@ -108,6 +220,19 @@ struct sw_pipeline {
     ~sw_pipeline()
     {
         if(stream) (void)hipStreamSynchronize(stream);
+        if(worker) {
+            worker->drain();
+            worker.reset();
+        }
+        if(!cb_bufs.empty()) {  // cb_out, ib_q and stats point into these
+            for(auto* b : cb_bufs) (void)hipFree(b);
+            for(auto* b : ib_bufs) (void)hipFree(b);
+            for(auto* b : stats_bufs) (void)hipFree(b);
+            cb_out = nullptr;
+            ib_q = nullptr;
+            stats = nullptr;
+        }
+        for(auto e : ev_ready) if(e) (void)hipEventDestroy(e);
         if(cd) sw_codedisp_destroy(cd);
         if(idp) sw_idpipe_destroy(idp);
         if(delays) sw_delay_model_close(delays);
@@ -264,10 +389,23 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
     SW_HIP_TRY(hipMalloc(&p->stats_ws, sw::fpat_stats_workspace(c.nchans, 2, c.nantennas, c.nsamples)));
     p->h_stats.resize((size_t)4 * c.nchans * 2 * c.nantennas);
     const int nv = p->nv();
-    SW_HIP_TRY(hipMalloc(&p->cb_out, p->tout() * c.nchans * c.nbeams * nv));
     // incoherent beam outputs sized for the largest supported beamset count
     SW_HIP_TRY(hipMalloc(&p->ib_raw, sizeof(float) * 16 * p->tout() * c.nchans * nv));
-    SW_HIP_TRY(hipMalloc(&p->ib_q, (size_t)16 * p->tout() * c.nchans * nv));
+    if(c.async_sinks) {
+        p->cb_bufs.assign(2 * c.ndms, nullptr);
+        p->ib_bufs.assign(2 * c.ndms, nullptr);
+        for(auto& b : p->cb_bufs) SW_HIP_TRY(hipMalloc(&b, p->tout() * c.nchans * c.nbeams * nv));
+        for(auto& b : p->ib_bufs) SW_HIP_TRY(hipMalloc(&b, (size_t)16 * p->tout() * c.nchans * nv));
+        p->stats_bufs[0] = p->stats;
+        SW_HIP_TRY(hipMalloc(&p->stats_bufs[1], sizeof(float) * 4 * c.nchans * 2 * c.nantennas));
+        p->ev_ready.assign(2 * c.ndms + 2, nullptr);
+        for(auto& e : p->ev_ready) SW_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        p->worker = std::make_unique<SinkWorker>();
+        p->worker->start();
+    } else {
+        SW_HIP_TRY(hipMalloc(&p->cb_out, p->tout() * c.nchans * c.nbeams * nv));
+        SW_HIP_TRY(hipMalloc(&p->ib_q, (size_t)16 * p->tout() * c.nchans * nv));
+    }
     SW_HIP_TRY(hipMalloc(&p->weights, (size_t)c.nchans * c.nbeams * c.nantennas * 2));
     SW_HIP_TRY(hipMalloc(&p->packed, (size_t)c.nchans * c.nbeams * c.nantennas * 4));
     // one beamset of all antennas until told otherwise
@@ -503,6 +641,10 @@ extern "C" int sw_pipeline_synchronize(sw_pipeline* p)
     SW_CHECK_ARG(p, "null argument");
     if(p->copy_stream) SW_HIP_TRY(hipStreamSynchronize(p->copy_stream));
     SW_HIP_TRY(hipStreamSynchronize(p->stream));
+    if(p->worker) {
+        p->worker->drain();
+        return p->worker->take_error();
+    }
     return SW_OK;
 }
 
@@ -518,6 +660,14 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
     const double utc_offset_gulp =
         p->header.utc_offset + (double)p->call_count * p->tick_per_block / p->header.sample_clock;
     int rc;
+    SinkWorker* wk = p->worker.get();
+    const int par = (int)(p->call_count & 1);  // the buffer set of this gulp (asynchronous sinks)
+    if(wk) {
+        rc = wk->take_error();  // a sink of an earlier gulp failed
+        if(rc) return rc;
+        p->stats = p->stats_bufs[par];
+        wk->wait_free(p->stats);
+    }
     SW_HIP_TRY(hipEventRecord(p->ev_w[0], st));
     if(p->delays) {
         // DelayManager::delays + WeightsManager::weights at the gulp's unix time
@@ -552,6 +702,7 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
     rc = sw_transpose_fpat_stats(in, p->ftpa, p->stats, c.nsamples / 256, nant_in, A, F, p->stats_ws, st);
     if(rc) return rc;
     if(consumed) SW_HIP_TRY(hipEventRecord(consumed, st));
+    if(wk) SW_HIP_TRY(hipEventRecord(p->ev_ready[2 * c.ndms + par], st));
     SW_HIP_TRY(hipEventRecord(p->ev[1], st));
     SW_HIP_TRY(hipEventRecord(p->ev[2], st));
     if(p->scalings_dirty) {
@@ -598,6 +749,12 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
         int fl = 0, ov = 0, lat = 0;
         sw_codedisp_info(p->cd, dm, &fl, &ov, &lat);
         const double tsamp = c.nchans / c.bandwidth;
+        if(wk) {  // this DM's output buffers of this gulp parity, once their last blocks are delivered
+            p->cb_out = p->cb_bufs[2 * dm + par];
+            p->ib_q = p->ib_bufs[2 * dm + par];
+            wk->wait_free(p->cb_out);
+            wk->wait_free(p->ib_q);
+        }
         SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm], st));
         const void* ring = nullptr;
         int ld = 0, w0 = 0;
@@ -633,15 +790,23 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
         blk.reference_dm = p->dms[dm];
         blk.tsamp = tsamp * c.tscrunch;
         blk.utc_offset = utc;
-        rc = call_sink(p, p->cb_sink, p->cb_user, blk);
-        if(rc) return rc;
-        fill_block(blk, SW_BLOCK_IB_BTF, p->ib_q, (size_t)S * T2 * F * nv, "BTF", {(size_t)S, T2, (size_t)F});
-        blk.dm_idx = dm;
-        blk.reference_dm = p->dms[dm];
-        blk.tsamp = tsamp * c.tscrunch;
-        blk.utc_offset = utc;
-        rc = call_sink(p, p->ib_sink, p->ib_user, blk);
-        if(rc) return rc;
+        sw_block iblk;
+        fill_block(iblk, SW_BLOCK_IB_BTF, p->ib_q, (size_t)S * T2 * F * nv, "BTF", {(size_t)S, T2, (size_t)F});
+        iblk.dm_idx = dm;
+        iblk.reference_dm = p->dms[dm];
+        iblk.tsamp = tsamp * c.tscrunch;
+        iblk.utc_offset = utc;
+        if(wk) {
+            hipEvent_t ready = p->ev_ready[2 * dm + par];
+            SW_HIP_TRY(hipEventRecord(ready, st));
+            if(p->cb_sink) wk->push({p->cb_sink, p->cb_user, blk, ready, p->cb_out});
+            if(p->ib_sink) wk->push({p->ib_sink, p->ib_user, iblk, ready, p->ib_q});
+        } else {
+            rc = call_sink(p, p->cb_sink, p->cb_user, blk);
+            if(rc) return rc;
+            rc = call_sink(p, p->ib_sink, p->ib_user, iblk);
+            if(rc) return rc;
+        }
         if(p->idp) {
             rc = sw_idpipe_push(p->idp, dm, p->cb_out, T2, 1, utc, tsamp * c.tscrunch, st);
             if(rc) return rc;
@@ -655,8 +820,12 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
                    {(size_t)F, 2, (size_t)A});
         blk.tsamp = c.nchans / c.bandwidth * c.nsamples;
         blk.utc_offset = utc_offset_gulp;
-        rc = call_sink(p, p->st_sink, p->st_user, blk);
-        if(rc) return rc;
+        if(wk) {
+            if(p->st_sink) wk->push({p->st_sink, p->st_user, blk, p->ev_ready[2 * c.ndms + par], p->stats});
+        } else {
+            rc = call_sink(p, p->st_sink, p->st_user, blk);
+            if(rc) return rc;
+        }
     }
     SW_HIP_TRY(hipStreamSynchronize(st));
     // per-stage device times of this gulp

@@ -465,3 +465,68 @@ def test_pipeline_sink_exception_propagates(sw, dev):
     with pytest.raises(Boom, match="sink failed"):
         p(DB.from_host(sw.synth_voltages((H, A, F, 256, 2, 2), seed=4, sigma=16.0)))
     p.close()
+
+
+def test_pipeline_async_sinks_match_synchronous(sw, dev):
+    """Sinks called by the library's thread (async_sinks) receive the same
+    blocks, in the same order and with the same metadata, as synchronous
+    sinks, also when they are slow enough that the pipeline must wait for a
+    block's delivery before reusing its buffer (four gulps, two buffers per
+    DM); synchronize() returns once every block is delivered."""
+    import time
+    from skyweaver_amd import DeviceBuffer as DB
+    H = T // 256
+    gulps = [DB.from_host(sw.synth_voltages((H, A, F, 256, 2, 2), seed=0x5EED0601 + i, sigma=16.0)) for i in range(4)]
+    weights = sw.synth_weights((F, B, A, 2), seed=0x5EED1601)
+    dms, fft_len = [0.0, 20.0, 50.0], [1024, 2048, 1024]
+
+    def run(async_sinks, delay):
+        p = sw.BeamformerPipeline(A, F, B, T, TS, dms, CFREQ, BW, fft_len=fft_len, async_sinks=async_sinks)
+        p.set_weights(weights)
+        p.init(A)
+        got = []
+
+        def grab(key):
+            def f(blk):
+                if delay:
+                    time.sleep(delay)
+                got.append((key, blk.dm_idx, blk.utc_offset, blk.tsamp, sw.download_block(blk)))
+            return f
+        p.set_sinks(cb=grab("cb"), ib=grab("ib"), stats=grab("stats"))
+        for g in gulps:
+            p(g)
+        p.synchronize()
+        n = len(got)
+        p.close()
+        return got, n
+
+    ref, _ = run(False, 0.0)
+    for delay in (0.0, 0.02):
+        got, n = run(True, delay)
+        assert n == len(ref) == 4 * (2 * len(dms) + 1)
+        for x, y in zip(ref, got):
+            assert x[:4] == y[:4]
+            np.testing.assert_array_equal(x[4], y[4])
+
+
+def test_pipeline_async_sink_exception_propagates(sw, dev):
+    """With asynchronous sinks a sink's exception is raised by a later call
+    into the pipeline: the process call itself if the library's thread has
+    already delivered the block by the time it returns, else synchronize."""
+    from skyweaver_amd import DeviceBuffer as DB
+    H = T // 256
+    p = sw.BeamformerPipeline(A, F, B, T, TS, [0.0], CFREQ, BW, fft_len=[1024], async_sinks=True)
+    p.set_weights(sw.synth_weights((F, B, A, 2), seed=5))
+    p.init(A)
+
+    class Boom(Exception):
+        pass
+
+    def bad(blk):
+        raise Boom("late sink failed")
+    p.set_sinks(ib=bad)
+    g = DB.from_host(sw.synth_voltages((H, A, F, 256, 2, 2), seed=6, sigma=16.0))
+    with pytest.raises(Boom, match="late sink failed"):
+        p(g)
+        p.synchronize()
+    p.close()

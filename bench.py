@@ -266,6 +266,24 @@ def bench_idisp(sw, steps):
             "xrealtime": round(disp * tsamp_cb / (ms * 1e-3), 2)}
 
 
+def bench_writer_sinks(ngulps=6):
+    """Gulp time with the C++ facade's MultiFileWriter handlers attached,
+    called synchronously (the reference's order, within each gulp) and by
+    the library's sink thread (async handlers), beside no handlers
+    (tests/cpp/writer_sinks_bench.cpp; files go to a scratch directory under
+    the system temporary directory and are removed)."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "writer_sinks_bench")
+    if not os.path.exists(exe):
+        return {"skipped": "tests/cpp/_build/writer_sinks_bench is not built"}
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([exe, d, str(ngulps)], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        return {"error": r.stderr.strip()[-400:]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def bench_config5(sw, steps, warmup):
     """The full pipeline with incoherent dedispersion at its full size: 64
     antennas x 256 channels (53.5 MHz at 1284 MHz) x 262144-sample gulps,
@@ -337,6 +355,7 @@ def main():
     ap.add_argument("--no-8dm", action="store_true", help="skip the 8-DM chosen vs plain measurement")
     ap.add_argument("--no-config5", action="store_true", help="skip the full-size 256-channel pipeline with "
                     "incoherent dedispersion")
+    ap.add_argument("--no-writers", action="store_true", help="skip the gulp time with DADA writers attached")
     ap.add_argument("--only-8dm", action="store_true",
                     help="time only the chosen form on the 8-DM workload (for kernel traces) and print its line")
     args = ap.parse_args()
@@ -357,8 +376,9 @@ def main():
     d_taftp = DeviceBuffer.from_host(taftp)
     if args.only_8dm:
         ms8, _, st8, _ = run_form(sw, d_taftp, weights, "mfma", True, args.steps, args.warmup, dms=DMS8,
-                                  fft_len=FFT_LEN8)
-        print(json.dumps({"workload": f"8 coherent DMs {DMS8} (FFT {FFT_LEN8}), chosen form",
+                                  fft_len=FFT_LEN8, share_forward=True)
+        print(json.dumps({"workload": f"8 coherent DMs {DMS8} (FFT {FFT_LEN8}, shared forward transforms), "
+                                     "chosen form",
                           "ms_per_step": round(ms8, 4), "stage_ms": {k: round(v, 4) for k, v in st8.items()}}))
         return
 
@@ -454,6 +474,11 @@ def main():
         log("full pipeline, 256 channels x 262144 samples, 2 coherent DMs x 1024 incoherent DMs")
         config5 = bench_config5(sw, max(3, args.steps // 5), 2)
 
+    writers = None
+    if not args.no_writers:
+        log("DADA writers attached: synchronous vs asynchronous handlers")
+        writers = bench_writer_sinks()
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -506,6 +531,7 @@ def main():
         "target_8dm_autotuned_fft": target_tuned,
         "incoherent_dedispersion": idisp,
         "config5_full_pipeline": config5,
+        "writer_sinks": writers,
         "host_input": host_input,
         "cpu_baseline": cpu,
     }

@@ -154,6 +154,16 @@ int sw_cb_beamform_packed(const void* d_ftpa, const void* d_packed,
                           const float* d_scale, const float* d_offset,
                           const int* d_beamset_map, const float* d_ib_raw,
                           void* d_out, const sw_bf_dims* dims, void* stream);
+/* The same with the workgroup shape chosen: `waves` (2, 4 or 8) waves of 32
+ * time rows per workgroup, each wave sweeping all beam tiles of its rows
+ * (sw_cb_beamform_packed: 4).  The reference's launch shape is fixed
+ * (cpp/skyweaver/src/CoherentBeamformer.cu:241-267); here it is a tuning
+ * parameter (sw_pipeline_autotune).  Stokes I and IQUV take all three
+ * shapes; Q, U and V only 4. */
+int sw_cb_beamform_packed_tile(const void* d_ftpa, const void* d_packed,
+                               const float* d_scale, const float* d_offset,
+                               const int* d_beamset_map, const float* d_ib_raw,
+                               void* d_out, const sw_bf_dims* dims, int waves, void* stream);
 
 /* ----------------------------------------------------------- WeightsManager
  * WeightsManager::weights(delays, current_epoch, delay_epoch) and
@@ -480,6 +490,22 @@ typedef struct sw_pipeline_config {
      * synchronously within sw_pipeline_process, as the reference's handlers
      * (cpp/skyweaver/detail/BeamformerPipeline.cu:240-257). */
     int async_sinks;
+    /* 1 (needs async_sinks): after the first gulp, each gulp's device work
+     * (transpose with the statistics' sums, dedispersion of every DM, every
+     * DM's beams) is captured into a hip graph and launched as one; the
+     * executable graph is updated in place from gulp to gulp.  The blocks are
+     * released to the sinks, and the incoherent dedispersion is fed, after
+     * the graph; results are identical to the stream launches.  Stage times
+     * then report the whole graph as "other". */
+    int graph;
+    /* Coherent beamformer workgroup shape (sw_cb_beamform_packed_tile): 2, 4
+     * or 8 waves of 32 time rows; 0: the tuning table's entry for this
+     * workload, else 4.  When fft_len is NULL and autotune_fft is 0, a table
+     * entry with the same DM list also gives the FFT lengths and
+     * share_forward. */
+    int cb_waves;
+    const char* tuning_file; /* NULL: $SKYWEAVER_AMD_TUNING, else tuning/gfx950.json
+                                in the library's package directory */
 } sw_pipeline_config;
 
 typedef struct sw_obs_header {
@@ -541,6 +567,29 @@ int sw_pipeline_synchronize(sw_pipeline* pipeline);
 int sw_pipeline_stage_times(const sw_pipeline* pipeline, float* ms, int n);
 /* The processing stream (hipStream_t) the pipeline enqueues work on. */
 void* sw_pipeline_stream(const sw_pipeline* pipeline);
+/* The tiles the pipeline runs with: the coherent beamformer's waves per
+ * workgroup, each coherent DM's FFT length (fft_len [ndms], NULL allowed),
+ * share_forward, and where they came from (*source: 0 defaults, 1 the
+ * tuning table's beamformer entry, 2 its entry with FFT lengths too, 3 the
+ * FFT autotuner at create). */
+int sw_pipeline_tile(const sw_pipeline* pipeline, int* cb_waves, int* fft_len, int* share_forward, int* source);
+
+/* ------------------------------------------------------------- tuning
+ * Tile autotuner (no reference counterpart: the reference's launch shapes
+ * are fixed, cpp/skyweaver/src/CoherentBeamformer.cu:241-267, and its FFT
+ * length is the gulp).  For the workload of cfg on min(tune_chans, nchans)
+ * channels of synthetic device input, times the coherent and incoherent
+ * beams with 2, 4 and 8 waves per workgroup and, when cfg->fft_len is NULL,
+ * runs sw_codedisp_autotune; writes the result as one JSON object (an
+ * "entries" item of the tuning table, with the measured times) into json
+ * (json_len bytes, NUL-terminated). */
+int sw_pipeline_autotune(const sw_pipeline_config* cfg, int tune_chans, char* json, size_t json_len);
+/* The tuning table's entry for a workload (path NULL: the default table):
+ * *found = 0 none, 1 an entry for antennas, beams, nsamples, tscrunch and
+ * stokes (cb_waves set), 2 one whose DM list equals dms (fft_len [ndms] and
+ * share_forward set too).  Host only: needs no GPU. */
+int sw_tuning_lookup(const char* path, int nantennas, int nbeams, int nsamples, int tscrunch, int stokes, int ndms,
+                     const float* dms, int* found, int* cb_waves, int* fft_len, int* share_forward);
 
 /* -------------------------------------------------------- microbenchmarks
  * Roofline peaks of the device, measured: which = 0 dense int8 MFMA (TOP/s),

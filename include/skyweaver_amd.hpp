@@ -43,6 +43,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -72,23 +73,42 @@ inline std::string last_error()
     return buf;
 }
 
-// Message of an exception a handler threw inside a library callback; the
-// callback returns an error code through the C ABI and check() rethrows it.
-inline std::string& handler_error()
+// Message of an exception a handler threw inside a library callback (on the
+// calling thread, or on the library's sink thread with asynchronous
+// handlers); the callback returns an error code through the C ABI and
+// check() rethrows it on the thread that receives the code.
+namespace detail {
+struct HandlerError {
+    std::mutex mu;
+    std::string msg;
+};
+inline HandlerError& handler_error()
 {
-    static thread_local std::string msg;
-    return msg;
+    static HandlerError e;
+    return e;
 }
+inline void set_handler_error(std::string const& m)
+{
+    auto& e = handler_error();
+    std::lock_guard<std::mutex> lk(e.mu);
+    if(e.msg.empty()) e.msg = m;
+}
+inline std::string take_handler_error()
+{
+    auto& e = handler_error();
+    std::lock_guard<std::mutex> lk(e.mu);
+    std::string m;
+    m.swap(e.msg);
+    return m;
+}
+} // namespace detail
 
 // Convert a C ABI status into the reference's exception types.
 inline void check(int rc, double epoch = 0.0)
 {
     if(rc == SW_OK) return;
-    if(!handler_error().empty()) {
-        std::string m = "handler failed: " + handler_error();
-        handler_error().clear();
-        throw std::runtime_error(m);
-    }
+    const std::string h = detail::take_handler_error();
+    if(!h.empty()) throw std::runtime_error("handler failed: " + h);
     if(rc == SW_ERR_DELAY_EPOCH) throw InvalidDelayEpoch(epoch);
     throw std::runtime_error(last_error());
 }
@@ -208,6 +228,19 @@ class PipelineConfig {
     void cb_form(int f) { _cb_form = f; }
     std::vector<int> const& fft_lengths() const { return _fft_len; }
     void fft_lengths(std::vector<int> const& v) { _fft_len = v; }
+    // coherent DMs of equal FFT length share one overlap and their forward
+    // transforms (sw_codedisp_config::share_forward)
+    bool share_forward() const { return _share_forward; }
+    void share_forward(bool v) { _share_forward = v; }
+    // FFT lengths (and share_forward) timed on the device at construction
+    // when no fft_lengths are given
+    bool autotune_fft() const { return _autotune_fft; }
+    void autotune_fft(bool v) { _autotune_fft = v; }
+    // handlers called by a library thread while the next DMs and gulps are
+    // processed (sw_pipeline_config::async_sinks); synchronize() waits for
+    // them and rethrows a handler's exception
+    bool async_handlers() const { return _async; }
+    void async_handlers(bool v) { _async = v; }
 
     // input (MultiFileReader, run_pipeline)
     std::vector<std::string> const& input_files() const { return _input_files; }
@@ -232,6 +265,7 @@ class PipelineConfig {
     float _level = 24.0f;
     int _cb_form = SW_CB_MFMA;
     std::vector<int> _fft_len;
+    bool _share_forward = false, _autotune_fft = false, _async = false;
     std::vector<std::string> _input_files;
     bool _check_contiguity = false;
     float _start_time = 0.0f, _duration = std::numeric_limits<float>::infinity();
@@ -415,10 +449,10 @@ int cb_trampoline(void* user, sw_block const* b)
         (*static_cast<H*>(user))(BlockView(*b), static_cast<std::size_t>(b->dm_idx));
         return 0;
     } catch(std::exception const& e) {
-        handler_error() = e.what();
+        set_handler_error(e.what());
         return 1;
     } catch(...) {
-        handler_error() = "unknown exception";
+        set_handler_error("unknown exception");
         return 1;
     }
 }
@@ -429,10 +463,10 @@ int stats_trampoline(void* user, sw_block const* b)
         (*static_cast<H*>(user))(BlockView(*b));
         return 0;
     } catch(std::exception const& e) {
-        handler_error() = e.what();
+        set_handler_error(e.what());
         return 1;
     } catch(...) {
-        handler_error() = "unknown exception";
+        set_handler_error("unknown exception");
         return 1;
     }
 }
@@ -527,6 +561,9 @@ class BeamformerPipeline {
         }
         c.cb_form = config.cb_form();
         c.batched_dms = 1;
+        c.share_forward = config.share_forward() ? 1 : 0;
+        c.autotune_fft = config.autotune_fft() ? 1 : 0;
+        c.async_sinks = config.async_handlers() ? 1 : 0;
         check(sw_pipeline_create(&c, &_p));
         try {
             check(sw_pipeline_set_sinks(_p, &detail::cb_trampoline<CBHandler>, &_cb,

@@ -90,6 +90,9 @@ def _declare(L):
     L.sw_cb_beamform.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(BfDims), _c, _vp, _vp]
     L.sw_cb_pack_weights.argtypes = [_vp, _vp, _c, _c, _c, _vp]
     L.sw_cb_beamform_packed.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(BfDims), _vp]
+    L.sw_cb_beamform_packed_tile.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(BfDims), _c, _vp]
+    L.sw_tuning_lookup.argtypes = [ctypes.c_char_p, _c, _c, _c, _c, _c, _c, _vp, ctypes.POINTER(_c),
+                                   ctypes.POINTER(_c), _vp, ctypes.POINTER(_c)]
     L.sw_synth_gaussian_c8.argtypes = [_vp, _sz, ctypes.c_uint64, ctypes.c_double, _c]
     L.sw_synth_phase_weights.argtypes = [_vp, _sz, ctypes.c_uint64]
     L.sw_microbench.argtypes = [_c, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _vp]
@@ -384,6 +387,52 @@ def cb_beamform_packed(ftpa, packed, scale, offset, bmap, ib_raw, out, d: BfDims
                                       _ptr(ib_raw), _ptr(out), ctypes.byref(d), _stream(stream)))
 
 
+def cb_beamform_packed_tile(ftpa, packed, scale, offset, bmap, ib_raw, out, d: BfDims, waves, stream=None):
+    """sw_cb_beamform_packed with `waves` (2, 4, 8) waves of 32 time rows per workgroup."""
+    check(lib().sw_cb_beamform_packed_tile(_ptr(ftpa), _ptr(packed), _ptr(scale), _ptr(offset), _ptr(bmap),
+                                           _ptr(ib_raw), _ptr(out), ctypes.byref(d), int(waves), _stream(stream)))
+
+
+def tuning_lookup(nantennas, nbeams, nsamples, tscrunch, stokes="I", dms=(), path=None):
+    """The tuning table's entry for a workload: None, or {"cb_waves"} and, when its DM
+    list equals dms, {"fft_len", "share_forward"} too.  Host only."""
+    import numpy as np
+    d = np.ascontiguousarray(dms, np.float32)
+    found, waves, share = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    fl = np.zeros(max(1, len(d)), np.int32)
+    check(lib().sw_tuning_lookup(None if path is None else str(path).encode(), nantennas, nbeams, nsamples, tscrunch,
+                                 STOKES[stokes] if isinstance(stokes, str) else stokes, len(d),
+                                 d.ctypes.data if len(d) else None, ctypes.byref(found), ctypes.byref(waves),
+                                 fl.ctypes.data, ctypes.byref(share)))
+    if found.value == 0:
+        return None
+    out = {"cb_waves": waves.value}
+    if found.value == 2:
+        out.update(fft_len=[int(x) for x in fl[:len(d)]], share_forward=bool(share.value))
+    return out
+
+
+def pipeline_autotune(nantennas, nchans, nbeams, nsamples, tscrunch, dms, centre_frequency, bandwidth,
+                      stokes="I", fft_len=None, tune_chans=8):
+    """sw_pipeline_autotune: the tuning-table entry (a dict) measured on the device for
+    this workload: beamformer waves per workgroup and, when fft_len is None, the FFT plan."""
+    import json
+    import numpy as np
+    L = lib()
+    if not hasattr(L, "_pipeline_declared"):
+        _declare_pipeline(L)
+        L._pipeline_declared = True
+    d = np.ascontiguousarray(dms, np.float32)
+    f = np.ascontiguousarray(fft_len, np.int32) if fft_len is not None else None
+    ip = ctypes.POINTER(ctypes.c_int)
+    cfg = PipelineConfigC(nantennas, nchans, nbeams, nsamples, tscrunch, STOKES[stokes], 1, 24.0, centre_frequency,
+                          bandwidth, 4096, len(d), d.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                          f.ctypes.data_as(ip) if f is not None else None, None, FORMS["mfma"], 1)
+    buf = ctypes.create_string_buffer(4096)
+    check(L.sw_pipeline_autotune(ctypes.byref(cfg), int(tune_chans), buf, 4096))
+    return json.loads(buf.value.decode())
+
+
 def update_scalings(stats_np, beamset_weights_np, nbeamsets, cb_tscrunch, ib_tscrunch, output_level):
     """Host scalings from an FPA statistics array [F][P][A][4] float32."""
     import numpy as np
@@ -628,7 +677,8 @@ class PipelineConfigC(ctypes.Structure):
                 ("overlap", ctypes.POINTER(ctypes.c_int)), ("cb_form", ctypes.c_int), ("batched_dms", ctypes.c_int),
                 ("id_enable", ctypes.c_int), ("id_tscrunch", ctypes.c_void_p), ("id_dms", ctypes.c_void_p),
                 ("id_ndms", ctypes.c_void_p), ("id_dispatch_size", ctypes.c_int), ("autotune_fft", ctypes.c_int),
-                ("share_forward", ctypes.c_int), ("async_sinks", ctypes.c_int)]
+                ("share_forward", ctypes.c_int), ("async_sinks", ctypes.c_int), ("graph", ctypes.c_int),
+                ("cb_waves", ctypes.c_int), ("tuning_file", ctypes.c_char_p)]
 
 
 class ObsHeaderC(ctypes.Structure):
@@ -694,6 +744,8 @@ def _declare_pipeline(L):
     L.sw_pipeline_set_tdb_sink.argtypes = [_vp, SINK_FN, _vp]
     L.sw_pipeline_stream.argtypes = [_vp]
     L.sw_pipeline_stream.restype = _vp
+    L.sw_pipeline_tile.argtypes = [_vp, ctypes.POINTER(_c), _vp, ctypes.POINTER(_c), ctypes.POINTER(_c)]
+    L.sw_pipeline_autotune.argtypes = [ctypes.POINTER(PipelineConfigC), _c, ctypes.c_char_p, _sz]
 
 
 STAGES = ("transpose", "statistics", "dedispersion", "incoherent_beam", "coherent_beam", "incoherent_dedispersion",
@@ -706,12 +758,15 @@ class BeamformerPipeline:
     def __init__(self, nantennas, nchans, nbeams, nsamples, tscrunch, dms, centre_frequency, bandwidth,
                  stokes="I", ib_subtract=True, output_level=24.0, fft_len=None, overlap=None, form="mfma",
                  batched=True, total_nchans=4096, id_plan=None, id_dispatch_size=0, autotune_fft=False,
-                 share_forward=False, async_sinks=False):
+                 share_forward=False, async_sinks=False, graph=False, cb_waves=0, tuning_file=None):
         """id_plan: None (no incoherent dedispersion) or, per coherent DM, (tscrunch, incoherent DMs).
         autotune_fft (with fft_len None): FFT lengths and share_forward chosen by sw_codedisp_autotune.
         share_forward: DMs of equal FFT length share one overlap (and, batched, their forward FFTs).
         async_sinks: sinks are called by a library thread while the pipeline goes on (blocks of each
-        DM alternate between two buffers); a sink's exception is raised by the next call."""
+        DM alternate between two buffers); a sink's exception is raised by the next call.
+        graph (with async_sinks): each gulp after the first is captured into a hip graph and launched.
+        cb_waves: MFMA beamformer waves per workgroup (2, 4, 8; 0: the tuning table's, else 4);
+        tuning_file: the tuning table (None: the default one, "" none)."""
         import numpy as np
         L = lib()
         if not hasattr(L, "_pipeline_declared"):
@@ -731,6 +786,10 @@ class BeamformerPipeline:
         cfg.autotune_fft = 1 if autotune_fft else 0
         cfg.share_forward = 1 if share_forward else 0
         cfg.async_sinks = 1 if async_sinks else 0
+        cfg.graph = 1 if graph else 0
+        cfg.cb_waves = int(cb_waves)
+        self._tuning_file = None if tuning_file is None else str(tuning_file).encode()
+        cfg.tuning_file = self._tuning_file
         if id_plan is not None:
             assert len(id_plan) == n
             self._id_ts = np.array([b[0] for b in id_plan], np.int32)
@@ -817,6 +876,15 @@ class BeamformerPipeline:
     @property
     def stream(self):
         return lib().sw_pipeline_stream(self.handle)
+
+    def tile(self):
+        """{"cb_waves", "fft_len", "share_forward", "source"} the pipeline runs with
+        (source: "default", "table", "table+fft", "autotune")."""
+        w, sh, src = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        fl = (ctypes.c_int * len(self._dms))()
+        check(lib().sw_pipeline_tile(self.handle, ctypes.byref(w), fl, ctypes.byref(sh), ctypes.byref(src)))
+        return {"cb_waves": w.value, "fft_len": list(fl), "share_forward": bool(sh.value),
+                "source": ("default", "table", "table+fft", "autotune")[src.value]}
 
     def close(self):
         if getattr(self, "handle", None):

@@ -213,7 +213,7 @@ cb_plain_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ we
 }
 
 // ========================================================== MFMA form
-constexpr int kMfmaWaves = 4;  // waves per workgroup, 32 time rows each
+constexpr int kMfmaWaves = 4;  // default waves per workgroup, 32 time rows each (WV)
 
 // Packed weights: for each (f, b) two rows of 2A bytes:
 //   row 0 (real part):      [ wr[0..A) | ~wi[0..A) ]
@@ -265,8 +265,8 @@ struct FusedIb {
 // (sample t of column c = p A + a of channel f at ftpa + ((f 2A + c) ld + w0
 // + t) 2) and are corner-turned through LDS into the FTPA rows the operands
 // are read from; else ftpa is FTPA [F][T][2][A].
-template <int S, int TS, int NSTEPS, bool TMI>
-__global__ void __launch_bounds__(kMfmaWaves * 64)
+template <int S, int TS, int NSTEPS, bool TMI, int WV>
+__global__ void __launch_bounds__(WV * 64)
 cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ packed,
                int8_t* __restrict__ out, const float* __restrict__ scale,
                const float* __restrict__ offset, const int* __restrict__ bmap,
@@ -280,20 +280,20 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     // that the 32 lanes of a half-wave read distinct banks
     constexpr int WPITCH = 2 * KP + 16;          // bytes per beam in LDS
     constexpr int WTILE  = 32 * 2 * KP / 16;     // uint4 per tile in global memory
-    constexpr int WPER   = (WTILE + kMfmaWaves * 64 - 1) / (kMfmaWaves * 64);
+    constexpr int WPER   = (WTILE + WV * 64 - 1) / (WV * 64);
     constexpr int NOUT   = TS <= 16 ? 16 / TS : 1;  // outputs per lane per tile
     // LDS: the weight tiles (double-buffered), or first, with TMI, the
-    // workgroup's 32 kMfmaWaves time rows of both polarisations (rows of
+    // workgroup's 32 WV time rows of both polarisations (rows of
     // 4A bytes padded by 16)
     constexpr int WL16   = 32 * WPITCH / 16;
     constexpr int SPITCH = 4 * A + 16;
-    constexpr int ST16   = TMI ? 32 * kMfmaWaves * SPITCH / 16 : 0;
+    constexpr int ST16   = TMI ? 32 * WV * SPITCH / 16 : 0;
     __shared__ uint4 smem[2 * WL16 > ST16 ? 2 * WL16 : ST16];
     uint4 (*wlds)[WL16] = reinterpret_cast<uint4 (*)[WL16]>(smem);
     // epilogue parameters of the channel's beams and the workgroup's
     // incoherent-beam powers, staged once (up to kBeamLds beams)
     constexpr bool kPre = EpiParams<S, NOUT>::kPreIb;
-    constexpr int NO_WG = 32 * kMfmaWaves / (TS <= 32 ? TS : 32);  // output samples of the workgroup
+    constexpr int NO_WG = 32 * WV / (TS <= 32 ? TS : 32);  // output samples of the workgroup
     __shared__ int tb_s[kPre ? kBeamLds : 1];
     __shared__ float tb_sc[kPre ? kBeamLds : 1], tb_off[kPre ? kBeamLds : 1];
     __shared__ float tb_ib[kPre ? 16 * NO_WG * NV : 1];
@@ -315,7 +315,7 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     const int tid  = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int t0   = (tc * kMfmaWaves + wave) * 32;  // first time row of this wave
+    const int t0   = (tc * WV + wave) * 32;  // first time row of this wave
     // waves past the end of the gulp still take part in the weight staging
     // and its barriers, but compute on row 0 and store nothing
     const bool active = t0 < nsamples;
@@ -332,13 +332,13 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     const bool tabs = kPre && nbeams <= kBeamLds && nbeamsets <= 16;  // uniform
     // TMI: the workgroup's time rows, 16-byte runs of 8 samples of each
     // column, issued first so that their latency overlaps the table loads
-    constexpr int NST = TMI ? 2 * A * 4 * kMfmaWaves / (kMfmaWaves * 64) : 1;  // runs per thread
+    constexpr int NST = TMI ? 2 * A * 4 * WV / (WV * 64) : 1;  // runs per thread
     uint4 run[NST];
     if constexpr(TMI) {
-        const int t0w = tc * kMfmaWaves * 32;
+        const int t0w = tc * WV * 32;
 #pragma unroll
         for(int k = 0; k < NST; ++k) {
-            const int i = tid + k * kMfmaWaves * 64;
+            const int i = tid + k * WV * 64;
             const int c = i % (2 * A), q = i / (2 * A);
             run[k] = make_uint4(0, 0, 0, 0);
             if(t0w + 8 * q < nsamples)
@@ -349,31 +349,31 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
         if(tabs) {
             // all of a thread's loads in flight together (beam map first,
             // then the scales and offsets it selects)
-            constexpr int NB = kBeamLds / (kMfmaWaves * 64);
+            constexpr int NB = kBeamLds / (WV * 64);
             int sbv[NB];
 #pragma unroll
             for(int k = 0; k < NB; ++k) {
-                const int b = tid + k * kMfmaWaves * 64;
+                const int b = tid + k * WV * 64;
                 sbv[k] = b < nbeams ? bmap[b] : 0;
             }
             float scv[NB], ofv[NB];
 #pragma unroll
             for(int k = 0; k < NB; ++k) {
-                const int b = tid + k * kMfmaWaves * 64;
+                const int b = tid + k * WV * 64;
                 scv[k] = b < nbeams ? scale[sbv[k] * nchans + f] : 0.0f;
                 ofv[k] = (b < nbeams && !ib_sub) ? offset[sbv[k] * nchans + f] : 0.0f;
             }
 #pragma unroll
             for(int k = 0; k < NB; ++k) {
-                const int b = tid + k * kMfmaWaves * 64;
+                const int b = tid + k * WV * 64;
                 if(b < nbeams) {
                     tb_s[b]   = sbv[k];
                     tb_sc[b]  = scv[k];
                     tb_off[b] = ofv[k];
                 }
             }
-            const int o0 = tc * kMfmaWaves * 32 / TS;
-            for(int i = tid; !(TMI && fib.raw) && i < nbeamsets * NO_WG * NV; i += kMfmaWaves * 64) {
+            const int o0 = tc * WV * 32 / TS;
+            for(int i = tid; !(TMI && fib.raw) && i < nbeamsets * NO_WG * NV; i += WV * 64) {
                 const int k = i % NV, o = (i / NV) % NO_WG, sb = i / (NV * NO_WG);
                 float v = 0.0f;
                 if(ib_sub && o0 + o < nsamps_out) v = ib_raw[(((size_t)sb * nsamps_out + o0 + o) * nchans + f) * NV + k];
@@ -406,7 +406,7 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
         uint8_t* stg = reinterpret_cast<uint8_t*>(smem);
 #pragma unroll
         for(int k = 0; k < NST; ++k) {
-            const int i = tid + k * kMfmaWaves * 64;
+            const int i = tid + k * WV * 64;
             const int c = i % (2 * A), q = i / (2 * A);
             const uint32_t w[4] = {run[k].x, run[k].y, run[k].z, run[k].w};
 #pragma unroll
@@ -419,8 +419,8 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
                 const int o0 = tc * NO_WG;
                 const int r0 = lane / LPR, g = lane % LPR;
 #pragma unroll
-                for(int oi = 0; oi < NO_WG / kMfmaWaves; ++oi) {
-                    const int o = wave * (NO_WG / kMfmaWaves) + oi;  // this wave's own rows
+                for(int oi = 0; oi < NO_WG / WV; ++oi) {
+                    const int o = wave * (NO_WG / WV) + oi;  // this wave's own rows
                     int lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
 #pragma unroll
                     for(int i = 0; i < TS / RPI; ++i) {
@@ -517,13 +517,13 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
         if((size_t)(bt + 1) * WTILE <= wtotal) {  // a whole tile (uniform)
 #pragma unroll
             for(int i = 0; i < WPER; ++i) {
-                const int idx = tid + i * kMfmaWaves * 64;
-                r[i] = (WTILE % (kMfmaWaves * 64) == 0 || idx < WTILE) ? src[idx] : make_uint4(0, 0, 0, 0);
+                const int idx = tid + i * WV * 64;
+                r[i] = (WTILE % (WV * 64) == 0 || idx < WTILE) ? src[idx] : make_uint4(0, 0, 0, 0);
             }
         } else {
 #pragma unroll
             for(int i = 0; i < WPER; ++i) {
-                const int idx = tid + i * kMfmaWaves * 64;
+                const int idx = tid + i * WV * 64;
                 const size_t g = (size_t)bt * WTILE + idx;
                 r[i] = (idx < WTILE && g < wtotal) ? wsrc[g] : make_uint4(0, 0, 0, 0);
             }
@@ -532,7 +532,7 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     auto stash = [&](int buf, const uint4* r) {
 #pragma unroll
         for(int i = 0; i < WPER; ++i) {
-            const int idx = tid + i * kMfmaWaves * 64;
+            const int idx = tid + i * WV * 64;
             if(idx < WTILE) {
                 const int beam = idx / (2 * KP / 16), c16 = idx % (2 * KP / 16);
                 wlds[buf][beam * (WPITCH / 16) + c16] = r[i];
@@ -686,19 +686,68 @@ int launch_plain(const uint8_t* ftpa, const uint8_t* w, int8_t* out, const float
     return SW_OK;
 }
 
+// LDS bytes of cb_mfma_kernel<S, TS, NSTEPS, TMI, WV> (its __shared__ arrays)
+template <int S, int TS, int NSTEPS, bool TMI, int WV>
+constexpr size_t cb_mfma_lds_bytes()
+{
+    constexpr int A = NSTEPS * 16, KP = 2 * A, NV = nstokes(S);
+    constexpr int WL16 = 32 * (2 * KP + 16) / 16;
+    constexpr int ST16 = TMI ? 32 * WV * (4 * A + 16) / 16 : 0;
+    constexpr int NOUT = TS <= 16 ? 16 / TS : 1;
+    constexpr bool kPre = NOUT * NV <= 4;
+    constexpr int NO_WG = 32 * WV / (TS <= 32 ? TS : 32);
+    return (size_t)16 * (2 * WL16 > ST16 ? 2 * WL16 : ST16) + (kPre ? (size_t)kBeamLds * 12 : 12) +
+           (kPre ? (size_t)16 * NO_WG * NV * 4 : 4);
+}
+
+// workgroup shapes (waves of 32 time rows) instantiated per form: the
+// pipeline's forms (Stokes I, on the ring or on FTPA, and full Stokes on
+// FTPA) take 2, 4 or 8 waves, the others 4
+template <int S, bool TMI>
+constexpr bool cb_wave_variants()
+{
+    return S == SW_STOKES_I || (S == SW_STOKES_IQUV && !TMI);
+}
+
+template <int S, int TS, int NS, bool TMI, int WV>
+int launch_mfma_wv(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, const float* sc, const float* off,
+                   const int* bmap, const float* ib, const sw_bf_dims& d, hipStream_t st, long ld, long w0,
+                   const FusedIb& fib)
+{
+    if constexpr(cb_mfma_lds_bytes<S, TS, NS, TMI, WV>() <= 163840) {
+        const int ntchunks = (d.nsamples + 32 * WV - 1) / (32 * WV);
+        hipLaunchKernelGGL((cb_mfma_kernel<S, TS, NS, TMI, WV>), dim3(ntchunks * d.nchans), dim3(WV * 64), 0, st,
+                           ftpa, packed, out, sc, off, bmap, ib, d.nchans, d.nsamples, d.nbeams, ntchunks,
+                           d.ib_subtract, d.nbeamsets, ld, w0, fib);
+        SW_LAUNCH_CHECK();
+        return SW_OK;
+    } else {
+        set_error("sw_cb_beamform: this workgroup shape does not fit the LDS at this antenna count");
+        return SW_ERR_UNSUPPORTED;
+    }
+}
+
 template <int S, int TS, bool TMI>
 int launch_mfma_ts(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, const float* sc,
                    const float* off, const int* bmap, const float* ib, const sw_bf_dims& d,
-                   hipStream_t st, long ld, long w0, const FusedIb& fib)
+                   hipStream_t st, long ld, long w0, const FusedIb& fib, int waves)
 {
-    const int ntchunks = (d.nsamples + 32 * kMfmaWaves - 1) / (32 * kMfmaWaves);
-    dim3 grid(ntchunks * d.nchans);
-#define SW_MFMA(NS)                                                                        \
-    case NS:                                                                               \
-        hipLaunchKernelGGL((cb_mfma_kernel<S, TS, NS, TMI>), grid, dim3(kMfmaWaves * 64), 0, \
-                           st, ftpa, packed, out, sc, off, bmap, ib, d.nchans,             \
-                           d.nsamples, d.nbeams, ntchunks, d.ib_subtract, d.nbeamsets, ld, w0, fib); \
-        break;
+#define SW_MFMA_WV(NS, WV) return launch_mfma_wv<S, TS, NS, TMI, WV>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib)
+#define SW_MFMA(NS)                                                   \
+    case NS:                                                          \
+        if constexpr(cb_wave_variants<S, TMI>()) {                    \
+            if(waves == 2) SW_MFMA_WV(NS, 2);                         \
+            if(waves == 8) SW_MFMA_WV(NS, 8);                         \
+        }                                                             \
+        SW_MFMA_WV(NS, kMfmaWaves);
+    if(waves != 2 && waves != 4 && waves != 8) {
+        set_error("sw_cb_beamform: waves per workgroup must be 2, 4 or 8");
+        return SW_ERR_INVALID;
+    }
+    if(waves != kMfmaWaves && !cb_wave_variants<S, TMI>()) {
+        set_error("sw_cb_beamform: this Stokes mode runs with 4 waves per workgroup only");
+        return SW_ERR_UNSUPPORTED;
+    }
     switch(d.nantennas / 16) {
         SW_MFMA(1) SW_MFMA(2) SW_MFMA(3) SW_MFMA(4) SW_MFMA(5) SW_MFMA(6) SW_MFMA(7) SW_MFMA(8)
     default:
@@ -706,22 +755,21 @@ int launch_mfma_ts(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, cons
         return SW_ERR_UNSUPPORTED;
     }
 #undef SW_MFMA
-    SW_LAUNCH_CHECK();
-    return SW_OK;
+#undef SW_MFMA_WV
 }
 
 template <int S, bool TMI>
 int launch_mfma(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, const float* sc,
                 const float* off, const int* bmap, const float* ib, const sw_bf_dims& d,
-                hipStream_t st, long ld = 0, long w0 = 0, const FusedIb& fib = FusedIb{})
+                hipStream_t st, long ld = 0, long w0 = 0, const FusedIb& fib = FusedIb{}, int waves = kMfmaWaves)
 {
     switch(d.tscrunch) {
-    case 1: return launch_mfma_ts<S, 1, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib);
-    case 2: return launch_mfma_ts<S, 2, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib);
-    case 4: return launch_mfma_ts<S, 4, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib);
-    case 8: return launch_mfma_ts<S, 8, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib);
-    case 16: return launch_mfma_ts<S, 16, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib);
-    case 32: return launch_mfma_ts<S, 32, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib);
+    case 1: return launch_mfma_ts<S, 1, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib, waves);
+    case 2: return launch_mfma_ts<S, 2, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib, waves);
+    case 4: return launch_mfma_ts<S, 4, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib, waves);
+    case 8: return launch_mfma_ts<S, 8, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib, waves);
+    case 16: return launch_mfma_ts<S, 16, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib, waves);
+    case 32: return launch_mfma_ts<S, 32, TMI>(ftpa, packed, out, sc, off, bmap, ib, d, st, ld, w0, fib, waves);
     default:
         set_error("sw_cb_beamform: MFMA form supports tscrunch in {1, 2, 4, 8, 16, 32}");
         return SW_ERR_UNSUPPORTED;
@@ -794,7 +842,7 @@ namespace sw {
 int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_packed, const float* d_scale,
                           const float* d_offset, const int* d_beamset_map, const float* d_ib_raw, void* d_out,
                           const sw_bf_dims& d, hipStream_t st, float* d_ib_out_raw, void* d_ib_out,
-                          const float* d_ib_scale, const float* d_ib_offset, const float* d_ant_weights)
+                          const float* d_ib_scale, const float* d_ib_offset, const float* d_ant_weights, int waves)
 {
     int rc = check_dims(d);
     if(rc) return rc;
@@ -815,7 +863,37 @@ int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_pa
     }
     return launch_mfma<SW_STOKES_I, true>(static_cast<const uint8_t*>(d_ring), static_cast<const uint8_t*>(d_packed),
                                           static_cast<int8_t*>(d_out), d_scale, d_offset, d_beamset_map, d_ib_raw, d,
-                                          st, ld, w0, fib);
+                                          st, ld, w0, fib, waves);
+}
+
+// sw_cb_beamform_packed with `waves` waves (32 time rows each) per workgroup
+int cb_beamform_packed_waves(const void* d_ftpa, const void* d_packed, const float* d_scale, const float* d_offset,
+                             const int* d_beamset_map, const float* d_ib_raw, void* d_out, const sw_bf_dims& d,
+                             hipStream_t st, int waves)
+{
+    int rc = check_dims(d);
+    if(rc) return rc;
+    SW_CHECK_ARG(d_ftpa && d_packed && d_scale && d_offset && d_beamset_map && d_ib_raw && d_out,
+                 "null buffer");
+    SW_CHECK_ARG((reinterpret_cast<uintptr_t>(d_ftpa) & 15) == 0 &&
+                     (reinterpret_cast<uintptr_t>(d_packed) & 15) == 0,
+                 "voltages and packed weights must be 16-byte aligned");
+    if(!cb_mfma_supported(d)) {
+        set_error("sw_cb_beamform_packed: shape not supported by the MFMA form "
+                  "(needs A % 16 == 0, A <= 128, T % 32 == 0, tscrunch in {1..32, pow2})");
+        return SW_ERR_UNSUPPORTED;
+    }
+    auto* in  = static_cast<const uint8_t*>(d_ftpa);
+    auto* pk  = static_cast<const uint8_t*>(d_packed);
+    auto* out = static_cast<int8_t*>(d_out);
+    const FusedIb none{};
+    switch(d.stokes) {
+    case SW_STOKES_I: return launch_mfma<SW_STOKES_I, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st, 0, 0, none, waves);
+    case SW_STOKES_Q: return launch_mfma<SW_STOKES_Q, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st, 0, 0, none, waves);
+    case SW_STOKES_U: return launch_mfma<SW_STOKES_U, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st, 0, 0, none, waves);
+    case SW_STOKES_V: return launch_mfma<SW_STOKES_V, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st, 0, 0, none, waves);
+    default: return launch_mfma<SW_STOKES_IQUV, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st, 0, 0, none, waves);
+    }
 }
 } // namespace sw
 
@@ -825,30 +903,17 @@ extern "C" int sw_cb_beamform_packed(const void* d_ftpa, const void* d_packed,
                                      void* d_out, const sw_bf_dims* dims, void* stream)
 {
     SW_CHECK_ARG(dims, "null dims");
-    int rc = sw::check_dims(*dims);
-    if(rc) return rc;
-    const sw_bf_dims& d = *dims;
-    SW_CHECK_ARG(d_ftpa && d_packed && d_scale && d_offset && d_beamset_map && d_ib_raw && d_out,
-                 "null buffer");
-    SW_CHECK_ARG((reinterpret_cast<uintptr_t>(d_ftpa) & 15) == 0 &&
-                     (reinterpret_cast<uintptr_t>(d_packed) & 15) == 0,
-                 "voltages and packed weights must be 16-byte aligned");
-    if(!sw::cb_mfma_supported(d)) {
-        sw::set_error("sw_cb_beamform_packed: shape not supported by the MFMA form "
-                      "(needs A % 16 == 0, A <= 128, T % 32 == 0, tscrunch in {1..32, pow2})");
-        return SW_ERR_UNSUPPORTED;
-    }
-    auto* in  = static_cast<const uint8_t*>(d_ftpa);
-    auto* pk  = static_cast<const uint8_t*>(d_packed);
-    auto* out = static_cast<int8_t*>(d_out);
-    hipStream_t st = sw::as_stream(stream);
-    switch(d.stokes) {
-    case SW_STOKES_I: return sw::launch_mfma<SW_STOKES_I, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
-    case SW_STOKES_Q: return sw::launch_mfma<SW_STOKES_Q, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
-    case SW_STOKES_U: return sw::launch_mfma<SW_STOKES_U, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
-    case SW_STOKES_V: return sw::launch_mfma<SW_STOKES_V, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
-    default: return sw::launch_mfma<SW_STOKES_IQUV, false>(in, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st);
-    }
+    return sw::cb_beamform_packed_waves(d_ftpa, d_packed, d_scale, d_offset, d_beamset_map, d_ib_raw, d_out, *dims,
+                                        sw::as_stream(stream), sw::kMfmaWaves);
+}
+
+extern "C" int sw_cb_beamform_packed_tile(const void* d_ftpa, const void* d_packed, const float* d_scale,
+                                          const float* d_offset, const int* d_beamset_map, const float* d_ib_raw,
+                                          void* d_out, const sw_bf_dims* dims, int waves, void* stream)
+{
+    SW_CHECK_ARG(dims, "null dims");
+    return sw::cb_beamform_packed_waves(d_ftpa, d_packed, d_scale, d_offset, d_beamset_map, d_ib_raw, d_out, *dims,
+                                        sw::as_stream(stream), waves);
 }
 
 extern "C" int sw_cb_beamform(const void* d_ftpa, const void* d_weights, const float* d_scale,

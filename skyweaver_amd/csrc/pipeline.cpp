@@ -37,6 +37,9 @@
 #include <vector>
 
 namespace sw {
+std::string default_tuning_path();
+int tuning_lookup(const std::string& path, int A, int B, int T, int ts, int stokes, int ndms, const float* dms,
+                  int* found, int* cb_waves, int* fft_len, int* share_forward);
 int cb_mfma_supported(const sw_bf_dims& d);
 size_t fpat_stats_workspace(int nchans, int npol, int nant, int nsamples);
 int ib_beamform_tm(const void* d_ring, long ld, long w0, float* d_raw, void* d_out, const float* d_scale,
@@ -44,7 +47,10 @@ int ib_beamform_tm(const void* d_ring, long ld, long w0, float* d_raw, void* d_o
 int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_packed, const float* d_scale,
                           const float* d_offset, const int* d_beamset_map, const float* d_ib_raw, void* d_out,
                           const sw_bf_dims& d, hipStream_t st, float* d_ib_out_raw, void* d_ib_out,
-                          const float* d_ib_scale, const float* d_ib_offset, const float* d_ant_weights);
+                          const float* d_ib_scale, const float* d_ib_offset, const float* d_ant_weights, int waves);
+int cb_beamform_packed_waves(const void* d_ftpa, const void* d_packed, const float* d_scale, const float* d_offset,
+                             const int* d_beamset_map, const float* d_ib_raw, void* d_out, const sw_bf_dims& d,
+                             hipStream_t st, int waves);
 int cb_fused_ib_supported(const sw_bf_dims& d);
 }
 
@@ -176,6 +182,9 @@ struct sw_pipeline {
     bool weights_set   = false;
     bool use_mfma      = false;
     bool scalings_dirty = true;    // recompute the scalings from this gulp's statistics
+    int cb_waves = 4;              // MFMA beamformer workgroup shape (waves of 32 time rows)
+    int tile_source = 0;           // sw_pipeline_tile's *source
+    std::string tuning_file;
     // sinks
     sw_sink_fn cb_sink = nullptr, ib_sink = nullptr, st_sink = nullptr;
     void *cb_user = nullptr, *ib_user = nullptr, *st_user = nullptr;
@@ -205,6 +214,9 @@ struct sw_pipeline {
     std::vector<int8_t*> cb_bufs, ib_bufs;  // [dm][2]
     float* stats_bufs[2]{};
     std::vector<hipEvent_t> ev_ready;       // [dm][2], then the statistics' [2]
+    // graph mode (cfg.graph): the executable graph of the last gulp
+    hipGraphExec_t gexec = nullptr;
+    int graph_instantiations = 0;
     // prefetched input (sw_pipeline_prefetch): two device gulp buffers filled
     // on a copy stream while the pipeline stream works on the previous gulp
     hipStream_t copy_stream = nullptr;
@@ -220,6 +232,7 @@ struct sw_pipeline {
     ~sw_pipeline()
     {
         if(stream) (void)hipStreamSynchronize(stream);
+        if(gexec) (void)hipGraphExecDestroy(gexec);
         if(worker) {
             worker->drain();
             worker.reset();
@@ -333,6 +346,7 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
     SW_CHECK_ARG(c.nchans > 0 && c.nbeams > 0, "empty dimensions");
     SW_CHECK_ARG(c.tscrunch > 0 && c.nsamples % c.tscrunch == 0, "gulp must be a multiple of tscrunch");
     SW_CHECK_ARG(c.bandwidth > 0 && c.centre_frequency > 0, "bad band description");
+    SW_CHECK_ARG(!c.graph || c.async_sinks, "graph mode needs asynchronous sinks (per-DM output buffers)");
     auto p = std::make_unique<sw_pipeline>();
     p->cfg = c;
     p->dms.assign(c.coherent_dms, c.coherent_dms + c.ndms);
@@ -343,6 +357,26 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
     p->cfg.overlap      = c.overlap ? p->overlap.data() : nullptr;
     if(p->cfg.output_level <= 0) p->cfg.output_level = 24.0f;
     if(p->cfg.total_nchans <= 0) p->cfg.total_nchans = 4096;
+    // tiles: the configuration's, else the tuning table's, else defaults
+    p->tuning_file = c.tuning_file ? c.tuning_file : sw::default_tuning_path();
+    p->cfg.tuning_file = nullptr;
+    {
+        int found = 0, waves = 0, share = 0;
+        std::vector<int> tl(c.ndms, 0);
+        const bool want_fft = !c.fft_len && !c.autotune_fft;
+        int rc = sw::tuning_lookup(p->tuning_file, c.nantennas, c.nbeams, c.nsamples, c.tscrunch, c.stokes, c.ndms,
+                                   p->dms.data(), &found, &waves, tl.data(), &share);
+        if(rc) return rc;
+        p->cb_waves = c.cb_waves > 0 ? c.cb_waves : (found && waves > 0 ? waves : 4);
+        if(found && c.cb_waves <= 0 && waves > 0) p->tile_source = 1;
+        SW_CHECK_ARG(p->cb_waves == 2 || p->cb_waves == 4 || p->cb_waves == 8, "cb_waves must be 0, 2, 4 or 8");
+        if(found == 2 && want_fft && std::all_of(tl.begin(), tl.end(), [](int n) { return n > 0; })) {
+            p->fft_len = tl;
+            p->cfg.fft_len = p->fft_len.data();
+            p->cfg.share_forward = share;
+            p->tile_source = 2;
+        }
+    }
     SW_HIP_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     for(auto& e : p->ev) SW_HIP_TRY(hipEventCreate(&e));
     for(auto& e : p->ev_w) SW_HIP_TRY(hipEventCreate(&e));
@@ -362,7 +396,7 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
     cd.dms       = p->dms.data();
     cd.fft_len   = p->cfg.fft_len;
     cd.overlap   = p->cfg.overlap;
-    cd.share_forward = c.share_forward;
+    cd.share_forward = p->cfg.share_forward;
     std::vector<int> tuned(c.ndms, 0);
     if(c.autotune_fft && !c.fft_len) {
         cd.input_fpat = 1;
@@ -372,6 +406,7 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
         cd.fft_len = tuned.data();
         cd.share_forward = share;
         p->cfg.share_forward = share;
+        p->tile_source = 3;
     }
     cd.input_fpat = 1;
     cd.history_in_place = 1;
@@ -460,6 +495,17 @@ extern "C" int sw_pipeline_destroy(sw_pipeline* p)
 }
 
 extern "C" void* sw_pipeline_stream(const sw_pipeline* p) { return p ? p->stream : nullptr; }
+
+extern "C" int sw_pipeline_tile(const sw_pipeline* p, int* cb_waves, int* fft_len, int* share_forward, int* source)
+{
+    SW_CHECK_ARG(p, "null pipeline");
+    if(cb_waves) *cb_waves = p->cb_waves;
+    if(fft_len)
+        for(int d = 0; d < p->cfg.ndms; ++d) sw_codedisp_info(p->cd, d, &fft_len[d], nullptr, nullptr);
+    if(share_forward) *share_forward = p->cfg.share_forward;
+    if(source) *source = p->tile_source;
+    return SW_OK;
+}
 
 extern "C" int sw_pipeline_set_sinks(sw_pipeline* p, sw_sink_fn cb, void* cb_user, sw_sink_fn ib,
                                      void* ib_user, sw_sink_fn stats, void* stats_user)
@@ -582,6 +628,98 @@ int check_gulp(const sw_pipeline* p, size_t bytes)
 // given, is recorded once the input has been read (after the transpose).
 int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed);
 
+// Stokes I with 0/1 antenna weights (what delay models give): the MFMA
+// beamformer (and the incoherent beam, inside it where supported, else its
+// own kernel) reads the dedisperser's time-major output ring directly, with
+// no corner turn to FTPA
+bool beams_on_ring(const sw_pipeline* p)
+{
+    const auto& c = p->cfg;
+    bool r = p->use_mfma && c.stokes == SW_STOKES_I && 2 * c.nantennas <= 256 && c.nsamples % 8 == 0;
+    for(float w : p->h_antw) r = r && (w == 0.0f || w == 1.0f);
+    return r;
+}
+
+// DM dm's incoherent and coherent beam kernels into p->ib_q / p->cb_out (the
+// dedispersion of the gulp is already enqueued); `mid`, if given, is
+// recorded between the two
+int enqueue_beams(sw_pipeline* p, int dm, bool ring_direct, bool fuse_ib, hipStream_t st, hipEvent_t mid)
+{
+    const auto& c = p->cfg;
+    const sw_bf_dims d = bf_dims(p);
+    const int S = p->nbeamsets, F = c.nchans;
+    float* cb_offsets = p->scl;
+    float* cb_scaling = p->scl + S * F;
+    float* ib_offsets = p->scl + 2 * S * F;
+    float* ib_scaling = p->scl + 3 * S * F;
+    int rc;
+    const void* ring = nullptr;
+    int ld = 0, w0 = 0;
+    if(ring_direct) {
+        rc = sw_codedisp_window(p->cd, dm, &ring, &ld, &w0);
+        if(rc) return rc;
+        // with the incoherent beam fused into the coherent beamformer (which
+        // then reads the ring once) there is nothing to do here
+        if(!fuse_ib) rc = sw::ib_beamform_tm(ring, ld, w0, p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, d, st);
+    } else {
+        rc = sw_ib_beamform(p->ftpa_dd[dm], p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, &d, st);
+    }
+    if(rc) return rc;
+    if(mid) SW_HIP_TRY(hipEventRecord(mid, st));
+    if(ring_direct)
+        return sw::cb_beamform_packed_tm(ring, ld, w0, p->packed, cb_scaling, cb_offsets, p->bmap, p->ib_raw,
+                                         p->cb_out, d, st, fuse_ib ? p->ib_raw : nullptr, p->ib_q, ib_scaling,
+                                         ib_offsets, p->antw, p->cb_waves);
+    if(p->use_mfma)
+        return sw::cb_beamform_packed_waves(p->ftpa_dd[dm], p->packed, cb_scaling, cb_offsets, p->bmap, p->ib_raw,
+                                            p->cb_out, d, st, p->cb_waves);
+    return sw_cb_beamform(p->ftpa_dd[dm], p->weights, cb_scaling, cb_offsets, p->bmap, p->ib_raw, p->cb_out, &d,
+                          SW_CB_PLAIN, nullptr, st);
+}
+
+// The blocks of DM dm of this gulp (coherent beams, incoherent beam) with
+// their metadata: the dedispersed blocks carry the filter delay tsamp O/2
+// and, when the FFT hop does not divide the gulp, the dedisperser's latency
+void dm_blocks(const sw_pipeline* p, int dm, double utc_offset_gulp, sw_block& cb, sw_block& ib, double& utc)
+{
+    const auto& c = p->cfg;
+    int fl = 0, ov = 0, lat = 0;
+    sw_codedisp_info(p->cd, dm, &fl, &ov, &lat);
+    const double tsamp = c.nchans / c.bandwidth;
+    utc = utc_offset_gulp + tsamp * (ov / 2.0 - lat);
+    const size_t T2 = p->tout(), F = c.nchans;
+    const int nv = p->nv();
+    fill_block(cb, SW_BLOCK_CB_TFB, p->cb_out, T2 * F * c.nbeams * nv, "TFB", {T2, F, (size_t)c.nbeams});
+    fill_block(ib, SW_BLOCK_IB_BTF, p->ib_q, (size_t)p->nbeamsets * T2 * F * nv, "BTF",
+               {(size_t)p->nbeamsets, T2, F});
+    for(sw_block* b : {&cb, &ib}) {
+        b->dm_idx = dm;
+        b->reference_dm = p->dms[dm];
+        b->tsamp = tsamp * c.tscrunch;
+        b->utc_offset = utc;
+    }
+}
+
+void stats_block(const sw_pipeline* p, double utc_offset_gulp, sw_block& blk)
+{
+    const auto& c = p->cfg;
+    fill_block(blk, SW_BLOCK_STATS_FPA, p->stats, sizeof(float) * 4 * c.nchans * 2 * c.nantennas, "FPA",
+               {(size_t)c.nchans, 2, (size_t)c.nantennas});
+    blk.tsamp = c.nchans / c.bandwidth * c.nsamples;
+    blk.utc_offset = utc_offset_gulp;
+}
+
+// The gulp's device work captured into a hip graph and launched: the
+// transpose with the statistics' sums, the dedispersion of every DM and
+// every DM's beams into this gulp's parity buffers.  The executable graph
+// is updated in place from each gulp's capture (hipGraphExecUpdate: the
+// kernels are the same, their arguments move with the segment grid and
+// the buffer parity) and instantiated again only when the launches differ.
+// The blocks are released to the sink thread, and the incoherent
+// dedispersion fed, once the graph has run.
+int run_gulp_graph(sw_pipeline* p, const void* in, hipEvent_t consumed, int par, double utc_offset_gulp);
+
+
 } // namespace
 
 extern "C" int sw_pipeline_process(sw_pipeline* p, const void* taftp, size_t bytes, int on_device)
@@ -692,6 +830,7 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
         }
     }
     SW_HIP_TRY(hipEventRecord(p->ev_w[1], st));
+    if(c.graph && !p->scalings_dirty) return run_gulp_graph(p, in, consumed, par, utc_offset_gulp);
     SW_HIP_TRY(hipEventRecord(p->ev[0], st));
     // the gulp is held time-major (FPAT) up to the dedisperser; the two FPAT
     // buffers alternate so that the previous gulp stays readable as history
@@ -720,17 +859,7 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
         SW_HIP_TRY(hipStreamSynchronize(st));  // h is freed at the end of this scope
         p->scalings_dirty = false;
     }
-    float* cb_offsets = p->scl;
-    float* cb_scaling = p->scl + S * F;
-    float* ib_offsets = p->scl + 2 * S * F;
-    float* ib_scaling = p->scl + 3 * S * F;
-
-    // Stokes I with 0/1 antenna weights (what delay models give): the MFMA
-    // beamformer (and the incoherent beam, inside it where supported, else
-    // its own kernel) reads the dedisperser's time-major output ring
-    // directly, with no corner turn to FTPA
-    bool ring_direct = p->use_mfma && c.stokes == SW_STOKES_I && 2 * A <= 256 && c.nsamples % 8 == 0;
-    for(float w : p->h_antw) ring_direct = ring_direct && (w == 0.0f || w == 1.0f);
+    const bool ring_direct = beams_on_ring(p);
     const bool fuse_ib = ring_direct && sw::cb_fused_ib_supported(d);
     if(!ring_direct)
         for(auto& b : p->ftpa_dd)
@@ -743,12 +872,8 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
     if(rc) return rc;
     SW_HIP_TRY(hipEventRecord(p->ev[3], st));
 
-    const int nv = p->nv();
     const size_t T2 = p->tout();
     for(int dm = 0; dm < c.ndms; ++dm) {
-        int fl = 0, ov = 0, lat = 0;
-        sw_codedisp_info(p->cd, dm, &fl, &ov, &lat);
-        const double tsamp = c.nchans / c.bandwidth;
         if(wk) {  // this DM's output buffers of this gulp parity, once their last blocks are delivered
             p->cb_out = p->cb_bufs[2 * dm + par];
             p->ib_q = p->ib_bufs[2 * dm + par];
@@ -756,46 +881,12 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
             wk->wait_free(p->ib_q);
         }
         SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm], st));
-        const void* ring = nullptr;
-        int ld = 0, w0 = 0;
-        if(ring_direct) {
-            rc = sw_codedisp_window(p->cd, dm, &ring, &ld, &w0);
-            if(rc) return rc;
-            // with the incoherent beam fused into the coherent beamformer
-            // (which then reads the ring once) there is nothing to do here
-            if(!fuse_ib) rc = sw::ib_beamform_tm(ring, ld, w0, p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, d, st);
-        } else {
-            rc = sw_ib_beamform(p->ftpa_dd[dm], p->ib_raw, p->ib_q, ib_scaling, ib_offsets, p->antw, &d, st);
-        }
-        if(rc) return rc;
-        SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm + 1], st));
-        if(ring_direct)
-            rc = sw::cb_beamform_packed_tm(ring, ld, w0, p->packed, cb_scaling, cb_offsets, p->bmap, p->ib_raw,
-                                           p->cb_out, d, st, fuse_ib ? p->ib_raw : nullptr, p->ib_q, ib_scaling,
-                                           ib_offsets, p->antw);
-        else if(p->use_mfma)
-            rc = sw_cb_beamform_packed(p->ftpa_dd[dm], p->packed, cb_scaling, cb_offsets, p->bmap,
-                                       p->ib_raw, p->cb_out, &d, st);
-        else
-            rc = sw_cb_beamform(p->ftpa_dd[dm], p->weights, cb_scaling, cb_offsets, p->bmap, p->ib_raw,
-                                p->cb_out, &d, SW_CB_PLAIN, nullptr, st);
+        rc = enqueue_beams(p, dm, ring_direct, fuse_ib, st, p->ev_dm[4 * dm + 1]);
         if(rc) return rc;
         SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm + 2], st));
-        // the dedispersed block carries the filter delay tsamp*O/2 and, when
-        // the FFT hop does not divide the gulp, the dedisperser's latency
-        const double utc = utc_offset_gulp + tsamp * (ov / 2.0 - lat);
-        sw_block blk;
-        fill_block(blk, SW_BLOCK_CB_TFB, p->cb_out, T2 * F * c.nbeams * nv, "TFB", {T2, (size_t)F, (size_t)c.nbeams});
-        blk.dm_idx = dm;
-        blk.reference_dm = p->dms[dm];
-        blk.tsamp = tsamp * c.tscrunch;
-        blk.utc_offset = utc;
-        sw_block iblk;
-        fill_block(iblk, SW_BLOCK_IB_BTF, p->ib_q, (size_t)S * T2 * F * nv, "BTF", {(size_t)S, T2, (size_t)F});
-        iblk.dm_idx = dm;
-        iblk.reference_dm = p->dms[dm];
-        iblk.tsamp = tsamp * c.tscrunch;
-        iblk.utc_offset = utc;
+        sw_block blk, iblk;
+        double utc = 0;
+        dm_blocks(p, dm, utc_offset_gulp, blk, iblk, utc);
         if(wk) {
             hipEvent_t ready = p->ev_ready[2 * dm + par];
             SW_HIP_TRY(hipEventRecord(ready, st));
@@ -808,7 +899,7 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
             if(rc) return rc;
         }
         if(p->idp) {
-            rc = sw_idpipe_push(p->idp, dm, p->cb_out, T2, 1, utc, tsamp * c.tscrunch, st);
+            rc = sw_idpipe_push(p->idp, dm, p->cb_out, T2, 1, utc, blk.tsamp, st);
             if(rc) return rc;
         }
         SW_HIP_TRY(hipEventRecord(p->ev_dm[4 * dm + 3], st));
@@ -816,10 +907,7 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
     SW_HIP_TRY(hipEventRecord(p->ev[4], st));
     {
         sw_block blk;
-        fill_block(blk, SW_BLOCK_STATS_FPA, p->stats, sizeof(float) * 4 * F * 2 * A, "FPA",
-                   {(size_t)F, 2, (size_t)A});
-        blk.tsamp = c.nchans / c.bandwidth * c.nsamples;
-        blk.utc_offset = utc_offset_gulp;
+        stats_block(p, utc_offset_gulp, blk);
         if(wk) {
             if(p->st_sink) wk->push({p->st_sink, p->st_user, blk, p->ev_ready[2 * c.ndms + par], p->stats});
         } else {
@@ -856,4 +944,220 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
     return SW_OK;
 }
 
+int run_gulp_graph(sw_pipeline* p, const void* in, hipEvent_t consumed, int par, double utc_offset_gulp)
+{
+    const auto& c      = p->cfg;
+    hipStream_t st     = p->stream;
+    SinkWorker* wk     = p->worker.get();
+    const sw_bf_dims d = bf_dims(p);
+    const bool ring_direct = beams_on_ring(p);
+    const bool fuse_ib = ring_direct && sw::cb_fused_ib_supported(d);
+    // allocations and buffer waits before the capture
+    if(!ring_direct)
+        for(auto& b : p->ftpa_dd)
+            if(!b) SW_HIP_TRY(hipMalloc(&b, p->ftpa_bytes()));
+    for(int dm = 0; dm < c.ndms; ++dm) {
+        wk->wait_free(p->cb_bufs[2 * dm + par]);
+        wk->wait_free(p->ib_bufs[2 * dm + par]);
+    }
+    std::vector<void*> outs(p->ftpa_dd.begin(), p->ftpa_dd.end());
+    if(ring_direct) std::fill(outs.begin(), outs.end(), nullptr);
+    SW_HIP_TRY(hipEventRecord(p->ev[0], st));
+    SW_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = SW_OK;
+    {
+        std::swap(p->ftpa, p->ftpa_prev);
+        rc = sw_transpose_fpat_stats(in, p->ftpa, p->stats, c.nsamples / 256, p->header.nantennas, c.nantennas,
+                                     c.nchans, p->stats_ws, st);
+        if(!rc) rc = sw_codedisp_run(p->cd, p->ftpa, outs.data(), nullptr, 0, c.ndms, c.batched_dms, st);
+        if(!rc) rc = sw_codedisp_push_history(p->cd, p->ftpa, st);
+        for(int dm = 0; dm < c.ndms && !rc; ++dm) {
+            p->cb_out = p->cb_bufs[2 * dm + par];
+            p->ib_q = p->ib_bufs[2 * dm + par];
+            rc = enqueue_beams(p, dm, ring_direct, fuse_ib, st, nullptr);
+        }
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &g);
+    if(rc) {
+        if(g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if(ec != hipSuccess) {
+        sw::set_error(std::string("sw_pipeline_process: hipStreamEndCapture failed: ") + hipGetErrorString(ec));
+        return SW_ERR_HIP;
+    }
+    if(p->gexec) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res{};
+        if(hipGraphExecUpdate(p->gexec, g, &err_node, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(p->gexec);
+            p->gexec = nullptr;
+        }
+    }
+    if(!p->gexec) {
+        const hipError_t e = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+        if(e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            p->gexec = nullptr;
+            sw::set_error(std::string("sw_pipeline_process: hipGraphInstantiate failed: ") + hipGetErrorString(e));
+            return SW_ERR_HIP;
+        }
+        ++p->graph_instantiations;
+    }
+    (void)hipGraphDestroy(g);
+    SW_HIP_TRY(hipGraphLaunch(p->gexec, st));
+    if(consumed) SW_HIP_TRY(hipEventRecord(consumed, st));
+    hipEvent_t ready = p->ev_ready[2 * c.ndms + par];
+    SW_HIP_TRY(hipEventRecord(ready, st));
+    for(int dm = 0; dm < c.ndms; ++dm) {
+        p->cb_out = p->cb_bufs[2 * dm + par];
+        p->ib_q = p->ib_bufs[2 * dm + par];
+        sw_block blk, iblk;
+        double utc = 0;
+        dm_blocks(p, dm, utc_offset_gulp, blk, iblk, utc);
+        if(p->cb_sink) wk->push({p->cb_sink, p->cb_user, blk, ready, p->cb_out});
+        if(p->ib_sink) wk->push({p->ib_sink, p->ib_user, iblk, ready, p->ib_q});
+        if(p->idp) {
+            rc = sw_idpipe_push(p->idp, dm, p->cb_out, p->tout(), 1, utc, blk.tsamp, st);
+            if(rc) return rc;
+        }
+    }
+    {
+        sw_block blk;
+        stats_block(p, utc_offset_gulp, blk);
+        if(p->st_sink) wk->push({p->st_sink, p->st_user, blk, ready, p->stats});
+    }
+    SW_HIP_TRY(hipEventRecord(p->ev[4], st));
+    SW_HIP_TRY(hipStreamSynchronize(st));
+    // the graph is timed as a whole: its time is reported as "other"
+    float t = 0;
+    for(int i = 0; i < SW_NSTAGES; ++i) p->stage_ms[i] = 0;
+    SW_HIP_TRY(hipEventElapsedTime(&t, p->ev_w[0], p->ev_w[1]));
+    p->stage_ms[6] = t;
+    SW_HIP_TRY(hipEventElapsedTime(&t, p->ev[0], p->ev[4]));
+    p->stage_ms[7] = t;
+    ++p->call_count;
+    return SW_OK;
+}
+
 } // namespace
+
+namespace {
+// deterministic pseudo-random complex int8 of moderate amplitude (|x| < 64)
+__global__ void tune_voltages_kernel(uint32_t* x, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    uint32_t h = (uint32_t)i * 0x9E3779B1u + 0x7F4A7C15u;
+    h ^= h >> 15;
+    h *= 0x85EBCA77u;
+    h ^= h >> 13;
+    x[i] = h & 0x3F3F3F3Fu;
+}
+
+std::string json_list(const std::vector<float>& v, int prec)
+{
+    std::string s = "[";
+    char b[64];
+    for(size_t i = 0; i < v.size(); ++i) {
+        std::snprintf(b, sizeof(b), "%s%.*f", i ? ", " : "", prec, (double)v[i]);
+        s += b;
+    }
+    return s + "]";
+}
+} // namespace
+
+extern "C" int sw_pipeline_autotune(const sw_pipeline_config* cfg, int tune_chans, char* json, size_t json_len)
+{
+    SW_CHECK_ARG(cfg && json && json_len > 0 && tune_chans > 0, "bad argument");
+    SW_CHECK_ARG(cfg->nchans > 0 && cfg->nsamples > 0 && cfg->nantennas > 0 && cfg->ndms > 0, "empty workload");
+    const sw_pipeline_config& c = *cfg;
+    const int Fs = std::min(tune_chans, c.nchans);
+    sw_pipeline_config sub = c;
+    sub.nchans = Fs;
+    sub.bandwidth = c.bandwidth / c.nchans * Fs;  // same channel width
+    sub.async_sinks = 0;
+    sub.graph = 0;
+    sub.id_enable = 0;
+    sub.autotune_fft = 0;
+    sub.tuning_file = "";  // no table: every tile as given here
+    // the FFT plan: the configuration's, else the FFT autotuner's
+    std::vector<int> fl(c.ndms, 0);
+    int share = c.share_forward;
+    if(c.fft_len) {
+        fl.assign(c.fft_len, c.fft_len + c.ndms);
+    } else {
+        sw_codedisp_config cd{};
+        cd.nchans = Fs;
+        cd.nantennas = c.nantennas;
+        cd.nsamples = c.nsamples;
+        cd.low_freq = c.centre_frequency - c.bandwidth / 2.0;
+        cd.chan_bw = c.bandwidth / c.nchans;
+        cd.tsamp = c.nchans / c.bandwidth;
+        cd.ndms = c.ndms;
+        cd.dms = c.coherent_dms;
+        cd.overlap = c.overlap;
+        cd.input_fpat = 1;
+        int rc = sw_codedisp_autotune(&cd, Fs, fl.data(), &share, nullptr, nullptr);
+        if(rc) return rc;
+    }
+    sub.fft_len = fl.data();
+    sub.share_forward = share;
+    // synthetic gulp and weights
+    const size_t gulp = (size_t)c.nsamples * c.nantennas * Fs * 4;
+    void* d_in = nullptr;
+    SW_HIP_TRY(hipMalloc(&d_in, gulp));
+    hipLaunchKernelGGL(tune_voltages_kernel, dim3((unsigned)((gulp / 4 + 255) / 256)), dim3(256), 0, nullptr,
+                       static_cast<uint32_t*>(d_in), gulp / 4);
+    SW_LAUNCH_CHECK();
+    std::vector<int8_t> w((size_t)Fs * c.nbeams * c.nantennas * 2);
+    sw_synth_phase_weights(w.data(), w.size() / 2, 0x5EED1000);
+    static const int kWaves[] = {2, 4, 8};
+    float ms[3] = {-1, -1, -1};
+    int best = -1;
+    int rc = SW_OK;
+    for(int k = 0; k < 3 && rc == SW_OK; ++k) {
+        sub.cb_waves = kWaves[k];
+        sw_pipeline* p = nullptr;
+        rc = sw_pipeline_create(&sub, &p);
+        if(rc) break;
+        sw_obs_header h{};
+        h.nantennas = c.nantennas;
+        h.utc_start = 1708082229.0;
+        h.sample_clock = 1712e6;
+        rc = sw_pipeline_set_weights(p, w.data());
+        if(!rc) rc = sw_pipeline_init(p, &h);
+        float acc = 0.0f;
+        for(int it = 0; it < 5 && rc == SW_OK; ++it) {  // two warm-up gulps, three timed
+            rc = sw_pipeline_process(p, d_in, gulp, 1);
+            if(rc == SW_OK && it >= 2) acc += p->stage_ms[3] + p->stage_ms[4];
+        }
+        sw_pipeline_destroy(p);
+        if(rc == SW_ERR_UNSUPPORTED) {  // this shape does not fit the workload (LDS): not a candidate
+            rc = SW_OK;
+            continue;
+        }
+        if(rc) break;
+        ms[k] = acc / 3.0f * (float)c.nchans / (float)Fs;
+        if(best < 0 || ms[k] < ms[best]) best = k;
+    }
+    (void)hipFree(d_in);
+    if(rc) return rc;
+    SW_CHECK_ARG(best >= 0, "autotune: no beamformer workgroup shape runs this workload");
+    static const char* stokes[] = {"I", "Q", "U", "V", "IQUV"};
+    std::vector<float> dms(c.coherent_dms, c.coherent_dms + c.ndms), fls(fl.begin(), fl.end());
+    char buf[2048];
+    const int n = std::snprintf(
+        buf, sizeof(buf),
+        "{\"antennas\": %d, \"beams\": %d, \"samples\": %d, \"tscrunch\": %d, \"stokes\": \"%s\", "
+        "\"cb_waves\": %d, \"dms\": %s, \"fft_len\": %s, \"share_forward\": %d, \"channels\": %d, "
+        "\"tuned_on_channels\": %d, \"beams_ms_per_gulp\": {\"2\": %.3f, \"4\": %.3f, \"8\": %.3f}}",
+        c.nantennas, c.nbeams, c.nsamples, c.tscrunch, stokes[c.stokes < 0 || c.stokes > 4 ? 0 : c.stokes],
+        kWaves[best], json_list(dms, 4).c_str(), json_list(fls, 0).c_str(), share, c.nchans, Fs, ms[0], ms[1],
+        ms[2]);
+    SW_CHECK_ARG(n > 0 && (size_t)n < json_len, "json buffer too small");
+    std::memcpy(json, buf, (size_t)n + 1);
+    return SW_OK;
+}

@@ -295,3 +295,38 @@ def test_corner_turn(sw, dev, F, R, C):
     d_y = _empty((F, C, R, 2), np.int8, dev)
     sw.corner_turn(d_x, d_y, F, R, C)
     np.testing.assert_array_equal(_to_host(d_y), x.transpose(0, 2, 1, 3))
+
+
+@pytest.mark.parametrize("stokes,A,ts", [("I", 64, 16), ("I", 128, 8), ("IQUV", 80, 16), ("I", 16, 1),
+                                         ("IQUV", 32, 32)])
+def test_cb_workgroup_shapes_identical(sw, orc, dev, stokes, A, ts):
+    """The MFMA beamformer with 2, 4 and 8 waves (of 32 time rows) per
+    workgroup (the tuning table's beamformer tile) gives the same beams,
+    equal to the oracle's; T = 800 leaves every shape a partial last
+    workgroup."""
+    print(f"cb workgroup shapes stokes={stokes} A={A} ts={ts}", flush=True)
+    F, T, B, S = 2, 800, 72, 2
+    v = _rand_c8((F, T, 2, A), 600 + A + ts)
+    w = _phase_weights((F, B, A), 601 + A)
+    bmap = (np.arange(B) % S).astype(np.int32)
+    antw = np.ones((S, A), np.float32)
+    antw[1, ::4] = 0
+    cbs, cbo = _cb_scales(F, S, A, ts)
+    ibs, ibo = _ib_scales(F, S, A, ts)
+    nv = 4 if stokes == "IQUV" else 1
+    d_v, d_antw = _to_dev(v, dev), _to_dev(antw, dev)
+    shp = (S, T // ts, F) + ((4,) if nv == 4 else ())
+    ib_raw, ib_q = _empty(shp, np.float32, dev), _empty(shp, np.int8, dev)
+    sw.ib_beamform(d_v, ib_raw, ib_q, _to_dev(ibs, dev), _to_dev(ibo, dev), d_antw, sw.dims(F, T, A, B, ts, stokes, S))
+    d = sw.dims(F, T, A, B, ts, stokes, S, True)
+    packed = _empty((F * B * A * 4,), np.int8, dev)
+    sw.cb_pack_weights(_to_dev(w, dev), packed, F, B, A)
+    outs = []
+    for waves in (2, 4, 8):
+        d_out = _empty((T // ts, F, B) + ((4,) if nv == 4 else ()), np.int8, dev)
+        sw.cb_beamform_packed_tile(d_v, packed, _to_dev(cbs, dev), _to_dev(cbo, dev), _to_dev(bmap, dev), ib_raw,
+                                   d_out, d, waves)
+        outs.append(_to_host(d_out))
+    exp = orc.cb_beamform(v, w, ts, cbs, cbo, antw, bmap, S, stokes, True)
+    for o in outs:
+        np.testing.assert_array_equal(o, exp)

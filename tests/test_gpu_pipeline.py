@@ -530,3 +530,96 @@ def test_pipeline_async_sink_exception_propagates(sw, dev):
         p(g)
         p.synchronize()
     p.close()
+
+
+@pytest.mark.parametrize("stokes,form", [("I", "mfma"), ("IQUV", "mfma"), ("I", "plain")])
+def test_pipeline_graph_mode_matches_stream_launches(sw, dev, stokes, form):
+    """Graph mode (each gulp after the first captured into a hip graph whose
+    executable is updated in place and launched) against the same gulps
+    launched on the stream: six gulps, three DMs whose segment grids move
+    from gulp to gulp (the kernels' arguments change every gulp), the sinks
+    downloading every block between the launches; every block of every
+    replay is identical.  With Stokes I the coherent beams also feed the
+    incoherent dedispersion, which runs after each graph."""
+    from skyweaver_amd import DeviceBuffer as DB
+    H = T // 256
+    gulps = [DB.from_host(sw.synth_voltages((H, A, F, 256, 2, 2), seed=0x5EED0701 + i, sigma=16.0)) for i in range(6)]
+    weights = sw.synth_weights((F, B, A, 2), seed=0x5EED1701)
+    dms, fft_len = [0.0, 20.0, 50.0], [1024, 2048, 1024]
+    idms = np.arange(0, 20, 2.5, dtype=np.float32)
+    id_plan = [(2, idms)] * 3 if stokes == "I" else None
+
+    def run(graph):
+        p = sw.BeamformerPipeline(A, F, B, T, TS, dms, CFREQ, BW, stokes=stokes, fft_len=fft_len, form=form,
+                                  batched=form == "mfma", async_sinks=True, graph=graph, id_plan=id_plan,
+                                  id_dispatch_size=200 if id_plan else 0)
+        p.set_weights(weights)
+        p.init(A)
+        got = []
+
+        def grab(key):
+            def f(blk):
+                got.append((key, blk.dm_idx, blk.utc_offset, sw.download_block(blk)))
+            return f
+        p.set_sinks(cb=grab("cb"), ib=grab("ib"), stats=grab("stats"))
+        if id_plan:
+            p.set_tdb_sink(grab("tdb"))
+        for g in gulps:
+            p(g)
+        p.synchronize()
+        n = len(got)
+        p.close()
+        return got, n
+
+    ref, nref = run(False)
+    got, n = run(True)
+    assert n == nref and nref >= 6 * (2 * len(dms) + 1)
+    if id_plan:
+        assert sum(1 for x in ref if x[0] == "tdb") >= 3
+    key = lambda x: (x[0], x[1], x[2])
+    # the TDB blocks come from the stream in both runs; the sink thread may
+    # deliver them between other blocks, so compare per kind in order
+    for kind in ("cb", "ib", "stats", "tdb"):
+        a = [x for x in ref if x[0] == kind]
+        b = [x for x in got if x[0] == kind]
+        assert [key(x) for x in a] == [key(x) for x in b]
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x[3], y[3])
+
+
+def test_pipeline_tiles_from_table_identical(sw, dev, tmp_path):
+    """Workgroup shapes from a tuning table (2 and 8 waves, on the
+    dedisperser's ring with the fused incoherent beam) and the table's FFT
+    lengths: the pipeline reports where its tiles came from, and its blocks
+    equal those of the default tiles."""
+    import json
+    from skyweaver_amd import DeviceBuffer as DB
+    H = T // 256
+    gulps = [DB.from_host(sw.synth_voltages((H, A, F, 256, 2, 2), seed=0x5EED0801 + i, sigma=16.0)) for i in range(2)]
+    weights = sw.synth_weights((F, B, A, 2), seed=0x5EED1801)
+    dms = [0.0, 50.0]
+
+    def run(**kw):
+        p = sw.BeamformerPipeline(A, F, B, T, TS, dms, CFREQ, BW, **kw)
+        p.set_weights(weights)
+        p.init(A)
+        got = []
+        p.set_sinks(cb=lambda blk: got.append(sw.download_block(blk)), ib=lambda blk: got.append(sw.download_block(blk)))
+        for g in gulps:
+            p(g)
+        tile = p.tile()
+        p.close()
+        return got, tile
+
+    ref, t0 = run(fft_len=[1024, 2048], tuning_file="")
+    assert t0 == {"cb_waves": 4, "fft_len": [1024, 2048], "share_forward": False, "source": "default"}
+    for waves in (2, 8):
+        table = tmp_path / f"t{waves}.json"
+        table.write_text(json.dumps({"entries": [
+            {"antennas": A, "beams": B, "samples": T, "tscrunch": TS, "stokes": "I", "cb_waves": waves,
+             "dms": dms, "fft_len": [1024, 2048], "share_forward": 0}]}))
+        got, t = run(tuning_file=str(table))
+        assert t == {"cb_waves": waves, "fft_len": [1024, 2048], "share_forward": False, "source": "table+fft"}
+        assert len(got) == len(ref)
+        for x, y in zip(ref, got):
+            np.testing.assert_array_equal(x, y)

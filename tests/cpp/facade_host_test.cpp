@@ -18,7 +18,11 @@ using namespace skyweaver_amd;
 
 int main(int argc, char** argv)
 {
-    const std::string outdir = argc > 1 ? argv[1] : "facade_out";
+    if(argc < 2) {
+        std::fprintf(stderr, "usage: %s OUTPUT_DIRECTORY\n", argv[0]);
+        return 2;
+    }
+    const std::string outdir = argv[1];
     // DedispersionPlan: the three block forms
     DedispersionPlan plan;
     plan.add_block("100:0:1000:0.9765625:1");

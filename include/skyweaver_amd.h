@@ -70,6 +70,14 @@ int sw_host_free(void* h_ptr);
 int sw_memcpy_htod(void* d_dst, const void* h_src, size_t bytes, void* stream);
 int sw_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes, void* stream);
 int sw_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes, void* stream);
+/* Synchronous device-to-host copy on a non-blocking stream of the calling
+ * thread's own (created on the thread's first call, reused by later
+ * threads once it ends); returns once the bytes are in h_dst.  It never
+ * touches the legacy default stream, so a sink may call it while the
+ * pipeline captures its next gulp into a hip graph.  No counterpart in the
+ * reference (its writers copy with thrust on the default stream,
+ * MultiFileWriter.cu:233-239). */
+int sw_memcpy_dtoh_sync(void* h_dst, const void* d_src, size_t bytes);
 int sw_memset(void* d_ptr, int value, size_t bytes, void* stream);
 int sw_stream_create(void** stream); /* default priority */
 int sw_stream_destroy(void* stream);
@@ -266,6 +274,17 @@ int sw_idisp_dedisperse_ftb(sw_idisp* plan, const void* d_ftb, size_t nsamples, 
  * arrays) select O = even-ceil of the intra-channel smear at the bottom of
  * the band and N = the smallest power of two >= max(1024, 16 O).
  * With share_forward, DMs of equal N get the largest O among them. */
+/* Frequency of FFT bin k of channel c in the dedispersion chirp
+ * H = exp(i pref nu^2 / ((nu0 + nu) nu0^2)), pref = -2 pi K DM, bandwidth B
+ * per channel, low_freq the band's bottom edge, dnu = B / N:
+ *  SW_CHIRP_REFERENCE: nu0 = low_freq + (c - 1/2) B, nu = k dnu for every k
+ *    (DMResponse, cpp/skyweaver/src/CoherentDedisperser.cu:313-335; the
+ *    default, and what the reference's golden output was made with);
+ *  SW_CHIRP_PHYSICAL: nu0 = low_freq + (c + 1/2) B (the channel centre),
+ *    nu = k dnu for k < N/2 and (k - N) dnu above (the negative baseband
+ *    frequencies of an unshifted FFT), so every frequency of the channel is
+ *    delayed to the centre's arrival time.  No counterpart in the reference. */
+enum sw_chirp { SW_CHIRP_REFERENCE = 0, SW_CHIRP_PHYSICAL = 1 };
 typedef struct sw_codedisp_config {
     int nchans;        /* F */
     int nantennas;     /* A (FTPA padded antenna count) */
@@ -287,6 +306,7 @@ typedef struct sw_codedisp_config {
                              largest of their own, so that they lie on one
                              segment grid and a batched run computes each
                              segment's forward transform once for all of them */
+    int chirp;            /* enum sw_chirp: which frequency FFT bin k stands for */
 } sw_codedisp_config;
 typedef struct sw_codedisp sw_codedisp;
 
@@ -496,7 +516,10 @@ typedef struct sw_pipeline_config {
      * executable graph is updated in place from gulp to gulp.  The blocks are
      * released to the sinks, and the incoherent dedispersion is fed, after
      * the graph; results are identical to the stream launches.  Stage times
-     * then report the whole graph as "other". */
+     * then report the whole graph as "other".  The sinks run on the
+     * library's thread while the next gulp is captured, so they must not
+     * use the legacy default (null) stream, which a capture forbids: copy
+     * blocks out with sw_memcpy_dtoh_sync or on a non-blocking stream. */
     int graph;
     /* Coherent beamformer workgroup shape (sw_cb_beamform_packed_tile): 2, 4
      * or 8 waves of 32 time rows; 0: the tuning table's entry for this
@@ -506,6 +529,7 @@ typedef struct sw_pipeline_config {
     int cb_waves;
     const char* tuning_file; /* NULL: $SKYWEAVER_AMD_TUNING, else tuning/gfx950.json
                                 in the library's package directory */
+    int chirp;               /* enum sw_chirp of the coherent dedisperser */
 } sw_pipeline_config;
 
 typedef struct sw_obs_header {

@@ -241,6 +241,15 @@ class PipelineConfig {
     // them and rethrows a handler's exception
     bool async_handlers() const { return _async; }
     void async_handlers(bool v) { _async = v; }
+    // each gulp after the first captured into a hip graph and launched as one
+    // (sw_pipeline_config::graph; needs async_handlers)
+    bool graph_mode() const { return _graph; }
+    void graph_mode(bool v) { _graph = v; }
+    // the dedispersion chirp's bin-to-frequency mapping: the reference's
+    // (false, the default) or the physical one, channel-centred with the
+    // upper half of the bins at negative baseband frequencies (true)
+    bool physical_chirp() const { return _physical_chirp; }
+    void physical_chirp(bool v) { _physical_chirp = v; }
 
     // input (MultiFileReader, run_pipeline)
     std::vector<std::string> const& input_files() const { return _input_files; }
@@ -265,7 +274,7 @@ class PipelineConfig {
     float _level = 24.0f;
     int _cb_form = SW_CB_MFMA;
     std::vector<int> _fft_len;
-    bool _share_forward = false, _autotune_fft = false, _async = false;
+    bool _share_forward = false, _autotune_fft = false, _async = false, _graph = false, _physical_chirp = false;
     std::vector<std::string> _input_files;
     bool _check_contiguity = false;
     float _start_time = 0.0f, _duration = std::numeric_limits<float>::infinity();
@@ -412,8 +421,7 @@ class BlockView {
     void copy_to_host(void* dst) const
     {
         if(on_device()) {
-            check(sw_memcpy_dtoh(dst, _b.data, _b.bytes, nullptr));
-            check(sw_stream_synchronize(nullptr));
+            check(sw_memcpy_dtoh_sync(dst, _b.data, _b.bytes));
         } else {
             std::memcpy(dst, _b.data, _b.bytes);
         }
@@ -564,6 +572,8 @@ class BeamformerPipeline {
         c.share_forward = config.share_forward() ? 1 : 0;
         c.autotune_fft = config.autotune_fft() ? 1 : 0;
         c.async_sinks = config.async_handlers() ? 1 : 0;
+        c.graph = config.graph_mode() ? 1 : 0;
+        c.chirp = config.physical_chirp() ? SW_CHIRP_PHYSICAL : SW_CHIRP_REFERENCE;
         check(sw_pipeline_create(&c, &_p));
         try {
             check(sw_pipeline_set_sinks(_p, &detail::cb_trampoline<CBHandler>, &_cb,

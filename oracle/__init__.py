@@ -63,6 +63,7 @@ def _declare(L):
     L.orc_incoherent_dedisperse.restype = _c
     L.orc_fft.argtypes = [_f64p, _c, _c]
     L.orc_dm_chirp.argtypes = [_f, _d, _d, _c, _c, _f64p]
+    L.orc_dm_chirp_physical.argtypes = [_f, _d, _d, _c, _c, _f64p]
     L.orc_codedisp_channel.argtypes = [_i8p, _l, _c, _f64p, _c, _c, _l, _f64p, _i8p, _c]
     L.orc_weights.argtypes = [_f32p, _i8p, _f64p, _c, _c, _c, _d, _d]
     L.orc_parse_beamsets.argtypes = [_f32p, _c, _c, _c, _c, _i32p, _f32p, _c]
@@ -172,10 +173,11 @@ def fft(x: np.ndarray, inverse=False) -> np.ndarray:
     return buf[:, 0] + 1j * buf[:, 1]
 
 
-def dm_chirp(dm, low_freq, coarse_chan_bw, nchans, fft_len) -> np.ndarray:
-    """FP64 chirp [nchans][N] complex128."""
+def dm_chirp(dm, low_freq, coarse_chan_bw, nchans, fft_len, physical=False) -> np.ndarray:
+    """FP64 chirp [nchans][N] complex128 (physical: orc_dm_chirp_physical's mapping)."""
     out = np.zeros((nchans, fft_len, 2), np.float64)
-    lib().orc_dm_chirp(float(dm), float(low_freq), float(coarse_chan_bw), nchans, fft_len, _p(out, _f64p))
+    fn = lib().orc_dm_chirp_physical if physical else lib().orc_dm_chirp
+    fn(float(dm), float(low_freq), float(coarse_chan_bw), nchans, fft_len, _p(out, _f64p))
     return out[..., 0] + 1j * out[..., 1]
 
 

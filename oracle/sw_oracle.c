@@ -473,6 +473,26 @@ void orc_dm_chirp(float dm, double low_freq, double coarse_chan_bw, int nchans,
     }
 }
 
+void orc_dm_chirp_physical(float dm, double low_freq, double coarse_chan_bw, int nchans,
+                           int fft_len, double* out)
+{
+    /* same prefactor and phase expression as orc_dm_chirp; nu0 is the
+     * channel centre and the upper half of the bins are negative offsets */
+    const double two_pi = 2 * 3.1415926535897932385;
+    const double pref   = -1.0f * two_pi * ORC_DM_CONST_HZ_S * (double)dm;
+    const double fine_chan_bw = coarse_chan_bw / fft_len;
+    for(int c = 0; c < nchans; ++c) {
+        const double nu_0 = low_freq + c * coarse_chan_bw + 0.5 * coarse_chan_bw;
+        for(int k = 0; k < fft_len; ++k) {
+            const double nu = (2 * k >= fft_len ? k - fft_len : k) * fine_chan_bw;
+            const double phase = nu * nu * pref / ((nu_0 + nu) * nu_0 * nu_0);
+            double* o = out + 2 * ((size_t)c * fft_len + k);
+            o[0] = cos(phase);
+            o[1] = sin(phase);
+        }
+    }
+}
+
 static inline int8_t wrap8(long v)
 {
     /* static_cast<char>(int): modulo 2^8, cpp/skyweaver/src/CoherentDedisperser.cu:217-221 */

@@ -113,6 +113,12 @@ void orc_fft(double* data, int n, int inverse);
  * DMPrefactor cpp/skyweaver/dedispersion_utils.cuh:42-48.  out [nchans][N][2] FP64. */
 void orc_dm_chirp(float dm, double low_freq, double coarse_chan_bw, int nchans,
                   int fft_len, double* out);
+/* The physical mapping of the same chirp (no reference counterpart): channel
+ * c's centre nu0 = low_freq + (c + 1/2) B, bin k at nu = k dnu for k < N/2 and
+ * (k - N) dnu above.  Pinned by the cold-plasma delay law (its phase slope)
+ * and by re-compressing a pulse dispersed in the time domain. */
+void orc_dm_chirp_physical(float dm, double low_freq, double coarse_chan_bw, int nchans,
+                           int fft_len, double* out);
 
 /* FP64 overlap-save coherent dedispersion of one channel.
  * tpa [nin][pa] complex int8 (pa = P*A columns); chirp [N][2].

@@ -66,7 +66,11 @@ class CodedispConfig(ctypes.Structure):
                 ("low_freq", ctypes.c_double), ("chan_bw", ctypes.c_double), ("tsamp", ctypes.c_double),
                 ("ndms", ctypes.c_int), ("dms", ctypes.POINTER(ctypes.c_float)),
                 ("fft_len", ctypes.POINTER(ctypes.c_int)), ("overlap", ctypes.POINTER(ctypes.c_int)),
-                ("input_fpat", ctypes.c_int), ("history_in_place", ctypes.c_int), ("share_forward", ctypes.c_int)]
+                ("input_fpat", ctypes.c_int), ("history_in_place", ctypes.c_int), ("share_forward", ctypes.c_int),
+                ("chirp", ctypes.c_int)]
+
+# enum sw_chirp
+CHIRP = {"reference": 0, "physical": 1}
 
 
 _vp = ctypes.c_void_p
@@ -117,6 +121,7 @@ def _declare(L):
     L.sw_host_free.argtypes = [_vp]
     L.sw_memcpy_htod.argtypes = [_vp, _vp, _sz, _vp]
     L.sw_memcpy_dtoh.argtypes = [_vp, _vp, _sz, _vp]
+    L.sw_memcpy_dtoh_sync.argtypes = [_vp, _vp, _sz]
     L.sw_memcpy_dtod.argtypes = [_vp, _vp, _sz, _vp]
     L.sw_memset.argtypes = [_vp, _c, _sz, _vp]
     L.sw_stream_create.argtypes = [_pvp]
@@ -586,12 +591,14 @@ class CoherentDedisperser:
     """Overlap-save coherent dedispersion plan (sw_codedisp_* in the C ABI)."""
 
     def __init__(self, nchans, nantennas, nsamples, low_freq, chan_bw, tsamp, dms, fft_len=None,
-                 overlap=None, stream=None, input_fpat=False, history_in_place=False, share_forward=False):
+                 overlap=None, stream=None, input_fpat=False, history_in_place=False, share_forward=False,
+                 chirp="reference"):
         """history_in_place (with input_fpat): the history is read from the
         gulp last given to push_history, which must stay unchanged until the
         next run.  share_forward: DMs of equal FFT length take the largest of
         their overlaps, so a batched run transforms each segment once for
-        all of them."""
+        all of them.  chirp: "reference" (the reference's bin-to-frequency
+        mapping) or "physical" (channel-centred, negative upper bins)."""
         import numpy as np
         self._dms = np.ascontiguousarray(dms, np.float32)
         n = len(self._dms)
@@ -601,7 +608,7 @@ class CoherentDedisperser:
                              self._dms.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                              self._fft.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                              self._ov.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 1 if input_fpat else 0,
-                             1 if history_in_place else 0, 1 if share_forward else 0)
+                             1 if history_in_place else 0, 1 if share_forward else 0, CHIRP[chirp])
         h = ctypes.c_void_p()
         check(lib().sw_codedisp_create(ctypes.byref(cfg), _stream(stream), ctypes.byref(h)))
         self.handle = h.value
@@ -678,7 +685,7 @@ class PipelineConfigC(ctypes.Structure):
                 ("id_enable", ctypes.c_int), ("id_tscrunch", ctypes.c_void_p), ("id_dms", ctypes.c_void_p),
                 ("id_ndms", ctypes.c_void_p), ("id_dispatch_size", ctypes.c_int), ("autotune_fft", ctypes.c_int),
                 ("share_forward", ctypes.c_int), ("async_sinks", ctypes.c_int), ("graph", ctypes.c_int),
-                ("cb_waves", ctypes.c_int), ("tuning_file", ctypes.c_char_p)]
+                ("cb_waves", ctypes.c_int), ("tuning_file", ctypes.c_char_p), ("chirp", ctypes.c_int)]
 
 
 class ObsHeaderC(ctypes.Structure):
@@ -758,7 +765,8 @@ class BeamformerPipeline:
     def __init__(self, nantennas, nchans, nbeams, nsamples, tscrunch, dms, centre_frequency, bandwidth,
                  stokes="I", ib_subtract=True, output_level=24.0, fft_len=None, overlap=None, form="mfma",
                  batched=True, total_nchans=4096, id_plan=None, id_dispatch_size=0, autotune_fft=False,
-                 share_forward=False, async_sinks=False, graph=False, cb_waves=0, tuning_file=None):
+                 share_forward=False, async_sinks=False, graph=False, cb_waves=0, tuning_file=None,
+                 chirp="reference"):
         """id_plan: None (no incoherent dedispersion) or, per coherent DM, (tscrunch, incoherent DMs).
         autotune_fft (with fft_len None): FFT lengths and share_forward chosen by sw_codedisp_autotune.
         share_forward: DMs of equal FFT length share one overlap (and, batched, their forward FFTs).
@@ -766,7 +774,8 @@ class BeamformerPipeline:
         DM alternate between two buffers); a sink's exception is raised by the next call.
         graph (with async_sinks): each gulp after the first is captured into a hip graph and launched.
         cb_waves: MFMA beamformer waves per workgroup (2, 4, 8; 0: the tuning table's, else 4);
-        tuning_file: the tuning table (None: the default one, "" none)."""
+        tuning_file: the tuning table (None: the default one, "" none).
+        chirp: the dedisperser's bin-to-frequency mapping, "reference" or "physical"."""
         import numpy as np
         L = lib()
         if not hasattr(L, "_pipeline_declared"):
@@ -788,6 +797,7 @@ class BeamformerPipeline:
         cfg.async_sinks = 1 if async_sinks else 0
         cfg.graph = 1 if graph else 0
         cfg.cb_waves = int(cb_waves)
+        cfg.chirp = CHIRP[chirp]
         self._tuning_file = None if tuning_file is None else str(tuning_file).encode()
         cfg.tuning_file = self._tuning_file
         if id_plan is not None:
@@ -902,8 +912,7 @@ def download_block(blk):
     """Copy a sink block (device memory) to a numpy array of its shape."""
     import numpy as np
     out = np.empty(blk.bytes, np.uint8)
-    check(lib().sw_memcpy_dtoh(out.ctypes.data, blk.data, blk.bytes, None))
-    check(lib().sw_stream_synchronize(None))
+    check(lib().sw_memcpy_dtoh_sync(out.ctypes.data, blk.data, blk.bytes))
     return out
 
 

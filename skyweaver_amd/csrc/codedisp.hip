@@ -678,14 +678,17 @@ __global__ void twiddle_kernel(float2* tw, int n)
 // [RP][L = N / RP] with k = RP m + r at r L + m', where m' = m, or for the
 // register kernels (R1 > 0) m = k1 + R1 k2 + 32 R1 k3 at m' = 32 R1 k3 + 32 k1 + k2
 __global__ void chirp_kernel(float2* out, int nchans, int N, int RP, int R1, double low_freq,
-                             double chan_bw, double dm_prefactor)
+                             double chan_bw, double dm_prefactor, int physical)
 {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(idx >= (size_t)nchans * N) return;
     const int chan = (int)(idx / N), k = (int)(idx % N);
     const double fine_bw = chan_bw / N;
-    const double nu_0 = low_freq + chan * chan_bw - 0.5 * chan_bw;
-    const double nu   = k * fine_bw;
+    // reference: nu_0 half a channel below the channel's bottom edge and
+    // nu = k dnu for every bin; physical: nu_0 the channel centre and the
+    // upper half of the bins the negative baseband frequencies
+    const double nu_0 = physical ? low_freq + chan * chan_bw + 0.5 * chan_bw : low_freq + chan * chan_bw - 0.5 * chan_bw;
+    const double nu   = (physical && 2 * k >= N ? k - N : k) * fine_bw;
     const double phase_prefactor = nu * nu * dm_prefactor;
     const double phase = phase_prefactor / ((nu_0 + nu) * nu_0 * nu_0);
     double s, c;
@@ -908,6 +911,7 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
     SW_CHECK_ARG(cfg->nchans > 0 && cfg->nantennas > 0 && cfg->nsamples > 0, "empty dimensions");
     SW_CHECK_ARG(cfg->ndms > 0 && cfg->dms, "No coherent DMs specified");
     SW_CHECK_ARG(cfg->chan_bw > 0 && cfg->tsamp > 0 && cfg->low_freq > 0, "bad band description");
+    SW_CHECK_ARG(cfg->chirp == SW_CHIRP_REFERENCE || cfg->chirp == SW_CHIRP_PHYSICAL, "unknown chirp mapping");
     hipStream_t st = sw::as_stream(stream);
     auto p = std::make_unique<sw_codedisp>();
     p->F = cfg->nchans;
@@ -1009,7 +1013,8 @@ extern "C" int sw_codedisp_create(const sw_codedisp_config* cfg, void* stream, s
         const double pref = -1.0f * (2 * 3.1415926535897932385) * sw::kDmConst * (double)p->dms[d];
         const size_t n = (size_t)p->F * N;
         hipLaunchKernelGGL(sw::chirp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                           p->chirp[d], p->F, N, RP, R1, p->low_freq, p->chan_bw, pref);
+                           p->chirp[d], p->F, N, RP, R1, p->low_freq, p->chan_bw, pref,
+                           cfg->chirp == SW_CHIRP_PHYSICAL ? 1 : 0);
         SW_LAUNCH_CHECK();
     }
     const size_t vol = (size_t)p->F * p->PA * p->T * 2;

@@ -214,6 +214,11 @@ struct sw_pipeline {
     std::vector<int8_t*> cb_bufs, ib_bufs;  // [dm][2]
     float* stats_bufs[2]{};
     std::vector<hipEvent_t> ev_ready;       // [dm][2], then the statistics' [2]
+    // the ready events are recorded on this stream, never captured, after a
+    // wait for the pipeline stream: the sink thread's hipEventSynchronize
+    // then never names an event of a stream that is capturing a graph
+    hipStream_t sig_stream = nullptr;
+    hipEvent_t ev_link     = nullptr;
     // graph mode (cfg.graph): the executable graph of the last gulp
     hipGraphExec_t gexec = nullptr;
     int graph_instantiations = 0;
@@ -246,6 +251,11 @@ struct sw_pipeline {
             stats = nullptr;
         }
         for(auto e : ev_ready) if(e) (void)hipEventDestroy(e);
+        if(sig_stream) {
+            (void)hipStreamSynchronize(sig_stream);
+            (void)hipStreamDestroy(sig_stream);
+        }
+        if(ev_link) (void)hipEventDestroy(ev_link);
         if(cd) sw_codedisp_destroy(cd);
         if(idp) sw_idpipe_destroy(idp);
         if(delays) sw_delay_model_close(delays);
@@ -397,6 +407,7 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
     cd.fft_len   = p->cfg.fft_len;
     cd.overlap   = p->cfg.overlap;
     cd.share_forward = p->cfg.share_forward;
+    cd.chirp     = c.chirp;
     std::vector<int> tuned(c.ndms, 0);
     if(c.autotune_fft && !c.fft_len) {
         cd.input_fpat = 1;
@@ -435,6 +446,8 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
         SW_HIP_TRY(hipMalloc(&p->stats_bufs[1], sizeof(float) * 4 * c.nchans * 2 * c.nantennas));
         p->ev_ready.assign(2 * c.ndms + 2, nullptr);
         for(auto& e : p->ev_ready) SW_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        SW_HIP_TRY(hipEventCreateWithFlags(&p->ev_link, hipEventDisableTiming));
+        SW_HIP_TRY(hipStreamCreateWithFlags(&p->sig_stream, hipStreamNonBlocking));
         p->worker = std::make_unique<SinkWorker>();
         p->worker->start();
     } else {
@@ -709,6 +722,16 @@ void stats_block(const sw_pipeline* p, double utc_offset_gulp, sw_block& blk)
     blk.utc_offset = utc_offset_gulp;
 }
 
+// Record `ready` once the work enqueued so far on the pipeline stream is
+// done, on the signal stream (the event the sink thread waits for).
+int signal_ready(sw_pipeline* p, hipEvent_t ready)
+{
+    SW_HIP_TRY(hipEventRecord(p->ev_link, p->stream));
+    SW_HIP_TRY(hipStreamWaitEvent(p->sig_stream, p->ev_link, 0));
+    SW_HIP_TRY(hipEventRecord(ready, p->sig_stream));
+    return SW_OK;
+}
+
 // The gulp's device work captured into a hip graph and launched: the
 // transpose with the statistics' sums, the dedispersion of every DM and
 // every DM's beams into this gulp's parity buffers.  The executable graph
@@ -841,7 +864,10 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
     rc = sw_transpose_fpat_stats(in, p->ftpa, p->stats, c.nsamples / 256, nant_in, A, F, p->stats_ws, st);
     if(rc) return rc;
     if(consumed) SW_HIP_TRY(hipEventRecord(consumed, st));
-    if(wk) SW_HIP_TRY(hipEventRecord(p->ev_ready[2 * c.ndms + par], st));
+    if(wk) {
+        rc = signal_ready(p, p->ev_ready[2 * c.ndms + par]);
+        if(rc) return rc;
+    }
     SW_HIP_TRY(hipEventRecord(p->ev[1], st));
     SW_HIP_TRY(hipEventRecord(p->ev[2], st));
     if(p->scalings_dirty) {
@@ -889,7 +915,8 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
         dm_blocks(p, dm, utc_offset_gulp, blk, iblk, utc);
         if(wk) {
             hipEvent_t ready = p->ev_ready[2 * dm + par];
-            SW_HIP_TRY(hipEventRecord(ready, st));
+            rc = signal_ready(p, ready);
+            if(rc) return rc;
             if(p->cb_sink) wk->push({p->cb_sink, p->cb_user, blk, ready, p->cb_out});
             if(p->ib_sink) wk->push({p->ib_sink, p->ib_user, iblk, ready, p->ib_q});
         } else {
@@ -1010,7 +1037,8 @@ int run_gulp_graph(sw_pipeline* p, const void* in, hipEvent_t consumed, int par,
     SW_HIP_TRY(hipGraphLaunch(p->gexec, st));
     if(consumed) SW_HIP_TRY(hipEventRecord(consumed, st));
     hipEvent_t ready = p->ev_ready[2 * c.ndms + par];
-    SW_HIP_TRY(hipEventRecord(ready, st));
+    rc = signal_ready(p, ready);
+    if(rc) return rc;
     for(int dm = 0; dm < c.ndms; ++dm) {
         p->cb_out = p->cb_bufs[2 * dm + par];
         p->ib_q = p->ib_bufs[2 * dm + par];
@@ -1100,6 +1128,7 @@ extern "C" int sw_pipeline_autotune(const sw_pipeline_config* cfg, int tune_chan
         cd.dms = c.coherent_dms;
         cd.overlap = c.overlap;
         cd.input_fpat = 1;
+        cd.chirp = c.chirp;
         int rc = sw_codedisp_autotune(&cd, Fs, fl.data(), &share, nullptr, nullptr);
         if(rc) return rc;
     }

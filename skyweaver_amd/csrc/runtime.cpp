@@ -293,6 +293,53 @@ extern "C" int sw_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes, void
     return SW_OK;
 }
 
+namespace sw {
+namespace {
+// Streams of sw_memcpy_dtoh_sync: one per live thread, returned to the pool
+// when the thread ends (no HIP call at thread or process exit) and reused.
+std::mutex g_copy_mu;
+std::vector<hipStream_t> g_copy_free;
+struct CopyStream {
+    hipStream_t s = nullptr;
+    ~CopyStream()
+    {
+        if(s) {
+            std::lock_guard<std::mutex> lk(g_copy_mu);
+            g_copy_free.push_back(s);
+        }
+    }
+};
+thread_local CopyStream g_copy_stream;
+
+int thread_copy_stream(hipStream_t* out)
+{
+    if(!g_copy_stream.s) {
+        {
+            std::lock_guard<std::mutex> lk(g_copy_mu);
+            if(!g_copy_free.empty()) {
+                g_copy_stream.s = g_copy_free.back();
+                g_copy_free.pop_back();
+            }
+        }
+        if(!g_copy_stream.s) SW_HIP_TRY(hipStreamCreateWithFlags(&g_copy_stream.s, hipStreamNonBlocking));
+    }
+    *out = g_copy_stream.s;
+    return SW_OK;
+}
+} // namespace
+} // namespace sw
+
+extern "C" int sw_memcpy_dtoh_sync(void* h_dst, const void* d_src, size_t bytes)
+{
+    SW_CHECK_ARG(h_dst && d_src, "null pointer");
+    hipStream_t s = nullptr;
+    const int rc = sw::thread_copy_stream(&s);
+    if(rc) return rc;
+    SW_HIP_TRY(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, s));
+    SW_HIP_TRY(hipStreamSynchronize(s));
+    return SW_OK;
+}
+
 extern "C" int sw_memcpy_dtod(void* d_dst, const void* d_src, size_t bytes, void* stream)
 {
     SW_CHECK_ARG(d_dst && d_src, "null pointer");

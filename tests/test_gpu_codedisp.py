@@ -106,6 +106,54 @@ def test_codedisp_vs_fp64_oracle(sw, orc, dev, dm, N):
     _check(out.download(), outf.download(), ei, ef, f"DM {dm} N {N}")
 
 
+@pytest.mark.parametrize("N", [512, 4096, 32768])
+def test_codedisp_physical_chirp_vs_fp64_oracle(sw, orc, dev, N):
+    """The physical bin-to-frequency mapping (channel-centred, upper bins
+    negative) through the LDS, register and two-half register kernels, against
+    the FP64 oracle with the same mapping."""
+    from skyweaver_amd import DeviceBuffer as DB
+    F, A, T, dm = 2, 64, 40000, 100.0
+    cur = sw.synth_voltages((F, T, 2, A, 2), seed=0x5EED0042, sigma=16.0)
+    plan = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, [dm], fft_len=[N], chirp="physical")
+    _, O = plan.info(0)
+    lat = plan.latency(0)
+    print(f"codedisp physical chirp dm={dm} N={N} O={O} latency={lat}", flush=True)
+    out = DB.empty(cur.shape, np.int8, fill=0x55)
+    outf = DB.empty(cur.shape, np.float32)
+    plan.run(DB.from_host(cur), [out], [outf])
+    chirp = orc.dm_chirp(dm, F_LOW, CHAN_BW, F, N, physical=True)
+    ei, ef = _oracle_stream(orc, np.zeros((F, O, 2, A, 2), np.int8), cur, chirp, N, O)
+    ei, ef = _window(ei, ef, -lat, T)
+    _check(out.download(), outf.download(), ei, ef, f"physical chirp DM {dm} N {N}")
+
+
+def test_codedisp_physical_chirp_recompresses_a_dispersed_pulse(sw, dev):
+    """A pulse dispersed in the time domain by the cold-plasma law over ~580
+    samples of one channel (DM 1000 at 856 MHz) comes out of the GPU
+    dedisperser with the physical chirp gathered into a few samples at its
+    centre-frequency arrival time."""
+    from skyweaver_amd import DeviceBuffer as DB
+    from test_oracle import _dispersed_pulse
+    dm, low, bwc, N, O, A = 1000.0, 856e6, 856e6 / 4096, 4096, 580, 4
+    T, n_c = 8192, 3000
+    x, width = _dispersed_pulse(n_c, T, dm, low + 0.5 * bwc, bwc, 40.0)
+    cur = np.zeros((1, T, 2, A, 2), np.int8)
+    cur[0, :, 1, 2] = x[:, 0, 0]                      # pol 1, antenna 2; the other columns stay zero
+    plan = sw.CoherentDedisperser(1, A, T, low, bwc, 1 / bwc, [dm], fft_len=[N], overlap=[O], chirp="physical")
+    lat = plan.latency(0)
+    outf = DB.empty(cur.shape, np.float32)
+    plan.run(DB.from_host(cur), [DB.empty(cur.shape, np.int8)], [outf])
+    y = outf.download()
+    p = y[0, :, 1, 2, 0].astype(np.float64) ** 2 + y[0, :, 1, 2, 1].astype(np.float64) ** 2
+    j = n_c - O // 2 + lat                            # output i holds stream sample i - lat = input i - lat + O/2
+    assert j + 5 <= T
+    print(f"pulse: width {width} latency {lat} peak {int(np.argmax(p))} expected {j} "
+          f"fraction {p[j - 4:j + 5].sum() / p.sum():.4f}", flush=True)
+    assert abs(int(np.argmax(p)) - j) <= 1
+    assert p[j - 4:j + 5].sum() / p.sum() > 0.9
+    assert np.abs(y[0, :, 0]).max() == 0 and np.abs(y[0, :, 1, :2]).max() == 0
+
+
 def test_codedisp_dm0_is_identity(sw, dev):
     from skyweaver_amd import DeviceBuffer as DB
     F, A, T = 3, 64, 9000

@@ -318,6 +318,82 @@ def test_chirp_closed_forms(orc):
     np.testing.assert_allclose(np.angle(H[2] * np.exp(-1j * ph)), 0, atol=1e-9)
 
 
+def test_physical_chirp_follows_the_delay_law(orc):
+    """The physical chirp's group delay -(1/2 pi) dphi/dnu is the cold-plasma
+    law K DM (1/nu0^2 - 1/(nu0 + nu)^2) at every baseband frequency nu of the
+    channel (negative in the upper half of the bins), from adjacent-bin phase
+    differences (midpoint rule); the reference mapping's upper half follows
+    it at the wrong frequencies."""
+    K, dm, N = 4.148806423e15, 1000.0, 8192
+    low, bwc = 856e6, 856e6 / 4096
+    Hp = orc.dm_chirp(dm, low, bwc, 3, N, physical=True)
+    Hr = orc.dm_chirp(dm, low, bwc, 3, N)
+    dnu = bwc / N
+    k = np.arange(N)
+    for c in range(3):
+        nu0 = low + (c + 0.5) * bwc
+        nu = np.where(2 * k >= N, k - N, k) * dnu
+        order = np.argsort(nu)                        # natural frequency order
+        h, f = Hp[c][order], nu[order]
+        dphi = np.angle(h[1:] * np.conj(h[:-1]))      # small steps: no wrap
+        assert np.abs(dphi).max() < 1.0
+        tau = -dphi / (2 * np.pi * dnu)
+        mid = f[:-1] + dnu / 2
+        law = K * dm * (1 / nu0 ** 2 - 1 / (nu0 + mid) ** 2)
+        np.testing.assert_allclose(tau, law, rtol=2e-6, atol=1e-9)
+        assert abs(Hp[c][0] - 1) < 1e-15               # zero delay and phase at the centre
+    # the reference chirp's bins N/2..N-1 carry the delays of frequencies
+    # above the channel: its slope there disagrees with the law at nu < 0
+    dr = -np.angle(Hr[0][N // 2 + 1:] * np.conj(Hr[0][N // 2:-1])) / (2 * np.pi * dnu)
+    nu0 = low + 0.5 * bwc
+    law_neg = K * dm * (1 / nu0 ** 2 - 1 / (nu0 + (np.arange(N // 2, N - 1) - N + 0.5) * dnu) ** 2)
+    assert np.max(np.abs(dr - law_neg)) > 1e-4
+
+
+def _dispersed_pulse(n_c, T, dm, f_c, bwc, amp):
+    """A pulse arriving at sample n_c at the channel centre f_c, dispersed by
+    the cold-plasma law: its instantaneous baseband frequency at time t is
+    the nu whose arrival time t_c + K DM (1/(f_c + nu)^2 - 1/f_c^2) is t
+    (nu within the channel), phase integrated sample by sample; complex int8."""
+    K = 4.148806423e15
+    tsamp = 1 / bwc
+    t = (np.arange(T) - n_c) * tsamp
+    inv2 = 1 / f_c ** 2 + t / (K * dm)
+    nu = np.full(T, np.nan)
+    ok = inv2 > 0
+    nu[ok] = 1 / np.sqrt(inv2[ok]) - f_c
+    on = ok & (np.abs(nu) < bwc / 2)
+    phase = 2 * np.pi * np.cumsum(np.where(on, nu, 0.0)) * tsamp
+    x = np.where(on, amp * np.exp(1j * phase), 0)
+    out = np.zeros((T, 1, 1, 2), np.int8)
+    out[:, 0, 0, 0] = np.clip(np.rint(x.real), -127, 127)
+    out[:, 0, 0, 1] = np.clip(np.rint(x.imag), -127, 127)
+    return out, int(on.sum())
+
+
+def test_physical_chirp_recompresses_a_dispersed_pulse(orc):
+    """Overlap-save dedispersion with the physical chirp gathers a pulse
+    dispersed over ~580 samples of one channel (DM 1000 at 856 MHz) back
+    into a few samples at its centre-frequency arrival time; with the
+    reference chirp half the band stays spread."""
+    dm, low, bwc, N, O = 1000.0, 856e6, 856e6 / 4096, 4096, 580
+    f_c = low + 0.5 * bwc
+    T, n_c = 6000, 3000
+    x, width = _dispersed_pulse(n_c, T, dm, f_c, bwc, 40.0)
+    assert 500 < width < 700
+    fracs = {}
+    for physical in (True, False):
+        chirp = orc.dm_chirp(dm, low, bwc, 1, N, physical=physical)[0]
+        yf, _ = orc.codedisp_channel(x, chirp, N, O, T - O)
+        p = np.abs(yf[:, 0, 0]) ** 2                  # output j is input sample j + O/2
+        j = n_c - O // 2
+        fracs[physical] = p[j - 4:j + 5].sum() / p.sum()
+        if physical:
+            assert abs(int(np.argmax(p)) - j) <= 1
+    assert fracs[True] > 0.9, fracs
+    assert fracs[False] < 0.6, fracs
+
+
 @pytest.mark.parametrize("N,O", [(256, 0), (256, 20), (128, 2)])
 def test_codedisp_dm0_identity(orc, N, O):
     T = 1000

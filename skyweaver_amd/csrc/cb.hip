@@ -99,6 +99,15 @@ struct Epi {
             reinterpret_cast<char4*>(out)[oi] = q;
         }
     }
+    // store_pre of a single-Stokes output through a buffer resource, at
+    // byte offset vo + so
+    __device__ __forceinline__ static void store_pre_buf(const float* power, __amdgpu_buffer_rsrc_t rs, int vo,
+                                                         int so, const float* ib, float sc, float off, int ib_sub)
+    {
+        static_assert(NV == 1, "single Stokes outputs");
+        const float r = ib_sub ? ib_subtract(power[0], ib[0], sc) : rescale<S>(power[0], off, sc);
+        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)clamp_i8(r), rs, vo, so, 0);
+    }
     // As store(), with the beamset's scale / offset and the incoherent-beam
     // powers already loaded.
     __device__ __forceinline__ static void store_pre(const float* power, int8_t* out, const float* ib,
@@ -511,14 +520,23 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     const int nbt = (nbeams + 31) / 32;
     const uint4* wsrc = reinterpret_cast<const uint4*>(packed + (size_t)f * nbeams * 2 * KP);
     const size_t wtotal = (size_t)nbeams * 2 * KP / 16;  // uint4 of this channel
+    const int nfull = (int)(wtotal / WTILE);             // whole tiles
+    // the channel's weights as a buffer: a whole tile's loads take the
+    // lane's fixed offset and the tile's scalar offset bt WTILE 16, which is
+    // at most (nfull - 1) WTILE 16, so every load lies in [0, 16 wtotal)
+    const auto wrs = buffer_rsrc(wsrc, (uint32_t)(wtotal * 16));
     // global -> registers for tile bt (beams past the end read as zero)
     auto fetch = [&](int bt, uint4* r) {
-        const uint4* src = wsrc + (size_t)bt * WTILE;
-        if((size_t)(bt + 1) * WTILE <= wtotal) {  // a whole tile (uniform)
+        if(bt < nfull) {  // a whole tile (uniform)
 #pragma unroll
             for(int i = 0; i < WPER; ++i) {
                 const int idx = tid + i * WV * 64;
-                r[i] = (WTILE % (WV * 64) == 0 || idx < WTILE) ? src[idx] : make_uint4(0, 0, 0, 0);
+                if(WTILE % (WV * 64) == 0 || idx < WTILE) {
+                    const auto v = __builtin_amdgcn_raw_buffer_load_b128(wrs, idx * 16, bt * WTILE * 16, 0);
+                    r[i] = make_uint4(v[0], v[1], v[2], v[3]);
+                } else {
+                    r[i] = make_uint4(0, 0, 0, 0);
+                }
             }
         } else {
 #pragma unroll
@@ -539,6 +557,19 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
             }
         }
     };
+    // single-Stokes outputs (TS <= 16, parameters preloaded) as a buffer
+    // over the workgroup's output rows [tw0, tw0 + NO_WG) of channel f:
+    // output (tout, f, b) at byte (tout - tw0) F B + row (the lane's fixed
+    // offset) + 32 bt (the tile's scalar offset); b < nbeams keeps every
+    // store below (NO_WG - 1) F B + nbeams, the range given
+    constexpr bool kBufOut = NV == 1 && TS <= 16 && EpiParams<S, NOUT>::kPreIb;
+    const int tw0 = tc * NO_WG;
+    const auto ors = buffer_rsrc(out + ((size_t)tw0 * nchans + f) * nbeams,
+                                 (uint32_t)((size_t)(NO_WG - 1) * nchans * nbeams + nbeams));
+    int ovo[kBufOut ? NOUT : 1];
+#pragma unroll
+    for(int o = 0; o < (kBufOut ? NOUT : 1); ++o)
+        ovo[o] = ((t0 + 16 * grp + o * TS) / TS - tw0) * nchans * nbeams + row;
     uint4 wreg[WPER];
     fetch(0, wreg);
     if constexpr(TMI) __syncthreads();  // every wave has its operands: the rows' LDS becomes the weight tiles
@@ -622,7 +653,9 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
                     for(int k = 0; k < NV; ++k) power[k] += st[o * TS + q][k];
                 if(active && bvalid) {
                     const int tout = (t0 + 16 * grp + o * TS) / TS;
-                    if constexpr(EpiParams<S, NOUT>::kPreIb)
+                    if constexpr(kBufOut)
+                        Epi<S>::store_pre_buf(power, ors, ovo[o], 32 * bt, ep.ib[o], ep.sc, ep.off, ib_sub);
+                    else if constexpr(EpiParams<S, NOUT>::kPreIb)
                         Epi<S>::store_pre(power, out, ep.ib[o], ep.sc, ep.off, tout, f, b, nchans, nbeams,
                                           ib_sub);
                     else

@@ -53,6 +53,15 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 void* scratch(size_t bytes, int slot);
 
 // ---------------------------------------------------------- device helpers
+// A raw buffer resource (gfx9 format) over [base, base + bytes): a buffer
+// access takes one 32-bit VGPR offset plus a scalar offset, so a thread's
+// run of accesses at uniform strides costs no per-access address
+// arithmetic.  Callers keep every access inside the range they give.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
 struct cf32 {
     float x, y;
 };

@@ -145,7 +145,7 @@ def test_codedisp_physical_chirp_recompresses_a_dispersed_pulse(sw, dev):
     plan.run(DB.from_host(cur), [DB.empty(cur.shape, np.int8)], [outf])
     y = outf.download()
     p = y[0, :, 1, 2, 0].astype(np.float64) ** 2 + y[0, :, 1, 2, 1].astype(np.float64) ** 2
-    j = n_c - O // 2 + lat                            # output i holds stream sample i - lat = input i - lat + O/2
+    j = n_c + O // 2 + lat                            # output i is input i - lat - O/2 (the DM 0 identity's shift)
     assert j + 5 <= T
     print(f"pulse: width {width} latency {lat} peak {int(np.argmax(p))} expected {j} "
           f"fraction {p[j - 4:j + 5].sum() / p.sum():.4f}", flush=True)

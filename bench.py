@@ -64,13 +64,15 @@ def log(msg):
 
 
 def run_form(sw, d_taftp, weights, form, batched, steps, warmup, capture=False, dms=None, fft_len=None,
-             share_forward=False):
+             share_forward=False, graph=False):
     """Times `steps` gulps after `warmup`; with capture, returns the TFB
-    output of every coherent DM for one more gulp."""
+    output of every coherent DM for one more gulp.  graph: each gulp's
+    device work captured into a hip graph and launched as one (asynchronous
+    sinks)."""
     dms = DMS if dms is None else dms
     fft_len = FFT_LEN if fft_len is None else fft_len
     p = sw.BeamformerPipeline(A, F, B, T, TS, dms, CFREQ, BW, stokes="I", fft_len=fft_len, form=form,
-                              batched=batched, share_forward=share_forward)
+                              batched=batched, share_forward=share_forward, async_sinks=graph, graph=graph)
     p.set_weights(weights)
     p.init(A)
     captured = {}
@@ -459,6 +461,12 @@ def main():
             "per_dm": {str(dm): [round(float(x), 3) for x in tms[d]] for d, dm in enumerate(DMS8)},
             "batched_all_that_fit": [round(float(x), 3) for x in tms[len(DMS8)]]}
 
+    log("graph mode: each gulp's device work as one hip graph")
+    gms, _, _, _ = run_form(sw, d_taftp, weights, "mfma", True, args.steps, args.warmup, graph=True)
+    graph_mode = {"ms_per_step": round(gms, 4), "ms_per_step_stream_launches": round(ms, 4),
+                  "note": "the headline gulp captured into a hip graph (updated in place each gulp) and launched as "
+                          "one, against the same work launched kernel by kernel"}
+
     host_input = None
     if not args.no_plain:
         log("PCIe-inclusive: gulps from pinned host memory")
@@ -532,6 +540,7 @@ def main():
         "incoherent_dedispersion": idisp,
         "config5_full_pipeline": config5,
         "writer_sinks": writers,
+        "graph_mode": graph_mode,
         "host_input": host_input,
         "cpu_baseline": cpu,
     }

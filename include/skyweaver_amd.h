@@ -316,14 +316,17 @@ int sw_codedisp_destroy(sw_codedisp* plan);
  * length for the whole gulp).  For every DM of cfg, with the overlap the
  * plan would choose, times one gulp of synthetic input through the
  * dedisperser on min(tune_chans, nchans) channels for each length in
- * {1024, ..., 32768} of at least 4 O, one DM per launch; then, for each
- * length, all DMs it fits (4 O <= N) as one batched launch sharing the
+ * {1024, ..., 32768} that fits it, one DM per launch; then, for each
+ * length, all DMs it fits as one batched launch sharing the
  * forward transforms (share_forward), with the other DMs at their own best
  * length.  The faster plan is written out: fft_len_out[ndms] and
  * *share_forward_out (NULL allowed; 1 when the batched plan won).  ms_out
  * (NULL allowed) receives [ndms + 1][6] times per gulp scaled to nchans:
  * rows 0 .. ndms-1 the single-DM times, row ndms the batched time of each
- * length (-1: not tried). */
+ * length (-1: not tried).  A length fits a DM when N >= 4 O (at most a
+ * quarter of each segment is overlap) and N <= nsamples (a longer transform
+ * delays the output by more than a gulp); when every length with
+ * N >= 4 O is longer than the gulp, the shortest of them fits. */
 int sw_codedisp_autotune(const sw_codedisp_config* cfg, int tune_chans, int* fft_len_out,
                          int* share_forward_out, float* ms_out, void* stream);
 int sw_codedisp_info(const sw_codedisp* plan, int dm_idx, int* fft_len, int* overlap,

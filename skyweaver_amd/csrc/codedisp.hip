@@ -1120,13 +1120,21 @@ extern "C" int sw_codedisp_autotune(const sw_codedisp_config* cfg, int tune_chan
     std::vector<float> best(nd, 1e30f), times((size_t)(nd + 1) * kNc, -1.0f);
     std::vector<int> N(nd, 0);
     std::vector<std::vector<int>> fits(kNc);
+    // a length fits DM d if it keeps the overlap waste below a quarter
+    // (N >= 4 O) and is no longer than the gulp (a longer transform delays
+    // the output by more than a gulp), or is the shortest length with
+    // N >= 4 O when every such length is longer than the gulp
+    auto fits_dm = [&](int c, int d) {
+        if(kCand[c] < 4 * O[d]) return false;
+        if(kCand[c] <= cfg->nsamples) return true;
+        return c == 0 || kCand[c - 1] < 4 * O[d];
+    };
     int rc = SW_OK;
     // one DM per launch, every length
     for(int c = 0; c < kNc && rc == SW_OK; ++c) {
-        // DMs for which this length keeps the overlap waste below a quarter
         std::vector<int> fl(nd);
         for(int d = 0; d < nd; ++d) {
-            const bool ok = kCand[c] >= 4 * O[d];
+            const bool ok = fits_dm(c, d);
             fl[d] = ok ? kCand[c] : 0;
             if(ok) fits[c].push_back(d);
         }
@@ -1177,7 +1185,7 @@ extern "C" int sw_codedisp_autotune(const sw_codedisp_config* cfg, int tune_chan
         times[(size_t)nd * kNc + c] = ms;
         float total = ms;
         for(int d = 0; d < nd; ++d)
-            if(kCand[c] < 4 * O[d]) total += best[d];
+            if(!fits_dm(c, d)) total += best[d];
         if(total < best_total) {
             best_total = total;
             best_c = c;
@@ -1189,7 +1197,7 @@ extern "C" int sw_codedisp_autotune(const sw_codedisp_config* cfg, int tune_chan
     if(rc) return rc;
     for(int d = 0; d < nd; ++d) {
         SW_CHECK_ARG(N[d] > 0, "autotune: no FFT length fits the overlap");
-        fft_len_out[d] = (best_c >= 0 && kCand[best_c] >= 4 * O[d]) ? kCand[best_c] : N[d];
+        fft_len_out[d] = (best_c >= 0 && fits_dm(best_c, d)) ? kCand[best_c] : N[d];
     }
     if(share_forward_out) *share_forward_out = best_c >= 0 ? 1 : 0;
     if(ms_out) std::memcpy(ms_out, times.data(), sizeof(float) * times.size());

@@ -288,6 +288,11 @@ def test_codedisp_autotune(sw, dev):
         fit = plan_of(won[0])[1]
         total = ms[nd][won[0]] + sum(ms[d][single[d]] for d in range(nd) if d not in fit)
         assert total < best_single
+    # lengths longer than the gulp (which would delay the output by more
+    # than a gulp) are not candidates while a shorter one fits the overlap
+    short, _, ms_short = sw.CoherentDedisperser.autotune(F, A, 4096, F_LOW, CHAN_BW, TSAMP, dms, tune_chans=2)
+    assert all(n <= 4096 for n in short), short
+    assert all(x < 0 for x in ms_short[:, 3:].ravel())
     from skyweaver_amd import DeviceBuffer as DB
     p = sw.BeamformerPipeline(A, F, 32, 4096, 16, dms, 1284e6, CHAN_BW * F, autotune_fft=True)
     p.set_weights(sw.synth_weights((F, 32, A, 2), seed=1))

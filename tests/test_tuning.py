@@ -65,3 +65,5 @@ def test_committed_table_entries(sw):
         got = sw.tuning_lookup(e["antennas"], e["beams"], e["samples"], e["tscrunch"], e["stokes"], e["dms"])
         assert got == {"cb_waves": e["cb_waves"], "fft_len": e["fft_len"], "share_forward": bool(e["share_forward"])}
         assert all(n in (1024, 2048, 4096, 8192, 16384, 32768) for n in e["fft_len"])
+        # no transform longer than the gulp: it would delay the output by more than a gulp
+        assert max(e["fft_len"]) <= e["samples"], e["workload"]

@@ -890,6 +890,17 @@ double smear_samples(double f_low, double chan_bw, double tsamp, double dm)
     return kDmConst * (1.0 / (f1 * f1) - 1.0 / (f2 * f2)) * dm / tsamp;
 }
 
+} // namespace
+
+// The overlap a plan gives a DM when none is configured: the even ceiling
+// of the intra-channel smear at the bottom of the band.
+int codedisp_overlap(double low_freq, double chan_bw, double tsamp, float dm)
+{
+    return even_ceil(smear_samples(low_freq, chan_bw, tsamp, dm));
+}
+
+namespace {
+
 float2* twiddle_table(sw_codedisp* p, int n, hipStream_t st)
 {
     auto it = p->twiddles.find(n);

@@ -380,7 +380,18 @@ extern "C" int sw_pipeline_create(const sw_pipeline_config* cfg, sw_pipeline** o
         p->cb_waves = c.cb_waves > 0 ? c.cb_waves : (found && waves > 0 ? waves : 4);
         if(found && c.cb_waves <= 0 && waves > 0) p->tile_source = 1;
         SW_CHECK_ARG(p->cb_waves == 2 || p->cb_waves == 4 || p->cb_waves == 8, "cb_waves must be 0, 2, 4 or 8");
-        if(found == 2 && want_fft && std::all_of(tl.begin(), tl.end(), [](int n) { return n > 0; })) {
+        // the table's FFT lengths were timed for its own band: they are taken
+        // only if each still leaves at most a quarter of a segment to the
+        // overlap here and is no longer than the gulp (the autotuner's rule)
+        bool fit = true;
+        for(int d = 0; d < c.ndms && found == 2; ++d) {
+            const int O = (c.overlap && c.overlap[d] >= 0)
+                              ? c.overlap[d]
+                              : sw::codedisp_overlap(c.centre_frequency - c.bandwidth / 2.0, c.bandwidth / c.nchans,
+                                                     c.nchans / c.bandwidth, p->dms[d]);
+            fit = fit && tl[d] >= 4 * O && (tl[d] <= c.nsamples || tl[d] < 8 * O || tl[d] == 1024);
+        }
+        if(found == 2 && want_fft && fit && std::all_of(tl.begin(), tl.end(), [](int n) { return n > 0; })) {
             p->fft_len = tl;
             p->cfg.fft_len = p->fft_len.data();
             p->cfg.share_forward = share;

@@ -52,6 +52,10 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // caller passes a NULL workspace.  Not thread-safe across concurrent calls.
 void* scratch(size_t bytes, int slot);
 
+// The overlap a coherent-dedispersion plan gives a DM when none is
+// configured (csrc/codedisp.hip).
+int codedisp_overlap(double low_freq, double chan_bw, double tsamp, float dm);
+
 // ---------------------------------------------------------- device helpers
 // A raw buffer resource (gfx9 format) over [base, base + bytes): a buffer
 // access takes one 32-bit VGPR offset plus a scalar offset, so a thread's

@@ -623,3 +623,24 @@ def test_pipeline_tiles_from_table_identical(sw, dev, tmp_path):
         assert len(got) == len(ref)
         for x, y in zip(ref, got):
             np.testing.assert_array_equal(x, y)
+
+
+def test_pipeline_table_fft_lengths_must_fit_the_band(sw, dev, tmp_path):
+    """A table's FFT lengths were timed for its own band: a pipeline takes
+    them only while each leaves at most a quarter of a segment to its own
+    overlap.  DM 350 with a 1024-point transform fits 1284 MHz (overlap 62);
+    at 580 MHz the smear is about 650 samples, so the pipeline keeps the
+    table's workgroup shape and falls back to its default length."""
+    import json
+    table = tmp_path / "t.json"
+    table.write_text(json.dumps({"entries": [
+        {"antennas": A, "beams": B, "samples": T, "tscrunch": TS, "stokes": "I", "cb_waves": 2,
+         "dms": [350.0], "fft_len": [1024], "share_forward": 0}]}))
+    p = sw.BeamformerPipeline(A, F, B, T, TS, [350.0], CFREQ, BW, tuning_file=str(table))
+    assert p.tile() == {"cb_waves": 2, "fft_len": [1024], "share_forward": False, "source": "table+fft"}
+    p.close()
+    p = sw.BeamformerPipeline(A, F, B, T, TS, [350.0], 580e6, BW, tuning_file=str(table))
+    t = p.tile()
+    p.close()
+    assert t["cb_waves"] == 2 and t["source"] == "table"
+    assert t["fft_len"][0] > 1024

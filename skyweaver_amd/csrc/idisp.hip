@@ -361,10 +361,23 @@ idisp_mfma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const
             for(int tt = 0; tt < NTT; ++tt)
                 for(int s = 0; s < ts; ++s) {
                     const int8_t* rp = base + (tt * 16 * ts + s) * kMfPitch;
+                    if constexpr(!PF) {
+                        // the row's eight reads in flight together (the
+                        // scheduler would otherwise issue them one by one)
+                        v4i bv[8];
 #pragma unroll
-                    for(int bg = 0; bg < 8; ++bg) {
-                        const v4i b = *reinterpret_cast<const v4i*>(rp + bg * 16);
-                        acc[jd][tt][bg] = __builtin_amdgcn_mfma_i32_16x16x64_i8(sel, b, acc[jd][tt][bg], 0, 0, 0);
+                        for(int bg = 0; bg < 8; ++bg) bv[bg] = *reinterpret_cast<const v4i*>(rp + bg * 16);
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for(int bg = 0; bg < 8; ++bg)
+                            acc[jd][tt][bg] =
+                                __builtin_amdgcn_mfma_i32_16x16x64_i8(sel, bv[bg], acc[jd][tt][bg], 0, 0, 0);
+                    } else {
+#pragma unroll
+                        for(int bg = 0; bg < 8; ++bg) {
+                            const v4i b = *reinterpret_cast<const v4i*>(rp + bg * 16);
+                            acc[jd][tt][bg] = __builtin_amdgcn_mfma_i32_16x16x64_i8(sel, b, acc[jd][tt][bg], 0, 0, 0);
+                        }
                     }
                 }
         }

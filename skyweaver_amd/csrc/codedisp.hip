@@ -241,6 +241,18 @@ __device__ __forceinline__ void apply_powers(float2* v, float2 w1)
     }
 }
 
+// The step 2 / 2' twiddles of an N = 512 R1 transform as two tables in
+// LDS (tw[m] = W_N^m): t1k[16 k2 + n3] = W_N^(R1 n3 k2) (k2 < 32, n3 < 16)
+// and t1k[512 + R1 n2 + k1] = W_N^(16 k1 n2) (n2 < 32, k1 < R1).  A thread
+// reads one row of each at a fixed column, so every read is its base
+// address plus an immediate offset.  1536 entries at most.
+template <int R1>
+__device__ __forceinline__ void load_step2_tables(float2* t1k, const float2* __restrict__ tw, int t)
+{
+    t1k[t] = tw[((t & 15) * (t >> 4)) * R1];  // R1 n3 k2 <= 465 R1 < N
+    for(int j = t; j < 32 * R1; j += 512) t1k[512 + j] = tw[16 * (j % R1) * (j / R1)];  // 16 k1 n2 < N
+}
+
 // Forward steps 1-3 of an N = 512 R1 transform of C = 32 / R1 columns: v
 // holds the thread's inputs (value c R1 + n1 = sample 512 n1 + t of column
 // c) and ends holding X[k] of the pairs q = t, t + 512 (16 values each,
@@ -267,21 +279,13 @@ __device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddl
     for(int n2 = 0; n2 < 32; ++n2) v[n2] = lds[j2 * kEx1Row + n2 * 16 + n3];
     __syncthreads();
     dft32<false>(v);
-    // x W_N^(R1 n3 k2) = W_1024^(2 n3 k2), straight from the table (the
-    // opaque step keeps the compiler from holding 31 table addresses in
-    // registers across units)
+    // x W_N^(R1 n3 k2) = W_512^(n3 k2) from the table of step 2 (row k2,
+    // column n3: one address per thread, the rows at immediate offsets; the
+    // 16 addresses of a wave are 128 consecutive bytes)
     {
-        // W_512^j sits at t1k[1024 + j]: with the 8-byte stride n3 the 16
-        // distinct addresses of a wave fall on fewer common banks than with
-        // W_1024's 16-byte stride
-        int step = n3;
-        asm volatile("" : "+v"(step));
-        const float2* tk = t1k + 1024;
+        const float2* tk = t1k + n3;
 #pragma unroll
-        for(int k2 = 1; k2 < 32; ++k2) {
-            tk += step;
-            v[k2] = cmul(v[k2], *tk);
-        }
+        for(int k2 = 1; k2 < 32; ++k2) v[k2] = cmul(v[k2], tk[16 * k2]);
     }
 #pragma unroll
     for(int k2 = 0; k2 < 32; ++k2) lds[j2 * kEx2Row + k2 * kEx2Stride + n3] = v[k2];
@@ -314,12 +318,17 @@ __device__ __forceinline__ void reg_chirp_inverse(const float2* X, float2* v, fl
     for(int h = 0; h < 2; ++h) {
         const int k2 = n3 + 16 * h, k1 = k1_2;
         float2* x = v + 16 * h;
+        float2 hk[16];
 #pragma unroll
-        for(int k3 = 0; k3 < 16; ++k3) x[k3] = cmul(X[16 * h + k3], H[32 * R1 * k3 + 32 * k1 + k2]);
-        // step 3': IDFT16 -> n3, x W_N^(-R1 n3 k2) and W_N^(-k1 n3): the
+        for(int k3 = 0; k3 < 16; ++k3) {
+            hk[k3] = H[32 * R1 * k3 + 32 * k1 + k2];
+            x[k3]  = X[16 * h + k3];
+        }
+        // step 3': IDFT16 of X H -> n3 (the chirp products folded into the
+        // first butterflies), x W_N^(-R1 n3 k2) and W_N^(-k1 n3): the
         // second factor of step 2's twiddle W_N^(-k1 (16 n2 + n3)) does not
         // depend on k2, so by linearity it is applied before the IDFT32
-        dft16<true>(x);
+        dft16_mul<true>(x, hk);
         apply_powers<16>(x, cconj(tw(R1 * k2 + k1)));
     }
     // row j2 of the exchange buffer is this wave's alone between the two
@@ -337,16 +346,11 @@ __device__ __forceinline__ void reg_chirp_inverse(const float2* X, float2* v, fl
     for(int k2 = 0; k2 < 32; ++k2) v[k2] = lds[j2 * kEx2Row + k2 * kEx2Stride + n3];
     __syncthreads();  // every wave is done with its rows before the buffer is reused
     dft32<true>(v);
-    // x W_N^(-16 k1 n2) = conj(W_1024^((32 / R1) k1 n2)), from the table
+    // x W_N^(-16 k1 n2), from the table of step 2' (row n2, column k1)
     {
-        int step = (32 / R1) * k1_2;
-        asm volatile("" : "+v"(step));
-        const float2* tk = t1k;
+        const float2* tk = t1k + 512 + k1_2;
 #pragma unroll
-        for(int n2 = 1; n2 < 32; ++n2) {
-            tk += step;
-            v[n2] = cmul(v[n2], cconj(*tk));
-        }
+        for(int n2 = 1; n2 < 32; ++n2) v[n2] = cmul(v[n2], cconj(tk[R1 * n2]));
     }
 #pragma unroll
     for(int n2 = 0; n2 < 32; ++n2) lds[j2 * kEx1Row + n2 * 16 + n3] = v[n2];
@@ -406,6 +410,67 @@ __device__ __forceinline__ void store_output(const CdParams& p, int d, size_t co
     if(pos >= half && pos < N - half) store_kept(p, d, colbase, j0, pos, y);
 }
 
+// The stores of one unit of the register kernels: value c R1 + n1 of v is
+// position 512 n1 + t of column col0 + c, and the kept positions
+// [O/2, N - O/2) go to DM d's ring at index j0 + position, wrapped once.
+// Ring offsets are carried in bytes and wrapped with one unsigned minimum
+// (o - 2 ylen wraps around to a large value while o < 2 ylen); the column
+// base is uniform, so a store needs no 64-bit address arithmetic.  When the
+// overlap is at most 1024 only the first and last position rows (n1 = 0 and
+// R1 - 1) hold dropped positions; the float ring, when asked for, is written
+// in a separate pass so the int8 stores carry no test of it.
+template <int R1>
+__device__ __forceinline__ void store_unit(const CdParams& p, int d, size_t col0, int j0, int t, const float2* v)
+{
+    constexpr int N = 512 * R1, C = 32 / R1;
+    int8_t* const o8 = p.out[d];
+    float2* const of = p.outf[d];
+    const int half = p.O / 2;
+    const uint32_t ylen2 = 2u * (uint32_t)p.ylen;
+    int tt = t;
+    asm volatile("" : "+v"(tt));  // keeps the per-row offsets from being precomputed and held
+    const uint32_t base = 2u * (uint32_t)(j0 + tt);  // byte offset of position tt (kept positions: >= 0)
+    auto off = [&](int n1) {
+        const uint32_t o = base + 1024u * n1;
+        return min(o, o - ylen2);
+    };
+    auto put = [&](int8_t* ob, int n1, float2 y) {
+        const uint32_t re = __float_as_uint(y.x + 12582912.0f), im = __float_as_uint(y.y + 12582912.0f);
+        *reinterpret_cast<uint16_t*>(ob + off(n1)) = (uint16_t)((re & 0xffu) | ((im & 0xffu) << 8));
+    };
+    const bool lo_kept = tt >= half, hi_kept = tt < 512 - half;  // rows 0 and R1 - 1
+    if(half <= 512) {
+#pragma unroll
+        for(int c = 0; c < C; ++c) {
+            int8_t* const ob = o8 + (col0 + c) * (size_t)ylen2;
+#pragma unroll
+            for(int n1 = 1; n1 < R1 - 1; ++n1) put(ob, n1, v[c * R1 + n1]);
+            if(lo_kept) put(ob, 0, v[c * R1]);
+            if(hi_kept) put(ob, R1 - 1, v[c * R1 + R1 - 1]);
+        }
+    } else {
+#pragma unroll
+        for(int c = 0; c < C; ++c)
+#pragma unroll
+            for(int n1 = 0; n1 < R1; ++n1) {
+                const int pos = 512 * n1 + tt;
+                if(pos >= half && pos < N - half) store_kept(p, d, (col0 + c) * p.ylen, j0, pos, v[c * R1 + n1]);
+            }
+        return;  // store_kept wrote the float ring too
+    }
+    if(of) {
+#pragma unroll
+        for(int c = 0; c < C; ++c) {
+            float2* const fb = of + (col0 + c) * (size_t)p.ylen;
+#pragma unroll
+            for(int n1 = 0; n1 < R1; ++n1) {
+                const int pos = 512 * n1 + tt;
+                if(pos >= half && pos < N - half) fb[off(n1) / 2] = v[c * R1 + n1];
+            }
+        }
+    }
+}
+
 // Persistent: a workgroup (one per CU, the exchange buffer fills the LDS)
 // walks units u = blockIdx.x + k gridDim.x, u = (channel, segment, column
 // group), and loads the next unit's input while transforming the current
@@ -417,10 +482,9 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
     constexpr int C = 32 / R1;
     __shared__ float2 lds[kRegLds + TwiddleLds<N>::SIZE + 1536];
     const Twiddles tw = load_twiddles<N, 512>(lds + kRegLds, p.tw);
-    float2* t1k = lds + kRegLds + TwiddleLds<N>::SIZE;  // W_1024^j = W_N^(j N / 1024), then W_512^j
+    float2* t1k = lds + kRegLds + TwiddleLds<N>::SIZE;  // step 2 and 2' twiddle tables
     const int t = threadIdx.x;
-    for(int j = t; j < 1024; j += 512) t1k[j] = p.tw[j * (N / 1024)];
-    t1k[1024 + t] = p.tw[t * (N / 512)];
+    load_step2_tables<R1>(t1k, p.tw, t);
     const int ngrp = p.PA / C;
     uint32_t nxt[16];  // two raw samples per register: value i = c R1 + n1
     auto fetch = [&](int u) {
@@ -445,24 +509,7 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
         reg_forward<R1>(v, lds, tw, t1k, t);
         reg_chirp_inverse<R1>(v, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N);
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
-        // positions 512 .. N - 513 are all kept when O / 2 <= 512; the
-        // opaque copy of t keeps the position tests of the first and last
-        // rows from being precomputed and held across units
-        const bool interior = p.O / 2 <= 512;
-        int tt = t;
-        asm volatile("" : "+v"(tt));
-#pragma unroll
-        for(int c = 0; c < C; ++c) {
-            const size_t cb = ((size_t)f * p.PA + cg * C + c) * p.ylen;
-#pragma unroll
-            for(int n1 = 0; n1 < R1; ++n1) {
-                const float2 y = v[c * R1 + n1];  // already / N (chirp)
-                if(interior && n1 > 0 && n1 < R1 - 1)
-                    store_kept(p, d, cb, j0, 512 * n1 + tt, y);
-                else
-                    store_output(p, d, cb, j0, 512 * n1 + tt, N, y);
-            }
-        }
+        store_unit<R1>(p, d, (size_t)f * p.PA + cg * C, j0, t, v);  // v already / N (chirp)
     }
 }
 
@@ -478,10 +525,9 @@ __global__ void __launch_bounds__(512) codedisp_regm_kernel(CdParams p, int nuni
     constexpr int C = 32 / R1;
     __shared__ float2 lds[kRegLds + TwiddleLds<N>::SIZE + 1536];
     const Twiddles tw = load_twiddles<N, 512>(lds + kRegLds, p.tw);
-    float2* t1k = lds + kRegLds + TwiddleLds<N>::SIZE;  // W_1024^j = W_N^(j N / 1024), then W_512^j
+    float2* t1k = lds + kRegLds + TwiddleLds<N>::SIZE;  // step 2 and 2' twiddle tables
     const int t = threadIdx.x;
-    for(int j = t; j < 1024; j += 512) t1k[j] = p.tw[j * (N / 1024)];
-    t1k[1024 + t] = p.tw[t * (N / 512)];
+    load_step2_tables<R1>(t1k, p.tw, t);
     const int ngrp = p.PA / C;
     uint32_t nxt[16];  // two raw samples per register: value i = c R1 + n1
     auto fetch = [&](int u) {
@@ -504,26 +550,12 @@ __global__ void __launch_bounds__(512) codedisp_regm_kernel(CdParams p, int nuni
         __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
         reg_forward<R1>(X, lds, tw, t1k, t);
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
-        const bool interior = p.O / 2 <= 512;
         for(int d = 0; d < p.ndm; ++d) {
             if(d > 0) __syncthreads();  // the previous DM's readers of the exchange buffer
             if(d == p.ndm - 1 && u + (int)gridDim.x < nunits) fetch(u + gridDim.x);
             float2 v[32];
             reg_chirp_inverse<R1>(X, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N);
-            int tt = t;
-            asm volatile("" : "+v"(tt));
-#pragma unroll
-            for(int c = 0; c < C; ++c) {
-                const size_t cb = ((size_t)f * p.PA + cg * C + c) * p.ylen;
-#pragma unroll
-                for(int n1 = 0; n1 < R1; ++n1) {
-                    const float2 y = v[c * R1 + n1];  // already / N (chirp)
-                    if(interior && n1 > 0 && n1 < R1 - 1)
-                        store_kept(p, d, cb, j0, 512 * n1 + tt, y);
-                    else
-                        store_output(p, d, cb, j0, 512 * n1 + tt, N, y);
-                }
-            }
+            store_unit<R1>(p, d, (size_t)f * p.PA + cg * C, j0, t, v);  // v already / N (chirp)
         }
     }
 }
@@ -544,10 +576,9 @@ __global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, i
     constexpr int L = 16384, N = 2 * L;
     __shared__ float2 lds[kRegLds + TwiddleLds<L>::SIZE + 1536];
     const Twiddles tw = load_twiddles<L, 512>(lds + kRegLds, p.tw);
-    float2* t1k = lds + kRegLds + TwiddleLds<L>::SIZE;  // W_1024^j = W_L^(j L / 1024), then W_512^j
+    float2* t1k = lds + kRegLds + TwiddleLds<L>::SIZE;  // step 2 and 2' twiddle tables
     const int t = threadIdx.x;
-    for(int j = t; j < 1024; j += 512) t1k[j] = p.tw[j * (L / 1024)];
-    t1k[1024 + t] = p.tw[t * (L / 512)];
+    load_step2_tables<32>(t1k, p.tw, t);
     uint32_t raw[32];  // x[512 n1 + t] at 0..15, x[L + 512 n1 + t] at 16..31, two per register
     for(int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int col = u % p.PA, fs = u / p.PA;

@@ -80,31 +80,83 @@ __device__ __forceinline__ void dft8(float2* v)
     for(int k = 0; k < 8; ++k) v[k] = o[k];
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft16(float2* v)
+// Butterflies with a twiddled second input in FMA form: a + w b, then
+// a - w b formed as 2 a - (a + w b), 6 instructions instead of the 8 of a
+// complex product followed by a sum and a difference.  The difference
+// carries the rounding of the sum, an error of the same size as that of the
+// product it replaces.
+__device__ __forceinline__ void bfly_w(float2& a, float2& b, float2 w)
 {
-    // n = 4 n1 + n2: DFT4 over n1 for each n2, twiddle W16^(n2 k1), DFT4 over n2
-#pragma unroll
-    for(int n2 = 0; n2 < 4; ++n2) dft4<INV>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
-    // a[k1][n2] now sits in v[4 k1 + n2]
+    const float2 s = make_float2(fmaf(w.x, b.x, fmaf(-w.y, b.y, a.x)), fmaf(w.x, b.y, fmaf(w.y, b.x, a.y)));
+    b = make_float2(fmaf(2.0f, a.x, -s.x), fmaf(2.0f, a.y, -s.y));
+    a = s;
+}
+// rot45<INV, M>(a) / h: the two sums of the 45-degree rotation, unscaled
+template <bool INV, int M>
+__device__ __forceinline__ float2 rot45_sums(float2 a)
+{
+    if constexpr(M == 1)
+        return INV ? make_float2(a.x - a.y, a.x + a.y) : make_float2(a.x + a.y, a.y - a.x);
+    else
+        return INV ? make_float2(-(a.x + a.y), a.x - a.y) : make_float2(a.y - a.x, -(a.x + a.y));
+}
+// bfly_w with w = exp(-+ 2 pi i M / 8), M = 1, 3: the product is h times two sums
+template <bool INV, int M>
+__device__ __forceinline__ void bfly_w8(float2& a, float2& b)
+{
+    constexpr float h = 0.70710678118654752440f;
+    const float2 u = rot45_sums<INV, M>(b);
+    const float2 s = make_float2(fmaf(h, u.x, a.x), fmaf(h, u.y, a.y));
+    b = make_float2(fmaf(2.0f, a.x, -s.x), fmaf(2.0f, a.y, -s.y));
+    a = s;
+}
+
+// DFT4 of (a0, a1, a2, a3) whose first two butterflies' sums and
+// differences are already formed: t0, t1 = a0 +- a2 and t2, t3r = a1 +- a3
+template <bool INV>
+__device__ __forceinline__ void dft4_tail(float2& v0, float2& v1, float2& v2, float2& v3, float2 t0, float2 t1,
+                                          float2 t2, float2 t3r)
+{
+    const float2 t3 = rot90<INV>(t3r);
+    v0 = cadd(t0, t2);
+    v2 = csub(t0, t2);
+    v1 = cadd(t1, t3);
+    v3 = csub(t1, t3);
+}
+
+// DFT16 after its first layer of DFT4s: v[4 k1 + n2] holds a[k1][n2]; the
+// second layer takes DFT4 over n2 of a[k1][n2] W16^(n2 k1), with the
+// twiddles folded into the butterflies, and the 4x4 register block is
+// transposed to natural order
+template <bool INV>
+__device__ __forceinline__ void dft16_second(float2* v)
+{
     constexpr float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
     constexpr float h  = 0.70710678118654752440f;
     const float sg = INV ? 1.0f : -1.0f;
     // W16^m = cos(2 pi m/16) -+ i sin(2 pi m/16)
-    const float2 w1 = make_float2(c1, sg * s1), w2 = make_float2(h, sg * h),
-                 w3 = make_float2(s1, sg * c1), w6 = make_float2(-h, sg * h),
-                 w9 = make_float2(-c1, -sg * s1);
-    v[5]  = cmul(v[5], w1);   // k1=1 n2=1
-    v[6]  = cmul(v[6], w2);   // k1=1 n2=2
-    v[7]  = cmul(v[7], w3);   // k1=1 n2=3
-    v[9]  = cmul(v[9], w2);   // k1=2 n2=1
-    v[10] = rot90<INV>(v[10]); // k1=2 n2=2: W16^4
-    v[11] = cmul(v[11], w6);  // k1=2 n2=3
-    v[13] = cmul(v[13], w3);  // k1=3 n2=1
-    v[14] = cmul(v[14], w6);  // k1=3 n2=2
-    v[15] = cmul(v[15], w9);  // k1=3 n2=3
-#pragma unroll
-    for(int k1 = 0; k1 < 4; ++k1) dft4<INV>(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+    const float2 w1 = make_float2(c1, sg * s1), w3 = make_float2(s1, sg * c1), w9 = make_float2(-c1, -sg * s1);
+    dft4<INV>(v[0], v[1], v[2], v[3]);
+    {   // k1 = 1: twiddles W16^1, W16^2 = W8^1, W16^3
+        float2 a0 = v[4], a2 = v[6], p = cmul(v[5], w1), a3 = v[7];
+        bfly_w8<INV, 1>(a0, a2);
+        bfly_w(p, a3, w3);
+        dft4_tail<INV>(v[4], v[5], v[6], v[7], a0, a2, p, a3);
+    }
+    {   // k1 = 2: twiddles W16^2 = W8^1, W16^4 = -+i, W16^6 = W8^3
+        const float2 r2 = rot90<INV>(v[10]);
+        const float2 t0 = cadd(v[8], r2), t1 = csub(v[8], r2);
+        const float2 u = rot45_sums<INV, 1>(v[9]);
+        float2 p = make_float2(h * u.x, h * u.y), a3 = v[11];
+        bfly_w8<INV, 3>(p, a3);
+        dft4_tail<INV>(v[8], v[9], v[10], v[11], t0, t1, p, a3);
+    }
+    {   // k1 = 3: twiddles W16^3, W16^6 = W8^3, W16^9
+        float2 a0 = v[12], a2 = v[14], p = cmul(v[13], w3), a3 = v[15];
+        bfly_w8<INV, 3>(a0, a2);
+        bfly_w(p, a3, w9);
+        dft4_tail<INV>(v[12], v[13], v[14], v[15], a0, a2, p, a3);
+    }
     // X[k1 + 4 k2] = v[4 k1 + k2]: transpose the 4x4 register block
     float2 o[16];
 #pragma unroll
@@ -113,6 +165,32 @@ __device__ __forceinline__ void dft16(float2* v)
         for(int k2 = 0; k2 < 4; ++k2) o[k1 + 4 * k2] = v[4 * k1 + k2];
 #pragma unroll
     for(int k = 0; k < 16; ++k) v[k] = o[k];
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft16(float2* v)
+{
+    // n = 4 n1 + n2: DFT4 over n1 for each n2, twiddle W16^(n2 k1), DFT4 over n2
+#pragma unroll
+    for(int n2 = 0; n2 < 4; ++n2) dft4<INV>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+    // a[k1][n2] now sits in v[4 k1 + n2]
+    dft16_second<INV>(v);
+}
+
+// DFT16 of v[i] m[i]: the products are folded into the first layer's
+// butterflies (v and m are read, v is overwritten with the transform)
+template <bool INV>
+__device__ __forceinline__ void dft16_mul(float2* v, const float2* m)
+{
+#pragma unroll
+    for(int n2 = 0; n2 < 4; ++n2) {
+        float2 p = cmul(v[n2], m[n2]), b = v[8 + n2];
+        float2 q = cmul(v[4 + n2], m[4 + n2]), d = v[12 + n2];
+        bfly_w(p, b, m[8 + n2]);
+        bfly_w(q, d, m[12 + n2]);
+        dft4_tail<INV>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2], p, b, q, d);
+    }
+    dft16_second<INV>(v);
 }
 
 // cos, sin of 2 pi k / 32, k < 16
@@ -149,19 +227,24 @@ __device__ __forceinline__ void dft32(float2* v)
     dft16<INV>(o);
 #pragma unroll
     for(int k = 0; k < 16; ++k) {
-        float2 t;
-        if(k == 0)
-            t = o[0];
-        else if(k == 8)
-            t = rot90<INV>(o[8]);
-        else if(k == 4)
-            t = rot45<INV, 1>(o[4]);
-        else if(k == 12)
-            t = rot45<INV, 3>(o[12]);
-        else
-            t = cmul(o[k], make_float2(kW32[k][0], INV ? kW32[k][1] : -kW32[k][1]));
-        v[k]      = cadd(e[k], t);
-        v[k + 16] = csub(e[k], t);
+        if(k == 0) {
+            v[0]  = cadd(e[0], o[0]);
+            v[16] = csub(e[0], o[0]);
+            continue;
+        } else if(k == 8) {
+            const float2 t = rot90<INV>(o[8]);
+            v[8]  = cadd(e[8], t);
+            v[24] = csub(e[8], t);
+            continue;
+        } else if(k == 4) {
+            bfly_w8<INV, 1>(e[4], o[4]);
+        } else if(k == 12) {
+            bfly_w8<INV, 3>(e[12], o[12]);
+        } else {
+            bfly_w(e[k], o[k], make_float2(kW32[k][0], INV ? kW32[k][1] : -kW32[k][1]));
+        }
+        v[k]      = e[k];
+        v[k + 16] = o[k];
     }
 }
 

@@ -258,12 +258,22 @@ __device__ __forceinline__ void load_step2_tables(float2* t1k, const float2* __r
 // c) and ends holding X[k] of the pairs q = t, t + 512 (16 values each,
 // k3 = 0..15).  The caller has synchronised since the exchange buffer was
 // last read.
-template <int R1>
-__device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddles& tw, const float2* t1k, int t)
+// wk, when given, holds the step-1 twiddles W_N^(t k1), k1 < R1 (they
+// depend on the thread alone), which are otherwise built per call.
+template <int R1, bool HOLD = false>
+__device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddles& tw, const float2* t1k, int t,
+                                            const float2* wk = nullptr)
 {
     constexpr int C = 32 / R1;
     const int n3 = t & 15, j2 = t >> 4;
-    {
+    if constexpr(HOLD) {
+#pragma unroll
+        for(int c = 0; c < C; ++c) {
+            dft<R1, false>(v + c * R1);
+#pragma unroll
+            for(int k = 1; k < R1; ++k) v[c * R1 + k] = cmul(v[c * R1 + k], wk[k]);
+        }
+    } else {
         const float2 w = tw(t);
 #pragma unroll
         for(int c = 0; c < C; ++c) {
@@ -495,6 +505,9 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
         for(int c = 0; c < C; ++c) load_column<R1 / 2>(p, f, cg * C + c, b, t, nxt + c * (R1 / 2));
     };
     if(blockIdx.x < nunits) fetch(blockIdx.x);
+    __syncthreads();  // twiddle tables
+    float2 wk[R1];    // step-1 twiddles W_N^(t k1), the same for every unit
+    twiddle_powers<R1>(tw(t), wk);
     for(int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int cg = u % ngrp, fs = u / ngrp;
         const int f = fs / p.nseg, s = fs % p.nseg;
@@ -506,7 +519,7 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
         }
         if(u + (int)gridDim.x < nunits) fetch(u + gridDim.x);  // in flight during the transforms
         __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
-        reg_forward<R1>(v, lds, tw, t1k, t);
+        reg_forward<R1, true>(v, lds, tw, t1k, t, wk);
         reg_chirp_inverse<R1>(v, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N);
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
         store_unit<R1>(p, d, (size_t)f * p.PA + cg * C, j0, t, v);  // v already / N (chirp)
@@ -538,6 +551,9 @@ __global__ void __launch_bounds__(512) codedisp_regm_kernel(CdParams p, int nuni
         for(int c = 0; c < C; ++c) load_column<R1 / 2>(p, f, cg * C + c, b, t, nxt + c * (R1 / 2));
     };
     if(blockIdx.x < nunits) fetch(blockIdx.x);
+    __syncthreads();  // twiddle tables
+    float2 wk[R1];    // step-1 twiddles W_N^(t k1), the same for every unit
+    twiddle_powers<R1>(tw(t), wk);
     for(int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int cg = u % ngrp, fs = u / ngrp;
         const int f = fs / p.nseg, s = fs % p.nseg;
@@ -548,7 +564,7 @@ __global__ void __launch_bounds__(512) codedisp_regm_kernel(CdParams p, int nuni
             X[2 * i + 1] = unpack_c8(nxt[i] >> 16);
         }
         __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
-        reg_forward<R1>(X, lds, tw, t1k, t);
+        reg_forward<R1, (R1 < 32)>(X, lds, tw, t1k, t, wk);  // at R1 = 32 the held twiddles would spill beside X
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
         for(int d = 0; d < p.ndm; ++d) {
             if(d > 0) __syncthreads();  // the previous DM's readers of the exchange buffer

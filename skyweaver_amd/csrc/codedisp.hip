@@ -241,89 +241,104 @@ __device__ __forceinline__ void apply_powers(float2* v, float2 w1)
     }
 }
 
-// The step 2 / 2' twiddles of an N = 512 R1 transform as two tables in
-// LDS (tw[m] = W_N^m): t1k[16 k2 + n3] = W_N^(R1 n3 k2) (k2 < 32, n3 < 16)
-// and t1k[512 + R1 n2 + k1] = W_N^(16 k1 n2) (n2 < 32, k1 < R1).  A thread
-// reads one row of each at a fixed column, so every read is its base
-// address plus an immediate offset.  1536 entries at most.
+// The twiddles between steps 2 and 3 of an N = 512 R1 transform, W_N^(R1 n3
+// k2) = W_512^(n3 k2) (n3 < 16, k2 < 32, tw[m] = W_N^m), as two tables in
+// LDS: t1k[16 k2 + n3] for the inverse (a thread reads the row of its n3 at
+// every k2) and t1k[512 + 32 n3 + k2] for the forward transform (a thread
+// reads the row of its k2 at every n3).  Either way every read is the
+// thread's base address plus an immediate offset, and the 16 addresses of a
+// wave's lanes are 128 consecutive bytes.
 template <int R1>
 __device__ __forceinline__ void load_step2_tables(float2* t1k, const float2* __restrict__ tw, int t)
 {
-    t1k[t] = tw[((t & 15) * (t >> 4)) * R1];  // R1 n3 k2 <= 465 R1 < N
-    for(int j = t; j < 32 * R1; j += 512) t1k[512 + j] = tw[16 * (j % R1) * (j / R1)];  // 16 k1 n2 < N
+    t1k[t]       = tw[((t & 15) * (t >> 4)) * R1];  // R1 n3 k2 <= 465 R1 < N
+    t1k[512 + t] = tw[((t >> 5) * (t & 31)) * R1];
+}
+
+// The step-1 twiddles W_N^(t k1), k1 = 1..R1-1, of thread t: copied from
+// wk where a kernel holds them for its whole run (HELD), else built from
+// W_N^t at the point of use (the same product tree, so the same values; the
+// opaque copy keeps the compiler from hoisting the tree and holding it).
+template <int R1, bool HELD>
+__device__ __forceinline__ void step1_twiddles(const float2* wk, const Twiddles& tw, int t, float2 (&w)[R1])
+{
+    if constexpr(HELD) {
+#pragma unroll
+        for(int k = 1; k < R1; ++k) w[k] = wk[k];
+    } else {
+        float2 w1 = tw(t);
+        asm volatile("" : "+v"(w1.x), "+v"(w1.y));
+        twiddle_powers<R1>(w1, w);
+    }
 }
 
 // Forward steps 1-3 of an N = 512 R1 transform of C = 32 / R1 columns: v
 // holds the thread's inputs (value c R1 + n1 = sample 512 n1 + t of column
 // c) and ends holding X[k] of the pairs q = t, t + 512 (16 values each,
-// k3 = 0..15).  The caller has synchronised since the exchange buffer was
-// last read.
-// wk, when given, holds the step-1 twiddles W_N^(t k1), k1 < R1 (they
-// depend on the thread alone), which are otherwise built per call.
-template <int R1, bool HOLD = false>
+// k3 = 0..15).  Step-1 twiddles as step1_twiddles<R1, HELD> gives them.
+// The caller has synchronised since the exchange buffer was last read.
+template <int R1, bool HELD>
 __device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddles& tw, const float2* t1k, int t,
-                                            const float2* wk = nullptr)
+                                            const float2* wk)
 {
     constexpr int C = 32 / R1;
     const int n3 = t & 15, j2 = t >> 4;
-    if constexpr(HOLD) {
 #pragma unroll
-        for(int c = 0; c < C; ++c) {
-            dft<R1, false>(v + c * R1);
+    for(int c = 0; c < C; ++c) dft<R1, false>(v + c * R1);
+    {
+        float2 w[R1];
+        step1_twiddles<R1, HELD>(wk, tw, t, w);
 #pragma unroll
-            for(int k = 1; k < R1; ++k) v[c * R1 + k] = cmul(v[c * R1 + k], wk[k]);
-        }
-    } else {
-        const float2 w = tw(t);
+        for(int c = 0; c < C; ++c)
 #pragma unroll
-        for(int c = 0; c < C; ++c) {
-            dft<R1, false>(v + c * R1);
-            apply_powers<R1>(v + c * R1, w);
-        }
+            for(int k = 1; k < R1; ++k) v[c * R1 + k] = cmul(v[c * R1 + k], w[k]);
     }
 #pragma unroll
     for(int i = 0; i < 32; ++i) lds[i * kEx1Row + t] = v[i];  // row i = c R1 + k1
     __syncthreads();
-    // step 2: thread (j2 = c R1 + k1, n3) holds n2 = 0..31
+    // step 2: thread (j2 = c R1 + k1, n3) holds n2 = 0..31: DFT32 -> k2
 #pragma unroll
     for(int n2 = 0; n2 < 32; ++n2) v[n2] = lds[j2 * kEx1Row + n2 * 16 + n3];
     __syncthreads();
     dft32<false>(v);
-    // x W_N^(R1 n3 k2) = W_512^(n3 k2) from the table of step 2 (row k2,
-    // column n3: one address per thread, the rows at immediate offsets; the
-    // 16 addresses of a wave are 128 consecutive bytes)
-    {
-        const float2* tk = t1k + n3;
-#pragma unroll
-        for(int k2 = 1; k2 < 32; ++k2) v[k2] = cmul(v[k2], tk[16 * k2]);
-    }
 #pragma unroll
     for(int k2 = 0; k2 < 32; ++k2) lds[j2 * kEx2Row + k2 * kEx2Stride + n3] = v[k2];
     wave_lds_sync();
     // step 3: thread (j2, n3) takes k2 = n3 and n3 + 16 of its own row j2,
-    // which the 16 lanes of that row (in this wave) have just written: DFT16
-    // over n3 -> k3
+    // which the 16 lanes of that row (in this wave) have just written:
+    // x W_N^(R1 n3 k2) (the twiddle of value n3, folded into the first
+    // butterflies), DFT16 over n3 -> k3
 #pragma unroll
     for(int h = 0; h < 2; ++h) {
         const int k2 = n3 + 16 * h;
         float2* x = v + 16 * h;
+        const float2* tk = t1k + 512 + k2;
+        float2 w[16];
 #pragma unroll
-        for(int m = 0; m < 16; ++m) x[m] = lds[j2 * kEx2Row + k2 * kEx2Stride + m];
-        dft16<false>(x);
+        for(int m = 0; m < 16; ++m) {
+            x[m] = lds[j2 * kEx2Row + k2 * kEx2Stride + m];
+            if(m > 0) w[m] = tk[32 * m];
+        }
+        dft16_mul<false, true>(x, w);
     }
 }
 
 // Y = X H with the chirp stored permuted (index 32 R1 k3 + 32 k1 + k2), then
 // the inverse steps 3'-1': v ends holding y[512 n1 + t] (unnormalised) at
 // c R1 + n1.  X (the spectrum reg_forward left) may be v itself, or be kept
-// unchanged for further DMs.
-template <int R1>
+// unchanged for further DMs.  The inverse's twiddle W_N^(-k1 (16 n2 + n3))
+// between steps 2' and 1' is the conjugate of step-1 twiddle k1 of the
+// thread t = 16 n2 + n3 that runs step 1'.
+template <int R1, bool HELD>
 __device__ __forceinline__ void reg_chirp_inverse(const float2* X, float2* v, float2* lds, const Twiddles& tw,
-                                                  const float2* t1k, int t, const float2* __restrict__ H)
+                                                  const float2* t1k, int t, const float2* __restrict__ H,
+                                                  const float2* wk)
 {
     constexpr int C = 32 / R1;
     const int n3 = t & 15, j2 = t >> 4;
     const int k1_2 = j2 % R1;
+    // step 3': IDFT16 of X H over k3 -> n3 (the chirp products folded into
+    // the first butterflies)
 #pragma unroll
     for(int h = 0; h < 2; ++h) {
         const int k2 = n3 + 16 * h, k1 = k1_2;
@@ -334,12 +349,7 @@ __device__ __forceinline__ void reg_chirp_inverse(const float2* X, float2* v, fl
             hk[k3] = H[32 * R1 * k3 + 32 * k1 + k2];
             x[k3]  = X[16 * h + k3];
         }
-        // step 3': IDFT16 of X H -> n3 (the chirp products folded into the
-        // first butterflies), x W_N^(-R1 n3 k2) and W_N^(-k1 n3): the
-        // second factor of step 2's twiddle W_N^(-k1 (16 n2 + n3)) does not
-        // depend on k2, so by linearity it is applied before the IDFT32
         dft16_mul<true>(x, hk);
-        apply_powers<16>(x, cconj(tw(R1 * k2 + k1)));
     }
     // row j2 of the exchange buffer is this wave's alone between the two
     // workgroup barriers of a transform: written and read back by its own
@@ -351,25 +361,31 @@ __device__ __forceinline__ void reg_chirp_inverse(const float2* X, float2* v, fl
         for(int m = 0; m < 16; ++m) lds[j2 * kEx2Row + k2 * kEx2Stride + m] = v[16 * h + m];
     }
     wave_lds_sync();
-    // step 2': thread (j2, n3) holds k2 = 0..31; IDFT32 -> n2, x W_N^(-k1 (16 n2 + n3))
+    // step 2': thread (j2, n3) holds k2 = 0..31: x W_N^(-R1 n3 k2) (folded
+    // into the first butterflies), IDFT32 -> n2
 #pragma unroll
     for(int k2 = 0; k2 < 32; ++k2) v[k2] = lds[j2 * kEx2Row + k2 * kEx2Stride + n3];
     __syncthreads();  // every wave is done with its rows before the buffer is reused
-    dft32<true>(v);
-    // x W_N^(-16 k1 n2), from the table of step 2' (row n2, column k1)
     {
-        const float2* tk = t1k + 512 + k1_2;
+        const float2* tk = t1k + n3;
+        float2 w[32];
 #pragma unroll
-        for(int n2 = 1; n2 < 32; ++n2) v[n2] = cmul(v[n2], cconj(tk[R1 * n2]));
+        for(int k2 = 1; k2 < 32; ++k2) w[k2] = cconj(tk[16 * k2]);
+        dft32_mul<true, true>(v, w);
     }
 #pragma unroll
     for(int n2 = 0; n2 < 32; ++n2) lds[j2 * kEx1Row + n2 * 16 + n3] = v[n2];
     __syncthreads();
-    // step 1': thread t holds k1 = 0..R1-1 of each column; IDFT_R1 -> n1
+    // step 1': thread t holds k1 = 0..R1-1 of each column: x W_N^(-k1 t)
+    // (folded into the first butterflies for R1 >= 16), IDFT_R1 -> n1
 #pragma unroll
     for(int i = 0; i < 32; ++i) v[i] = lds[i * kEx1Row + t];
+    float2 wc[R1];
+    step1_twiddles<R1, HELD>(wk, tw, t, wc);
 #pragma unroll
-    for(int c = 0; c < C; ++c) dft<R1, true>(v + c * R1);
+    for(int k = 1; k < R1; ++k) wc[k] = cconj(wc[k]);
+#pragma unroll
+    for(int c = 0; c < C; ++c) dft_mul<R1, true>(v + c * R1, wc);
 }
 
 // Raw complex int8 samples b + 512 i + t, i = 0..2 n - 1, of one column,
@@ -490,9 +506,9 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
 {
     constexpr int N = 512 * R1;
     constexpr int C = 32 / R1;
-    __shared__ float2 lds[kRegLds + TwiddleLds<N>::SIZE + 1536];
+    __shared__ float2 lds[kRegLds + TwiddleLds<N>::SIZE + 1024];
     const Twiddles tw = load_twiddles<N, 512>(lds + kRegLds, p.tw);
-    float2* t1k = lds + kRegLds + TwiddleLds<N>::SIZE;  // step 2 and 2' twiddle tables
+    float2* t1k = lds + kRegLds + TwiddleLds<N>::SIZE;  // step 2/3 twiddle tables
     const int t = threadIdx.x;
     load_step2_tables<R1>(t1k, p.tw, t);
     const int ngrp = p.PA / C;
@@ -520,7 +536,7 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
         if(u + (int)gridDim.x < nunits) fetch(u + gridDim.x);  // in flight during the transforms
         __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
         reg_forward<R1, true>(v, lds, tw, t1k, t, wk);
-        reg_chirp_inverse<R1>(v, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N);
+        reg_chirp_inverse<R1, true>(v, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N, wk);
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
         store_unit<R1>(p, d, (size_t)f * p.PA + cg * C, j0, t, v);  // v already / N (chirp)
     }
@@ -536,9 +552,9 @@ __global__ void __launch_bounds__(512) codedisp_regm_kernel(CdParams p, int nuni
 {
     constexpr int N = 512 * R1;
     constexpr int C = 32 / R1;
-    __shared__ float2 lds[kRegLds + TwiddleLds<N>::SIZE + 1536];
+    __shared__ float2 lds[kRegLds + TwiddleLds<N>::SIZE + 1024];
     const Twiddles tw = load_twiddles<N, 512>(lds + kRegLds, p.tw);
-    float2* t1k = lds + kRegLds + TwiddleLds<N>::SIZE;  // step 2 and 2' twiddle tables
+    float2* t1k = lds + kRegLds + TwiddleLds<N>::SIZE;  // step 2/3 twiddle tables
     const int t = threadIdx.x;
     load_step2_tables<R1>(t1k, p.tw, t);
     const int ngrp = p.PA / C;
@@ -552,8 +568,11 @@ __global__ void __launch_bounds__(512) codedisp_regm_kernel(CdParams p, int nuni
     };
     if(blockIdx.x < nunits) fetch(blockIdx.x);
     __syncthreads();  // twiddle tables
-    float2 wk[R1];    // step-1 twiddles W_N^(t k1), the same for every unit
-    twiddle_powers<R1>(tw(t), wk);
+    // step-1 twiddles W_N^(t k1), the same for every unit; from R1 = 16
+    // they are rebuilt at each use (held beside X they would spill)
+    constexpr bool HELD = R1 <= 8;
+    float2 wk[R1];
+    if constexpr(HELD) twiddle_powers<R1>(tw(t), wk);
     for(int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int cg = u % ngrp, fs = u / ngrp;
         const int f = fs / p.nseg, s = fs % p.nseg;
@@ -564,13 +583,13 @@ __global__ void __launch_bounds__(512) codedisp_regm_kernel(CdParams p, int nuni
             X[2 * i + 1] = unpack_c8(nxt[i] >> 16);
         }
         __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
-        reg_forward<R1, (R1 < 32)>(X, lds, tw, t1k, t, wk);  // at R1 = 32 the held twiddles would spill beside X
+        reg_forward<R1, HELD>(X, lds, tw, t1k, t, wk);
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
         for(int d = 0; d < p.ndm; ++d) {
             if(d > 0) __syncthreads();  // the previous DM's readers of the exchange buffer
             if(d == p.ndm - 1 && u + (int)gridDim.x < nunits) fetch(u + gridDim.x);
             float2 v[32];
-            reg_chirp_inverse<R1>(X, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N);
+            reg_chirp_inverse<R1, HELD>(X, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N, wk);
             store_unit<R1>(p, d, (size_t)f * p.PA + cg * C, j0, t, v);  // v already / N (chirp)
         }
     }
@@ -590,9 +609,9 @@ __global__ void __launch_bounds__(512) codedisp_regm_kernel(CdParams p, int nuni
 __global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, int nunits, float2* scratch)
 {
     constexpr int L = 16384, N = 2 * L;
-    __shared__ float2 lds[kRegLds + TwiddleLds<L>::SIZE + 1536];
+    __shared__ float2 lds[kRegLds + TwiddleLds<L>::SIZE + 1024];
     const Twiddles tw = load_twiddles<L, 512>(lds + kRegLds, p.tw);
-    float2* t1k = lds + kRegLds + TwiddleLds<L>::SIZE;  // step 2 and 2' twiddle tables
+    float2* t1k = lds + kRegLds + TwiddleLds<L>::SIZE;  // step 2/3 twiddle tables
     const int t = threadIdx.x;
     load_step2_tables<32>(t1k, p.tw, t);
     uint32_t raw[32];  // x[512 n1 + t] at 0..15, x[L + 512 n1 + t] at 16..31, two per register
@@ -630,8 +649,10 @@ __global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, i
                 for(int n1 = 0; n1 < 32; ++n1) v[n1] = cmul(v[n1], wn[512 * n1]);
             }
             __syncthreads();  // twiddle tables; previous readers of the exchange buffer
-            reg_forward<32>(v, lds, tw, t1k, t);
-            reg_chirp_inverse<32>(v, v, lds, tw, t1k, t, H + r * L);
+            // step-1 twiddles rebuilt at each use (held, they would not fit
+            // the registers)
+            reg_forward<32, false>(v, lds, tw, t1k, t, nullptr);
+            reg_chirp_inverse<32, false>(v, v, lds, tw, t1k, t, H + r * L, nullptr);
             if(r == 0) {
 #pragma unroll
                 for(int n1 = 0; n1 < 32; ++n1) mine[512 * n1] = v[n1];

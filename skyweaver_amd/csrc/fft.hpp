@@ -178,13 +178,14 @@ __device__ __forceinline__ void dft16(float2* v)
 }
 
 // DFT16 of v[i] m[i]: the products are folded into the first layer's
-// butterflies (v and m are read, v is overwritten with the transform)
-template <bool INV>
+// butterflies (v and m are read, v is overwritten with the transform).
+// ONE0: m[0] is 1 and is not read.
+template <bool INV, bool ONE0 = false>
 __device__ __forceinline__ void dft16_mul(float2* v, const float2* m)
 {
 #pragma unroll
     for(int n2 = 0; n2 < 4; ++n2) {
-        float2 p = cmul(v[n2], m[n2]), b = v[8 + n2];
+        float2 p = (ONE0 && n2 == 0) ? v[0] : cmul(v[n2], m[n2]), b = v[8 + n2];
         float2 q = cmul(v[4 + n2], m[4 + n2]), d = v[12 + n2];
         bfly_w(p, b, m[8 + n2]);
         bfly_w(q, d, m[12 + n2]);
@@ -212,19 +213,11 @@ __device__ constexpr float kW32[16][2] = {
     {-0.92387953251128673848f, 0.38268343236508989280f},
     {-0.98078528040323043058f, 0.19509032201612860891f}};
 
-// natural order in and out: radix-2 split into two DFT16 of the even and odd
-// samples, X[k] = E[k] + W32^k O[k], X[k + 16] = E[k] - W32^k O[k]
+// The radix-2 step of a DFT32 from the DFT16s E of the even and O of the
+// odd samples: X[k] = E[k] + W32^k O[k], X[k + 16] = E[k] - W32^k O[k]
 template <bool INV>
-__device__ __forceinline__ void dft32(float2* v)
+__device__ __forceinline__ void dft32_combine(float2* v, float2* e, float2* o)
 {
-    float2 e[16], o[16];
-#pragma unroll
-    for(int i = 0; i < 16; ++i) {
-        e[i] = v[2 * i];
-        o[i] = v[2 * i + 1];
-    }
-    dft16<INV>(e);
-    dft16<INV>(o);
 #pragma unroll
     for(int k = 0; k < 16; ++k) {
         if(k == 0) {
@@ -248,6 +241,40 @@ __device__ __forceinline__ void dft32(float2* v)
     }
 }
 
+// natural order in and out: radix-2 split into two DFT16 of the even and odd
+// samples
+template <bool INV>
+__device__ __forceinline__ void dft32(float2* v)
+{
+    float2 e[16], o[16];
+#pragma unroll
+    for(int i = 0; i < 16; ++i) {
+        e[i] = v[2 * i];
+        o[i] = v[2 * i + 1];
+    }
+    dft16<INV>(e);
+    dft16<INV>(o);
+    dft32_combine<INV>(v, e, o);
+}
+
+// DFT32 of v[i] m[i] (natural order), the products folded into the DFT16s'
+// first butterflies; ONE0: m[0] is 1 and is not read
+template <bool INV, bool ONE0 = false>
+__device__ __forceinline__ void dft32_mul(float2* v, const float2* m)
+{
+    float2 e[16], o[16], me[16], mo[16];
+#pragma unroll
+    for(int i = 0; i < 16; ++i) {
+        e[i]  = v[2 * i];
+        o[i]  = v[2 * i + 1];
+        me[i] = (ONE0 && i == 0) ? make_float2(1.0f, 0.0f) : m[2 * i];
+        mo[i] = m[2 * i + 1];
+    }
+    dft16_mul<INV, ONE0>(e, me);
+    dft16_mul<INV>(o, mo);
+    dft32_combine<INV>(v, e, o);
+}
+
 template <int R, bool INV>
 __device__ __forceinline__ void dft(float2* v)
 {
@@ -262,6 +289,22 @@ __device__ __forceinline__ void dft(float2* v)
     } else {
         static_assert(R == 32, "radix");
         dft32<INV>(v);
+    }
+}
+
+// DFT_R of v[k] m[k] with m[0] = 1 (not read): the products folded into
+// the first butterflies for R = 16, 32
+template <int R, bool INV>
+__device__ __forceinline__ void dft_mul(float2* v, const float2* m)
+{
+    if constexpr(R == 32) {
+        dft32_mul<INV, true>(v, m);
+    } else if constexpr(R == 16) {
+        dft16_mul<INV, true>(v, m);
+    } else {
+#pragma unroll
+        for(int k = 1; k < R; ++k) v[k] = cmul(v[k], m[k]);
+        dft<R, INV>(v);
     }
 }
 

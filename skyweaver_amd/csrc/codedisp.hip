@@ -205,7 +205,13 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
 // 16 k1 n2 in units of 1/N) and come from LDS tables of W_1024^j and W_512^j;
 // the others are built by product trees from one table value.  (Keeping the
 // spectrum for a second DM would not fit the registers: one DM per launch.)
-constexpr int kEx1Row = 528;   // exchange 1/4 row (c, k1): 512 points + 16 pad
+// Every exchange row j2 = (c, k1) has one pitch: the 512 points of
+// exchanges 1/4 and the 32 rows of 17 of exchanges 2/3 both sit in
+// [560 j2, 560 j2 + 560).  Row j2 is read in steps 2 and 2' and written in
+// exchanges 2, 3 and 4 only by the 16 lanes (j2, n3) of one wave, so between
+// the two column-wise exchanges (1 and 4) a wave touches only its own rows
+// and needs no workgroup barrier.
+constexpr int kEx1Row = 560;   // exchange 1/4 row (c, k1): 512 points + padding
 constexpr int kEx2Row = 560;   // exchange 2/3 block (c, k1): 32 rows of 17
 constexpr int kEx2Stride = 17; // padded row of 16 points
 constexpr int kRegLds = 32 * kEx2Row;
@@ -299,7 +305,7 @@ __device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddl
     // step 2: thread (j2 = c R1 + k1, n3) holds n2 = 0..31: DFT32 -> k2
 #pragma unroll
     for(int n2 = 0; n2 < 32; ++n2) v[n2] = lds[j2 * kEx1Row + n2 * 16 + n3];
-    __syncthreads();
+    wave_lds_sync();  // the wave's own rows are rewritten next
     dft32<false>(v);
 #pragma unroll
     for(int k2 = 0; k2 < 32; ++k2) lds[j2 * kEx2Row + k2 * kEx2Stride + n3] = v[k2];
@@ -365,7 +371,7 @@ __device__ __forceinline__ void reg_chirp_inverse(const float2* X, float2* v, fl
     // into the first butterflies), IDFT32 -> n2
 #pragma unroll
     for(int k2 = 0; k2 < 32; ++k2) v[k2] = lds[j2 * kEx2Row + k2 * kEx2Stride + n3];
-    __syncthreads();  // every wave is done with its rows before the buffer is reused
+    wave_lds_sync();  // the wave's own rows are rewritten next
     {
         const float2* tk = t1k + n3;
         float2 w[32];
@@ -534,7 +540,8 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
             v[2 * i + 1] = unpack_c8(nxt[i] >> 16);
         }
         if(u + (int)gridDim.x < nunits) fetch(u + gridDim.x);  // in flight during the transforms
-        __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
+        // no barrier: step 1 writes the column t that this thread alone read
+        // in the previous unit's step 1'
         reg_forward<R1, true>(v, lds, tw, t1k, t, wk);
         reg_chirp_inverse<R1, true>(v, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N, wk);
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
@@ -582,11 +589,12 @@ __global__ void __launch_bounds__(512) codedisp_regm_kernel(CdParams p, int nuni
             X[2 * i]     = unpack_c8(nxt[i]);
             X[2 * i + 1] = unpack_c8(nxt[i] >> 16);
         }
-        __syncthreads();  // twiddle tables; previous unit's readers of the exchange buffer
+        // no barrier: step 1 writes the column t that this thread alone read
+        // in the previous DM's step 1'
         reg_forward<R1, HELD>(X, lds, tw, t1k, t, wk);
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
         for(int d = 0; d < p.ndm; ++d) {
-            if(d > 0) __syncthreads();  // the previous DM's readers of the exchange buffer
+            if(d > 0) __syncthreads();  // the previous DM's step-1' readers of every row
             if(d == p.ndm - 1 && u + (int)gridDim.x < nunits) fetch(u + gridDim.x);
             float2 v[32];
             reg_chirp_inverse<R1, HELD>(X, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N, wk);

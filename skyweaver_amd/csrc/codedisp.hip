@@ -261,6 +261,19 @@ __device__ __forceinline__ void load_step2_tables(float2* t1k, const float2* __r
     t1k[512 + t] = tw[((t >> 5) * (t & 31)) * R1];
 }
 
+// Chirp value k3 of the thread's column (k1, k2) of an N = 512 R1 chirp row
+// (index 32 R1 k3 + 32 k1 + k2), read through a buffer resource over the
+// row: one offset register per thread and a scalar offset per k3, so a
+// run of these loads needs no address registers of its own.
+template <int R1>
+__device__ __forceinline__ float2 chirp_at(__amdgpu_buffer_rsrc_t rs, int voff, int k3)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, 256 * R1 * k3, 0);
+    // (__builtin_bit_cast of an element of the returned vector reads element
+    // 0 with this compiler; __uint_as_float takes the element's value)
+    return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+}
+
 // The step-1 twiddles W_N^(t k1), k1 = 1..R1-1, of thread t: copied from
 // wk where a kernel holds them for its whole run (HELD), else built from
 // W_N^t at the point of use (the same product tree, so the same values; the
@@ -283,9 +296,12 @@ __device__ __forceinline__ void step1_twiddles(const float2* wk, const Twiddles&
 // c) and ends holding X[k] of the pairs q = t, t + 512 (16 values each,
 // k3 = 0..15).  Step-1 twiddles as step1_twiddles<R1, HELD> gives them.
 // The caller has synchronised since the exchange buffer was last read.
-template <int R1, bool HELD>
+// With Hp given, the first half of the chirp values that reg_chirp_inverse
+// multiplies by (k2 = n3, the chirp of the DM that follows) is loaded into
+// hk0 during steps 2 and 3, so that the inverse does not wait for it.
+template <int R1, bool HELD, bool PRE = false>
 __device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddles& tw, const float2* t1k, int t,
-                                            const float2* wk)
+                                            const float2* wk, const float2* __restrict__ Hp, float2 (&hk0)[16])
 {
     constexpr int C = 32 / R1;
     const int n3 = t & 15, j2 = t >> 4;
@@ -302,6 +318,12 @@ __device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddl
 #pragma unroll
     for(int i = 0; i < 32; ++i) lds[i * kEx1Row + t] = v[i];  // row i = c R1 + k1
     __syncthreads();
+    if constexpr(PRE) {
+        const auto rs = buffer_rsrc(Hp, 512u * R1 * 8u);
+        const int voff = 8 * (32 * (j2 % R1) + n3);
+#pragma unroll
+        for(int k3 = 0; k3 < 16; ++k3) hk0[k3] = chirp_at<R1>(rs, voff, k3);
+    }
     // step 2: thread (j2 = c R1 + k1, n3) holds n2 = 0..31: DFT32 -> k2
 #pragma unroll
     for(int n2 = 0; n2 < 32; ++n2) v[n2] = lds[j2 * kEx1Row + n2 * 16 + n3];
@@ -335,24 +357,29 @@ __device__ __forceinline__ void reg_forward(float2* v, float2* lds, const Twiddl
 // unchanged for further DMs.  The inverse's twiddle W_N^(-k1 (16 n2 + n3))
 // between steps 2' and 1' is the conjugate of step-1 twiddle k1 of the
 // thread t = 16 n2 + n3 that runs step 1'.
-template <int R1, bool HELD>
+template <int R1, bool HELD, bool PRE = false>
 __device__ __forceinline__ void reg_chirp_inverse(const float2* X, float2* v, float2* lds, const Twiddles& tw,
                                                   const float2* t1k, int t, const float2* __restrict__ H,
-                                                  const float2* wk)
+                                                  const float2* wk, const float2 (&hk0)[16])
 {
     constexpr int C = 32 / R1;
     const int n3 = t & 15, j2 = t >> 4;
     const int k1_2 = j2 % R1;
     // step 3': IDFT16 of X H over k3 -> n3 (the chirp products folded into
     // the first butterflies)
+    const auto rs = buffer_rsrc(H, 512u * R1 * 8u);
 #pragma unroll
     for(int h = 0; h < 2; ++h) {
         const int k2 = n3 + 16 * h, k1 = k1_2;
         float2* x = v + 16 * h;
         float2 hk[16];
+        const int voff = 8 * (32 * k1 + k2);
 #pragma unroll
         for(int k3 = 0; k3 < 16; ++k3) {
-            hk[k3] = H[32 * R1 * k3 + 32 * k1 + k2];
+            if constexpr(PRE)
+                hk[k3] = h == 0 ? hk0[k3] : chirp_at<R1>(rs, voff, k3);
+            else
+                hk[k3] = H[32 * R1 * k3 + 32 * k1 + k2];
             x[k3]  = X[16 * h + k3];
         }
         dft16_mul<true>(x, hk);
@@ -542,8 +569,10 @@ __global__ void __launch_bounds__(512) codedisp_reg_kernel(CdParams p, int d, in
         if(u + (int)gridDim.x < nunits) fetch(u + gridDim.x);  // in flight during the transforms
         // no barrier: step 1 writes the column t that this thread alone read
         // in the previous unit's step 1'
-        reg_forward<R1, true>(v, lds, tw, t1k, t, wk);
-        reg_chirp_inverse<R1, true>(v, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N, wk);
+        const float2* H = p.chirp[d] + (size_t)f * N;
+        float2 hk0[16];
+        reg_forward<R1, true, true>(v, lds, tw, t1k, t, wk, H, hk0);
+        reg_chirp_inverse<R1, true, true>(v, v, lds, tw, t1k, t, H, wk, hk0);
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
         store_unit<R1>(p, d, (size_t)f * p.PA + cg * C, j0, t, v);  // v already / N (chirp)
     }
@@ -591,13 +620,14 @@ __global__ void __launch_bounds__(512) codedisp_regm_kernel(CdParams p, int nuni
         }
         // no barrier: step 1 writes the column t that this thread alone read
         // in the previous DM's step 1'
-        reg_forward<R1, HELD>(X, lds, tw, t1k, t, wk);
+        float2 hk0[16];  // no chirp prefetch here: each DM has its own chirp
+        reg_forward<R1, HELD>(X, lds, tw, t1k, t, wk, nullptr, hk0);
         const int j0 = (int)(p.y0 + (long)s * p.hop) - p.O / 2;  // ring index of position 0
         for(int d = 0; d < p.ndm; ++d) {
             if(d > 0) __syncthreads();  // the previous DM's step-1' readers of every row
             if(d == p.ndm - 1 && u + (int)gridDim.x < nunits) fetch(u + gridDim.x);
             float2 v[32];
-            reg_chirp_inverse<R1, HELD>(X, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N, wk);
+            reg_chirp_inverse<R1, HELD>(X, v, lds, tw, t1k, t, p.chirp[d] + (size_t)f * N, wk, hk0);
             store_unit<R1>(p, d, (size_t)f * p.PA + cg * C, j0, t, v);  // v already / N (chirp)
         }
     }
@@ -659,8 +689,9 @@ __global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, i
             __syncthreads();  // twiddle tables; previous readers of the exchange buffer
             // step-1 twiddles rebuilt at each use (held, they would not fit
             // the registers)
-            reg_forward<32, false>(v, lds, tw, t1k, t, nullptr);
-            reg_chirp_inverse<32, false>(v, v, lds, tw, t1k, t, H + r * L, nullptr);
+            float2 hk0[16];  // no chirp prefetch (registers)
+            reg_forward<32, false>(v, lds, tw, t1k, t, nullptr, nullptr, hk0);
+            reg_chirp_inverse<32, false>(v, v, lds, tw, t1k, t, H + r * L, nullptr, hk0);
             if(r == 0) {
 #pragma unroll
                 for(int n1 = 0; n1 < 32; ++n1) mine[512 * n1] = v[n1];

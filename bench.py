@@ -44,7 +44,7 @@ VALU_ADD16_PEAK = 78.6e12   # 16-bit lane adds/s: 256 CUs x 64 lanes x 2 lanes p
 # HBM bytes per launch of the two large kernels at this workload, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE calibrated on a 1 GiB copy kernel
 # (profiles/hbm_traffic_pipeline_64ant_64ch_65536samp_800beam_dm50.md)
-TRAFFIC_BYTES = {"coherent_beam": 1.314e9, "dedispersion": 2.217e9}
+TRAFFIC_BYTES = {"coherent_beam": 1.314e9, "dedispersion": 2.218e9}
 INT8_LDS_BYTES_PER_CLK = 128   # LDS bytes per clock per CU
 NUM_CU, PEAK_CLK_GHZ = 256, 2.4
 

@@ -37,7 +37,7 @@ def main():
     here = os.path.dirname(os.path.abspath(__file__))
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--out", default=os.path.join(here, "tuning", "gfx950.json"))
-    ap.add_argument("--chans", type=int, default=8, help="channels the timings run on (scaled to the workload)")
+    ap.add_argument("--chans", type=int, default=64, help="channels the timings run on (scaled to the workload)")
     args = ap.parse_args()
     sys.path.insert(0, os.path.dirname(here))
     import skyweaver_amd as sw

@@ -704,12 +704,35 @@ __global__ void __launch_bounds__(512) codedisp_reg2_kernel(CdParams p, int d, i
         asm volatile("" : "+v"(tt));
         const float2* mine = scratch + (size_t)blockIdx.x * L + tt;
         const float2* wn = p.twN + tt;
+        const int half = p.O / 2;
+        if(half <= 512 && !p.outf[d]) {
+            // as store_unit: byte offsets from the column's base wrapped by
+            // one unsigned minimum; only rows 0 and 63 can hold dropped
+            // positions
+            int8_t* const ob = p.out[d] + cb * 2;
+            const uint32_t ylen2 = 2u * (uint32_t)p.ylen;
+            const uint32_t base = 2u * (uint32_t)(j0 + tt);
+            auto put = [&](int row, float2 y) {
+                const uint32_t o = base + 1024u * row;
+                const uint32_t re = __float_as_uint(y.x + 12582912.0f), im = __float_as_uint(y.y + 12582912.0f);
+                *reinterpret_cast<uint16_t*>(ob + min(o, o - ylen2)) = (uint16_t)((re & 0xffu) | ((im & 0xffu) << 8));
+            };
+            const bool lo_kept = tt >= half, hi_kept = tt < 512 - half;
 #pragma unroll
-        for(int n1 = 0; n1 < 32; ++n1) {
-            const float2 a = mine[512 * n1];
-            const float2 b = cmul(v[n1], cconj(wn[512 * n1]));
-            store_output(p, d, cb, j0, 512 * n1 + tt, N, cadd(a, b));  // V_0, V_1 already / N (chirp)
-            store_output(p, d, cb, j0, L + 512 * n1 + tt, N, csub(a, b));
+            for(int n1 = 0; n1 < 32; ++n1) {
+                const float2 a = mine[512 * n1];
+                const float2 b = cmul(v[n1], cconj(wn[512 * n1]));  // V_0, V_1 already / N (chirp)
+                if(n1 > 0 || lo_kept) put(n1, cadd(a, b));
+                if(n1 < 31 || hi_kept) put(32 + n1, csub(a, b));
+            }
+        } else {
+#pragma unroll
+            for(int n1 = 0; n1 < 32; ++n1) {
+                const float2 a = mine[512 * n1];
+                const float2 b = cmul(v[n1], cconj(wn[512 * n1]));
+                store_output(p, d, cb, j0, 512 * n1 + tt, N, cadd(a, b));
+                store_output(p, d, cb, j0, L + 512 * n1 + tt, N, csub(a, b));
+            }
         }
     }
 }

@@ -106,6 +106,34 @@ def test_codedisp_vs_fp64_oracle(sw, orc, dev, dm, N):
     _check(out.download(), outf.download(), ei, ef, f"DM {dm} N {N}")
 
 
+@pytest.mark.parametrize("N,O", [(4096, 1024), (8192, 1200), (32768, 1100)])
+def test_codedisp_large_overlap(sw, orc, dev, N, O):
+    """Overlap 1024 (the first and last position rows of a register unit
+    dropped whole) and overlaps above it (more rows dropped: the kernels'
+    general store path): against the oracle, and int8 outputs identical
+    with and without the float ring."""
+    from skyweaver_amd import DeviceBuffer as DB
+    F, A, T, dm = 2, 64, 40000, 50.0
+    cur = sw.synth_voltages((F, T, 2, A, 2), seed=0x5EED0012, sigma=16.0)
+    plan = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, [dm], fft_len=[N], overlap=[O])
+    plan_i = sw.CoherentDedisperser(F, A, T, F_LOW, CHAN_BW, TSAMP, [dm], fft_len=[N], overlap=[O])
+    assert plan.info(0)[1] == O
+    lat = plan.latency(0)
+    print(f"codedisp large overlap N={N} O={O} latency={lat}", flush=True)
+    d_cur = DB.from_host(cur)
+    out = DB.empty(cur.shape, np.int8, fill=0x55)
+    outf = DB.empty(cur.shape, np.float32)
+    out_i = DB.empty(cur.shape, np.int8, fill=0x33)
+    plan.run(d_cur, [out], [outf])
+    plan_i.run(d_cur, [out_i])
+    got = out.download()
+    np.testing.assert_array_equal(out_i.download(), got)
+    chirp = orc.dm_chirp(dm, F_LOW, CHAN_BW, F, N)
+    ei, ef = _oracle_stream(orc, np.zeros((F, O, 2, A, 2), np.int8), cur, chirp, N, O)
+    ei, ef = _window(ei, ef, -lat, T)
+    _check(got, outf.download(), ei, ef, f"DM {dm} N {N} O {O}")
+
+
 @pytest.mark.parametrize("N", [512, 4096, 32768])
 def test_codedisp_physical_chirp_vs_fp64_oracle(sw, orc, dev, N):
     """The physical bin-to-frequency mapping (channel-centred, upper bins

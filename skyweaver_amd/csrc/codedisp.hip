@@ -192,19 +192,17 @@ __global__ void __launch_bounds__(NT) codedisp_tm_kernel(CdParams p)
 // k = k1 + R1 k2 + 32 R1 k3 (n1, k1 < R1; n2, k2 < 32; n3, k3 < 16) the
 // forward transform is
 //   DFT_R1 over n1 (thread t = 16 n2 + n3, each column), x W_N^(t k1)
-//   -> exchange -> DFT32 over n2 (thread (c, k1, n3)), x W_N^(R1 n3 k2)
-//   -> exchange -> DFT16 over n3 (thread (c, k1, n3) takes k2 = n3, n3 + 16),
+//   -> exchange -> DFT32 over n2 (thread (c, k1, n3))
+//   -> exchange -> x W_N^(R1 n3 k2), DFT16 over n3 (thread (c, k1, n3)
+//      takes k2 = n3, n3 + 16),
 // which leaves X[k] in registers; the chirp is stored permuted to the order
 // the threads read it (index 32 R1 k3 + 32 k1 + k2).  The inverse runs the
 // same steps backwards with conjugate twiddles and ends with each thread
 // holding output samples 512 n1 + t of its columns.  Four LDS exchanges per
-// DM instead of a read-write round trip per radix pass; the middle two (step
-// 2 <-> step 3) stay inside the 16 lanes of a row, i.e. inside one wave, and
-// need no workgroup barrier.  The step-2
-// twiddles of both directions are powers of W_1024 (R1 n3 k2 and
-// 16 k1 n2 in units of 1/N) and come from LDS tables of W_1024^j and W_512^j;
-// the others are built by product trees from one table value.  (Keeping the
-// spectrum for a second DM would not fit the registers: one DM per launch.)
+// transform pair instead of a read-write round trip per radix pass; the
+// middle two (step 2 <-> step 3) stay inside the 16 lanes of a row, i.e.
+// inside one wave.  Twiddles that multiply a DFT's inputs are folded into
+// its first butterflies.
 // Every exchange row j2 = (c, k1) has one pitch: the 512 points of
 // exchanges 1/4 and the 32 rows of 17 of exchanges 2/3 both sit in
 // [560 j2, 560 j2 + 560).  Row j2 is read in steps 2 and 2' and written in
@@ -216,8 +214,8 @@ constexpr int kEx2Row = 560;   // exchange 2/3 block (c, k1): 32 rows of 17
 constexpr int kEx2Stride = 17; // padded row of 16 points
 constexpr int kRegLds = 32 * kEx2Row;
 // Lane groups of a wave that read or write one exchange row at a time start
-// 32 banks apart (528 and 560 float2 are 32 dwords mod 64), and the rows of
-// 17 points put 32 consecutive k2 on distinct banks.
+// 32 banks apart (560 float2 are 32 dwords mod 64), and the rows of 17
+// points put 32 consecutive k2 on distinct banks.
 
 // An LDS exchange among the lanes of one wave: a wave's LDS operations are
 // performed in program order, so a wave-scope fence (which also keeps the
@@ -227,24 +225,6 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// v[k] *= w1^k for k = 1..R-1 (R = 2 ... 32), powers by the product tree
-// of twiddle_powers, built 16 at a time to bound register use
-template <int R>
-__device__ __forceinline__ void apply_powers(float2* v, float2 w1)
-{
-    constexpr int R0 = R < 16 ? R : 16;
-    float2 w[R0];
-    twiddle_powers<R0>(w1, w);
-#pragma unroll
-    for(int k = 1; k < R0; ++k) v[k] = cmul(v[k], w[k]);
-    if constexpr(R == 32) {
-        const float2 w16 = cmul(w[8], w[8]);
-        v[16] = cmul(v[16], w16);
-#pragma unroll
-        for(int k = 1; k < 16; ++k) v[16 + k] = cmul(v[16 + k], cmul(w16, w[k]));
-    }
 }
 
 // The twiddles between steps 2 and 3 of an N = 512 R1 transform, W_N^(R1 n3

@@ -427,26 +427,37 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
             if(fib.raw) {  // uniform: the incoherent beam from the staged rows
                 const int o0 = tc * NO_WG;
                 const int r0 = lane / LPR, g = lane % LPR;
+                // one beamset over every antenna (uniform): a word's two
+                // antennas are summed by one dot product, no masks
+                const bool all1 =
+                    nbeamsets == 1 && __all(((ibm[0] & ibm[1] & ibm[2] & ibm[3]) & 3u) == 3u);
 #pragma unroll
                 for(int oi = 0; oi < NO_WG / WV; ++oi) {
                     const int o = wave * (NO_WG / WV) + oi;  // this wave's own rows
-                    int lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
+                    int lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0}, tot = 0;
 #pragma unroll
                     for(int i = 0; i < TS / RPI; ++i) {
                         const uint4 v = *reinterpret_cast<const uint4*>(stg + (o * TS + RPI * i + r0) * SPITCH + 16 * g);
                         const int w[4] = {(int)v.x, (int)v.y, (int)v.z, (int)v.w};
+                        if(all1) {
 #pragma unroll
-                        for(int j = 0; j < 4; ++j) {
-                            lo[j] = __builtin_amdgcn_sdot4(w[j], w[j] & 0xFFFF, lo[j], false);
-                            hi[j] = __builtin_amdgcn_sdot4(w[j], (int)((unsigned)w[j] & 0xFFFF0000u), hi[j], false);
+                            for(int j = 0; j < 4; ++j) tot = __builtin_amdgcn_sdot4(w[j], w[j], tot, false);
+                        } else {
+#pragma unroll
+                            for(int j = 0; j < 4; ++j) {
+                                lo[j] = __builtin_amdgcn_sdot4(w[j], w[j] & 0xFFFF, lo[j], false);
+                                hi[j] = __builtin_amdgcn_sdot4(w[j], (int)((unsigned)w[j] & 0xFFFF0000u), hi[j], false);
+                            }
                         }
                     }
                     for(int sb = 0; sb < nbeamsets; ++sb) {
-                        int part = 0;
+                        int part = tot;
+                        if(!all1) {
 #pragma unroll
-                        for(int j = 0; j < 4; ++j) {
-                            if(ibm[j] >> (2 * sb) & 1u) part += lo[j];
-                            if(ibm[j] >> (2 * sb) & 2u) part += hi[j];
+                            for(int j = 0; j < 4; ++j) {
+                                if(ibm[j] >> (2 * sb) & 1u) part += lo[j];
+                                if(ibm[j] >> (2 * sb) & 2u) part += hi[j];
+                            }
                         }
 #pragma unroll
                         for(int m = 32; m > 0; m >>= 1) part += __shfl_xor(part, m, 64);

@@ -361,13 +361,16 @@ def test_pipeline_full_size_8dm_forms_identical(sw, dev):
     assert np.unique(a["cb"][-1][1]).size > 100
 
 
-@pytest.mark.parametrize("nant,ts", [(64, 4), (32, 8), (128, 32), (16, 16), (48, 16), (64, 2)])
-def test_pipeline_fused_incoherent_beam_shapes(sw, dev, nant, ts):
+@pytest.mark.parametrize("nant,ts,sets", [(64, 4, 2), (32, 8, 2), (128, 32, 2), (16, 16, 2), (48, 16, 2), (64, 2, 2),
+                                          (64, 4, 1), (128, 32, 1), (64, 16, -1)])
+def test_pipeline_fused_incoherent_beam_shapes(sw, dev, nant, ts, sets):
     """The incoherent beam computed inside the coherent beamformer (antenna
     counts dividing 256, tscrunch * antennas >= 256) and, for the other
     shapes here (48 antennas; tscrunch 2), by its own kernel on the ring:
-    both equal the plain form's incoherent beam on FTPA, for two beamsets
-    over uneven antenna subsets and a DM with latency."""
+    both equal the plain form's incoherent beam on FTPA, for a DM with
+    latency and sets = 2: two beamsets over uneven antenna subsets; 1: one
+    beamset over every antenna (the fused form's single-dot-product path);
+    -1: one beamset over a subset."""
     from skyweaver_amd import DeviceBuffer as DB
     Fq, Tq, Bq = 4, 4096, 96
     bw = 13.375e6 * Fq / 64
@@ -375,10 +378,17 @@ def test_pipeline_fused_incoherent_beam_shapes(sw, dev, nant, ts):
     g = [DB.from_host(sw.synth_voltages((H, nant, Fq, 256, 2, 2), seed=100 + nant + ts + i, sigma=24.0))
          for i in range(2)]
     weights = sw.synth_weights((Fq, Bq, nant, 2), seed=200 + nant)
-    bmap = (np.arange(Bq) % 2).astype(np.int32)
-    antw = np.ones((2, nant), np.float32)
-    antw[1, nant // 3:] = 0
-    weights[:, 1::2, nant // 3:] = 0
+    if sets == 2:
+        bmap = (np.arange(Bq) % 2).astype(np.int32)
+        antw = np.ones((2, nant), np.float32)
+        antw[1, nant // 3:] = 0
+        weights[:, 1::2, nant // 3:] = 0
+    else:
+        bmap = np.zeros(Bq, np.int32)
+        antw = np.ones((1, nant), np.float32)
+        if sets < 0:
+            antw[0, ::5] = 0
+            weights[:, :, ::5] = 0
 
     def run(form, batched):
         p = sw.BeamformerPipeline(nant, Fq, Bq, Tq, ts, [0.0, 30.0], 1284e6, bw, fft_len=[1024, 2048], form=form,

@@ -323,13 +323,17 @@ idisp_mfma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const
             pre[k] = v;
         }
     };
-    auto stash = [&](const P (&pre)[NPF]) {
+    // only the rows the quad's DMs read are written: a channel of quad q
+    // needs TT ts + (its span of delays) rows, often far fewer than RCAP
+    auto stash = [&](int q, const P (&pre)[NPF]) {
 #pragma unroll
         for(int k = 0; k < NPF; ++k) {
             const int s = threadIdx.x + NT * k;
             const int j = (NT * k) / (RCAP * PPR);
             const int r = (s / PPR) % RCAP, c = s % PPR;
-            *reinterpret_cast<P*>(stage + (j * RCAP + r) * kMfPitch + c * PIECE) = pre[k];
+            const int2 spj = tspan[4 * q + j];
+            if(r < TT * ts + spj.y - spj.x)
+                *reinterpret_cast<P*>(stage + (j * RCAP + r) * kMfPitch + c * PIECE) = pre[k];
         }
     };
     // this lane's channel of quad q: its span and the delays of the wave's DMs
@@ -392,7 +396,7 @@ idisp_mfma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const
             lane_delays(q, spc, dc);  // issued with the row loads, before any wait
             fetch(q, preA);
             __syncthreads();  // the previous quad's reads are done
-            stash(preA);
+            stash(q, preA);
             __syncthreads();
             products(spc, dc);
         }
@@ -409,7 +413,7 @@ idisp_mfma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const
     for(int q = 0; q < nq; q += 2) {
         {
             __syncthreads();  // the previous quad's reads are done
-            stash(preA);
+            stash(q, preA);
             __syncthreads();
             const int2 spc = spA;
             int dc[DPW];
@@ -423,7 +427,7 @@ idisp_mfma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const
         }
         if(q + 1 < nq) {
             __syncthreads();
-            stash(preB);
+            stash(q + 1, preB);
             __syncthreads();
             const int2 spc = spB;
             int dc[DPW];

@@ -254,7 +254,7 @@ def bench_idisp(sw, steps):
     # the matrix-core form issues one 16x16x64 int8 product (16 useful adds
     # of 16384 MACs) per 16 beams x 16 samples x 4 channels, i.e. 64 int8
     # operations (MAC = 2 ops) per useful add
-    mfma_ops = adds * 64 if idd.kernel == "idisp_mfma_kernel" else 0.0
+    mfma_ops = adds * 64 if idd.kernel in ("idisp_mfma_kernel", "idisp_dma_kernel") else 0.0
     lds_bound_ms = adds / (INT8_LDS_BYTES_PER_CLK * NUM_CU * PEAK_CLK_GHZ * 1e9) * 1e3
     return {"kernel": idd.kernel, "layout": "FTB (channel-major aggregation ring)", "ms_per_dispatch": round(ms, 3), "steps": steps,
             "dispatch_samples": disp, "max_delay": idd.max_delay, "ndms": len(ID_DMS), "nchans": ID_F,

@@ -243,7 +243,11 @@ typedef struct {
 typedef struct sw_idisp sw_idisp;
 int sw_idisp_create(const sw_idisp_config* config, sw_idisp** out);
 int sw_idisp_destroy(sw_idisp* plan);
-/* "idisp_mfma_kernel" or "idisp_kernel": the kernel the plan launches */
+/* the kernel the plan launches: "idisp_dma_kernel" (matrix cores, 32 output
+ * samples per workgroup, tscrunch <= 2; rows whose width or strides are not
+ * multiples of 16 bytes run idisp_mfma_kernel's 32-sample form instead),
+ * "idisp_mfma_kernel" (matrix cores, 16 samples per workgroup) or
+ * "idisp_kernel" (VALU) */
 const char* sw_idisp_kernel(const sw_idisp* plan);
 /* max_sample_delay(), delays() [ndms][nchans] and the scale sqrt(F ts);
  * any output may be NULL. */

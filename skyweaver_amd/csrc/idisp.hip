@@ -29,6 +29,10 @@
 // selector, row m holding a single 1 at byte m of every k-group, so
 // C[m][n] = sum over g of beam b0 + m of channel f0 + g at sample n: four
 // channels x 16 beams x 16 samples per instruction, summed exactly in int32.
+// For tscrunch <= 2 with 16-byte rows the same products run in
+// idisp_dma_kernel: 16 waves of one DM each, 32 output samples per
+// workgroup, each quad of channels loaded straight into one of two LDS stage
+// buffers while the other quad's products run.
 #include "sw_common.hpp"
 
 #include <algorithm>
@@ -242,6 +246,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int kMfWaves = 8;    // waves per workgroup
 constexpr int kMfDpw   = 2;    // DMs per wave
+constexpr int kDmaWaves = 16;  // waves (one DM each) per workgroup of the double-buffered form
 constexpr int kMfRows  = 128;  // staged rows per channel
 constexpr int kMfPitch = 144;  // bytes between staged rows: 128 B of beams + 16;
                                // 16 consecutive rows start on distinct banks
@@ -467,6 +472,165 @@ idisp_mfma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const
     }
 }
 
+// Double-buffered form: a workgroup of NW = 16 waves (one DM each) owns TT =
+// 16 NTT output samples, a 128-byte chunk of beams and a tile of up to 16
+// consecutive DMs.  Each quad's rows go from global memory straight into one
+// of two LDS stage buffers (global_load_lds_dwordx4: lane L of a wave's load
+// writes the 16-byte slot L of a 1 KB block) while the previous quad's
+// products read the other buffer, one barrier per quad and no staging
+// registers.  The stage keeps idisp_mfma_kernel's layout (row pitch
+// kMfPitch: nine 16-byte slots, the ninth padding that no lane loads), and
+// the products read it exactly as there.  The two buffers take 144 KB: one
+// workgroup of 16 waves per CU.  Slots that no lane loads (padding, rows
+// past the tile's span or the input, beams past W) keep stale bytes: no
+// stored output reads them (rows past the span are read by no DM of the
+// tile, rows past the input feed only output samples >= nout, beams past W
+// only beams >= W, and the one-hot products sum each beam from its own
+// bytes).
+template <int NW, int NTT, int RCAP>
+__global__ void __launch_bounds__(NW * 64, 1)
+idisp_dma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const int* __restrict__ delays,
+                 const int2* __restrict__ span, const int2* __restrict__ tiles, int ntiles, int nchunks,
+                 int ntb, long w0, long nwork, int nchans, int W, int ndms, int ts, int max_delay,
+                 long nsamples, int nout, float scale, long stride_t, long stride_f)
+{
+    constexpr int TT  = 16 * NTT;
+    constexpr int SPR = kMfPitch / 16;  // 16-byte slots per staged row
+    constexpr int SPC = RCAP * SPR;     // slots per channel
+    static_assert(kMfPitch % 16 == 0 && SPC % 64 == 0, "a wave's 64-slot block stays in one channel");
+    constexpr int NBLK = 4 * SPC / 64;          // 1 KB blocks per quad
+    constexpr int NB   = (NBLK + NW - 1) / NW;  // blocks per wave per quad (the last round partial)
+    constexpr int QB   = 4 * RCAP * kMfPitch;
+    __shared__ __attribute__((aligned(16))) int8_t stage0[QB];
+    __shared__ __attribute__((aligned(16))) int8_t stage1[QB];
+
+    const long per = (nwork + 7) / 8;
+    const long wl  = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if(wl >= nwork) return;  // whole workgroup, before any barrier
+    const long w = w0 + wl;
+    constexpr int GRP = kIdTbGroup * kIdTileGroup;
+    const int gi = (int)(w % GRP);
+    const long rest  = w / GRP;
+    const int chunk  = (int)(rest % nchunks);
+    const long blk   = rest / nchunks;
+    const int ntbb   = (ntb + kIdTbGroup - 1) / kIdTbGroup;
+    const int tb     = (int)(blk % ntbb) * kIdTbGroup + gi % kIdTbGroup;
+    const int tile   = (int)(blk / ntbb) * kIdTileGroup + gi / kIdTbGroup;
+    if(tb >= ntb || tile >= ntiles) return;  // whole workgroup, before any barrier
+    const int2 td    = tiles[tile];
+    const int t0     = tb * TT;
+    const long in0   = (long)max_delay + (long)t0 * ts;
+    const int lane = threadIdx.x & 63;
+    // the wave index made uniform: the spans below are scalar loads whose
+    // waits do not hold back the row loads
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int n = lane & 15, g = lane >> 4;
+    const int b0 = chunk * 128;
+    const int nq = nchans / 4;
+    const int2* tspan = span + (size_t)tile * nchans;
+    const bool active = wv < td.y;  // this wave's DM is in the tile
+
+    // quad q's rows into stg: block k NW + wave of the quad, slot lane
+    auto issue = [&](int q, int8_t* stg) {
+        // the quad's four spans: scalar loads in flight together
+        const int2 sq0 = tspan[4 * q], sq1 = tspan[4 * q + 1], sq2 = tspan[4 * q + 2], sq3 = tspan[4 * q + 3];
+        int ln = lane;
+        asm volatile("" : "+v"(ln));  // the slot arithmetic is redone per quad, not held in registers
+#pragma unroll
+        for(int k = 0; k < NB; ++k) {
+            const int bk = k * NW + wv;  // wave-uniform
+            if(NBLK % NW != 0 && bk >= NBLK) break;
+            const int j  = bk * 64 / SPC;  // channel of the quad, wave-uniform
+            const int s  = bk * 64 - j * SPC + ln;
+            const int r = s / SPR, c = s % SPR;
+            const int2 spj = j == 0 ? sq0 : j == 1 ? sq1 : j == 2 ? sq2 : sq3;
+            const long row = in0 - spj.y + r;
+            const int b = b0 + c * 16;
+            if(c < 8 && r < TT * ts + spj.y - spj.x && b < W && row < nsamples)
+                __builtin_amdgcn_global_load_lds(in + row * stride_t + (long)(4 * q + j) * stride_f + b,
+                                                 (__attribute__((address_space(3))) void*)(stg + bk * 1024), 16,
+                                                 0, 0);
+        }
+    };
+    // this lane's channel of quad q: its span and the delay of the wave's DM
+    // (a wave past the tile's DMs loads the last DM's and does not use it, so
+    // neither load waits on the other)
+    const int dmw = td.x + min(wv, td.y - 1);
+    auto lane_delay = [&](int q, int2& spg, int& dly) {
+        spg = tspan[4 * q + g];
+        dly = delays[(size_t)dmw * nchans + 4 * q + g];
+    };
+
+    v4i sel = {0, 0, 0, 0};
+    sel[n >> 2] = 1 << (8 * (n & 3));
+    v4i acc[NTT][8];
+#pragma unroll
+    for(int tt = 0; tt < NTT; ++tt)
+#pragma unroll
+        for(int bg = 0; bg < 8; ++bg) acc[tt][bg] = v4i{0, 0, 0, 0};
+
+    auto products = [&](const int8_t* stg, int2 spc, int dc) {
+        if(!active) return;  // wave-uniform
+        const int8_t* base = stg + (g * RCAP + spc.y - dc + n * ts) * kMfPitch;
+#pragma unroll
+        for(int tt = 0; tt < NTT; ++tt)
+            for(int s = 0; s < ts; ++s) {
+                const int8_t* rp = base + (tt * 16 * ts + s) * kMfPitch;
+                v4i bv[8];  // the row's eight reads in flight together
+#pragma unroll
+                for(int bg = 0; bg < 8; ++bg) bv[bg] = *reinterpret_cast<const v4i*>(rp + bg * 16);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for(int bg = 0; bg < 8; ++bg)
+                    acc[tt][bg] = __builtin_amdgcn_mfma_i32_16x16x64_i8(sel, bv[bg], acc[tt][bg], 0, 0, 0);
+            }
+    };
+
+    // quads alternate between the buffers: even quads in stage0, odd in stage1
+    int2 spA, spB;
+    int dA, dB;
+    lane_delay(0, spA, dA);
+    issue(0, stage0);
+    for(int q = 0; q < nq; q += 2) {
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of quad q
+        __syncthreads();                // every wave's; quad q - 1's products are done, stage1 is free
+        if(q + 1 < nq) {
+            lane_delay(q + 1, spB, dB);
+            issue(q + 1, stage1);
+        }
+        products(stage0, spA, dA);
+        if(q + 1 < nq) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();  // quad q + 1 staged; quad q's products are done, stage0 is free
+            if(q + 2 < nq) {
+                lane_delay(q + 2, spA, dA);
+                issue(q + 2, stage0);
+            }
+            products(stage1, spB, dB);
+        }
+    }
+
+    // C lane (n, g), register r: beam b0 + 16 bg + 4 g + r at sample t0 + 16 tt + n
+    if(!active) return;
+#pragma unroll
+    for(int tt = 0; tt < NTT; ++tt) {
+        const int t = t0 + 16 * tt + n;
+        if(t >= nout) continue;
+        int8_t* o = tdb + ((size_t)t * ndms + td.x + wv) * W;
+#pragma unroll
+        for(int bg = 0; bg < 8; ++bg) {
+            const int b = b0 + 16 * bg + 4 * g;
+            if(b >= W) continue;
+            const v4i a = acc[tt][bg];
+            const uint32_t packed = (uint32_t)(uint8_t)id_quant(a[0], scale) |
+                                    ((uint32_t)(uint8_t)id_quant(a[1], scale) << 8) |
+                                    ((uint32_t)(uint8_t)id_quant(a[2], scale) << 16) |
+                                    ((uint32_t)(uint8_t)id_quant(a[3], scale) << 24);
+            *reinterpret_cast<uint32_t*>(o + b) = packed;
+        }
+    }
+}
+
 void free_plan_device(sw_idisp* p)
 {
     (void)hipFree(p->d_delays);
@@ -589,11 +753,12 @@ extern "C" int sw_idisp_create(const sw_idisp_config* cfg, sw_idisp** out)
     int cap = sw::kRowsCap, dmax_tile = 8;
     p->mfma = cfg->kernel == 0 && p->nchans % 4 == 0 && 16 * p->tscrunch <= sw::kMfRows;
     if(p->mfma) {
-        // 16 output samples per workgroup, no register prefetch: at most
-        // 128 VGPRs, so two workgroups share a CU and hide each other's loads
-        p->tt = 16;
+        // 32 output samples per workgroup of 16 waves (one DM each), the
+        // quads' rows double-buffered in LDS; 16 samples per workgroup of 8
+        // waves (two DMs each) where 32 samples' rows would not fit the stage
+        p->tt = 32 * p->tscrunch <= sw::kMfRows / 2 ? 32 : 16;
         cap = sw::kMfRows;
-        dmax_tile = sw::kMfWaves * sw::kMfDpw;
+        dmax_tile = p->tt == 32 ? sw::kDmaWaves : sw::kMfWaves * sw::kMfDpw;
     } else {
         p->wide = (long)p->nchans * p->tscrunch > 256;
         p->tt = p->wide ? 8 : 16;
@@ -657,7 +822,8 @@ extern "C" int sw_idisp_destroy(sw_idisp* p)
 extern "C" const char* sw_idisp_kernel(const sw_idisp* p)
 {
     if(!p) return "";
-    return p->mfma ? "idisp_mfma_kernel" : "idisp_kernel";
+    if(!p->mfma) return "idisp_kernel";
+    return p->tt == 32 ? "idisp_dma_kernel" : "idisp_mfma_kernel";
 }
 
 extern "C" int sw_idisp_info(const sw_idisp* p, int* max_delay, int* h_delays, float* scale)
@@ -728,7 +894,12 @@ int sw::idisp_launch(sw_idisp* p, const void* d_tfb, size_t nsamples, int nbeams
                        (long)nsamples, nout, p->scale, stride_t * 4, stride_f * 4)
             if(p->tt == 32) {
                 if(v16)
-                    SW_IDM_LAUNCH(2, true);
+                    hipLaunchKernelGGL((sw::idisp_dma_kernel<sw::kDmaWaves, 2, sw::kMfRows>), dim3(nblk),
+                                       dim3(sw::kDmaWaves * 64), 0, st, static_cast<const int8_t*>(d_tfb),
+                                       static_cast<int8_t*>(d_tdb), p->d_delays, p->d_span, p->d_tiles,
+                                       (int)ntiles, (int)nchunks128, (int)ntb128, w0, nw, p->nchans, (int)nbv,
+                                       p->ndms, p->tscrunch, p->max_delay, (long)nsamples, nout, p->scale,
+                                       stride_t * 4, stride_f * 4);
                 else
                     SW_IDM_LAUNCH(2, false);
             } else {

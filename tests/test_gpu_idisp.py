@@ -62,7 +62,10 @@ CASES = [
 def test_vs_oracle(sw, orc, dev, case, valu):
     F, B, nvec, ts, dms, nout, (fc, bw, cbts) = CASES[case]
     idd = sw.IncoherentDedisperser(F, fc, bw, cbts, dms, tscrunch=ts, valu_kernel=valu)
-    want_kernel = "idisp_mfma_kernel" if not valu and F % 4 == 0 and 16 * ts <= 128 else "idisp_kernel"
+    if valu or F % 4 or 16 * ts > 128:
+        want_kernel = "idisp_kernel"
+    else:
+        want_kernel = "idisp_dma_kernel" if 32 * ts <= 64 else "idisp_mfma_kernel"
     assert idd.kernel == want_kernel
     f = sw.channel_frequencies(fc, bw, F)
     d_o, mx_o = orc.id_delays(dms, f, cbts / (bw / F))

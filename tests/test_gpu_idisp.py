@@ -54,6 +54,9 @@ CASES = [
     (256, 32, 1, 1, np.arange(1024, dtype=np.float32) * np.float32(0.9765625), 24, (1284e6, 53.5e6, 8)),
     (64, 72, 1, 2, np.arange(0, 40, 2.5, dtype=np.float32), 37, (1284e6, 13.375e6, 16)),      # 18-word rows
     (32, 300, 1, 1, np.arange(0, 80, 4.0, dtype=np.float32), 21, (1284e6, 6.6875e6, 16)),     # 75 words, 2 chunks
+    (4, 48, 1, 1, np.arange(0, 400, 25.0, dtype=np.float32), 40, (1284e6, 13.375e6, 16)),    # one quad, part chunk
+    (8, 144, 1, 2, np.arange(0, 300, 10.0, dtype=np.float32), 29, (1284e6, 26.75e6, 8)),     # two quads, 128 + 16 B
+    (16, 64, 4, 1, np.float32([0, 1000]), 17, (1284e6, 53.5e6, 8)),                          # one DM per tile
 ]
 
 

@@ -501,6 +501,9 @@ idisp_dma_kernel(const int8_t* __restrict__ in, int8_t* __restrict__ tdb, const 
     constexpr int NBLK = 4 * SPC / 64;          // 1 KB blocks per quad
     constexpr int NB   = (NBLK + NW - 1) / NW;  // blocks per wave per quad (the last round partial)
     constexpr int QB   = 4 * RCAP * kMfPitch;
+    // two arrays, not one: the compiler then knows that a buffer's reads do
+    // not touch the loads in flight into the other (with one array it waits
+    // for every row load before any read)
     __shared__ __attribute__((aligned(16))) int8_t stage0[QB];
     __shared__ __attribute__((aligned(16))) int8_t stage1[QB];
 

@@ -274,8 +274,19 @@ struct FusedIb {
 // (sample t of column c = p A + a of channel f at ftpa + ((f 2A + c) ld + w0
 // + t) 2) and are corner-turned through LDS into the FTPA rows the operands
 // are read from; else ftpa is FTPA [F][T][2][A].
+//
+// Stokes I at tscrunch 8 with 64 antennas and 4 waves takes 176 VGPRs
+// unconstrained, two waves per SIMD; asked for three (168 VGPRs, two values
+// spilled) it runs three workgroups per CU, which its LDS allows.  Other
+// shapes keep the compiler's allocation (three waves would spill them far
+// more).
+template <int S, int TS, int NSTEPS, int WV>
+constexpr int cb_min_waves()
+{
+    return S == SW_STOKES_I && TS == 8 && NSTEPS == 4 && WV == 4 ? 3 : 1;
+}
 template <int S, int TS, int NSTEPS, bool TMI, int WV>
-__global__ void __launch_bounds__(WV * 64)
+__global__ void __launch_bounds__(WV * 64, (cb_min_waves<S, TS, NSTEPS, WV>()))
 cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ packed,
                int8_t* __restrict__ out, const float* __restrict__ scale,
                const float* __restrict__ offset, const int* __restrict__ bmap,

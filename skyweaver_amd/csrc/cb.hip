@@ -41,6 +41,30 @@ __device__ __forceinline__ uint32_t im_bytes(uint32_t d0, uint32_t d1)
 }
 
 // ------------------------------------------------------------ epilogue
+// The FP32 division x / d as the compiler expands it (two v_div_scale, v_rcp,
+// five FMAs, v_div_fmas, v_div_fixup) reduces, when its scale steps are
+// identities, to five FMA-type steps on the divisor's refined reciprocal
+// r1 = fma(fma(-d, r0, 1), r0, r0), r0 = rcp(d): bit for bit the same
+// quotient.  The scale steps are identities for d in [2^-40, 2^60]
+// (kFastDivLo, kFastDivHi) and quotients within [2^-100, 2^90]: every
+// numerator here is an integer-valued power below 2^50, its difference with
+// a float offset, or 0 (+-0 gives +0 where the expansion gives +-0: the same
+// int8 after rint and the clamp).  The caller checks the divisors.
+constexpr float kFastDivLo = 0x1p-40f, kFastDivHi = 0x1p60f;
+__device__ __forceinline__ float div_recip(float d)
+{
+    const float r0 = __builtin_amdgcn_rcpf(d);
+    return __fmaf_rn(__fmaf_rn(-d, r0, 1.0f), r0, r0);
+}
+__device__ __forceinline__ float div_by(float x, float d, float r1)
+{
+    const float q0 = x * r1;
+    const float e0 = __fmaf_rn(-d, q0, x);
+    const float q1 = __fmaf_rn(e0, r1, q0);
+    const float e1 = __fmaf_rn(-d, q1, x);
+    return __fmaf_rn(e1, r1, q1);
+}
+
 template <int S>
 struct Epi {
     static constexpr int NV = nstokes(S);
@@ -100,12 +124,21 @@ struct Epi {
         }
     }
     // store_pre of a single-Stokes output through a buffer resource, at
-    // byte offset vo + so
+    // byte offset vo + so; FD: the division by sc is div_by with r1
+    template <bool FD>
     __device__ __forceinline__ static void store_pre_buf(const float* power, __amdgpu_buffer_rsrc_t rs, int vo,
-                                                         int so, const float* ib, float sc, float off, int ib_sub)
+                                                         int so, const float* ib, float sc, float r1, float off,
+                                                         int ib_sub)
     {
         static_assert(NV == 1, "single Stokes outputs");
-        const float r = ib_sub ? ib_subtract(power[0], ib[0], sc) : rescale<S>(power[0], off, sc);
+        float r;
+        if constexpr(FD) {
+            // the numerators of ib_subtract and rescale, in their operation order
+            const float x = ib_sub ? power[0] - ib[0] * 16129.0f : (S == SW_STOKES_I ? power[0] - off : power[0]);
+            r = rintf(div_by(x, sc, r1));
+        } else {
+            r = ib_sub ? ib_subtract(power[0], ib[0], sc) : rescale<S>(power[0], off, sc);
+        }
         __builtin_amdgcn_raw_buffer_store_b8((uint8_t)clamp_i8(r), rs, vo, so, 0);
     }
     // As store(), with the beamset's scale / offset and the incoherent-beam
@@ -253,7 +286,7 @@ template <int S, int NOUT>
 struct EpiParams {
     static constexpr int NV = nstokes(S);
     static constexpr bool kPreIb = NOUT * NV <= 4;
-    float sc = 0, off = 0;
+    float sc = 0, off = 0, r1 = 0;
     float ib[kPreIb ? NOUT : 1][NV];
     int s = 0;
 };
@@ -285,7 +318,10 @@ constexpr int cb_min_waves()
 {
     return S == SW_STOKES_I && TS == 8 && NSTEPS == 4 && WV == 4 ? 3 : 1;
 }
-template <int S, int TS, int NSTEPS, bool TMI, int WV>
+//
+// FD: every scale lies in [kFastDivLo, kFastDivHi] (checked by the caller),
+// so the requantisation divides with div_by.
+template <int S, int TS, int NSTEPS, bool TMI, int WV, bool FD = false>
 __global__ void __launch_bounds__(WV * 64, (cb_min_waves<S, TS, NSTEPS, WV>()))
 cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ packed,
                int8_t* __restrict__ out, const float* __restrict__ scale,
@@ -316,6 +352,7 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
     constexpr int NO_WG = 32 * WV / (TS <= 32 ? TS : 32);  // output samples of the workgroup
     __shared__ int tb_s[kPre ? kBeamLds : 1];
     __shared__ float tb_sc[kPre ? kBeamLds : 1], tb_off[kPre ? kBeamLds : 1];
+    __shared__ float tb_r1[kPre && FD ? kBeamLds : 1];  // div_recip of the scales
     __shared__ float tb_ib[kPre ? 16 * NO_WG * NV : 1];
     // XCD-aware mapping: consecutive workgroups go round-robin over the 8
     // XCDs; keep all time chunks of a channel on one XCD so its packed
@@ -390,6 +427,7 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
                     tb_s[b]   = sbv[k];
                     tb_sc[b]  = scv[k];
                     tb_off[b] = ofv[k];
+                    if constexpr(FD) tb_r1[b] = div_recip(scv[k]);
                 }
             }
             const int o0 = tc * WV * 32 / TS;
@@ -610,6 +648,7 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
                 if(active && bvalid) {
                     ep.s = tb_s[b];
                     ep.sc = tb_sc[b];
+                    if constexpr(FD) ep.r1 = tb_r1[b];
                     ep.off = tb_off[b];
 #pragma unroll
                     for(int o = 0; o < NOUT; ++o) {
@@ -623,6 +662,7 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
             ep.s = bmap[b];
             const int so = ep.s * nchans + f;
             ep.sc = scale[so];
+            if constexpr(FD) ep.r1 = div_recip(ep.sc);
             if(!ib_sub) ep.off = offset[so];
             if constexpr(EpiParams<S, NOUT>::kPreIb) {
                 if(ib_sub) {
@@ -676,7 +716,8 @@ cb_mfma_kernel(const uint8_t* __restrict__ ftpa, const uint8_t* __restrict__ pac
                 if(active && bvalid) {
                     const int tout = (t0 + 16 * grp + o * TS) / TS;
                     if constexpr(kBufOut)
-                        Epi<S>::store_pre_buf(power, ors, ovo[o], 32 * bt, ep.ib[o], ep.sc, ep.off, ib_sub);
+                        Epi<S>::template store_pre_buf<FD>(power, ors, ovo[o], 32 * bt, ep.ib[o], ep.sc, ep.r1,
+                                                           ep.off, ib_sub);
                     else if constexpr(EpiParams<S, NOUT>::kPreIb)
                         Epi<S>::store_pre(power, out, ep.ib[o], ep.sc, ep.off, tout, f, b, nchans, nbeams,
                                           ib_sub);
@@ -751,7 +792,7 @@ constexpr size_t cb_mfma_lds_bytes()
     constexpr int NOUT = TS <= 16 ? 16 / TS : 1;
     constexpr bool kPre = NOUT * NV <= 4;
     constexpr int NO_WG = 32 * WV / (TS <= 32 ? TS : 32);
-    return (size_t)16 * (2 * WL16 > ST16 ? 2 * WL16 : ST16) + (kPre ? (size_t)kBeamLds * 12 : 12) +
+    return (size_t)16 * (2 * WL16 > ST16 ? 2 * WL16 : ST16) + (kPre ? (size_t)kBeamLds * 16 : 16) +
            (kPre ? (size_t)16 * NO_WG * NV * 4 : 4);
 }
 
@@ -764,14 +805,14 @@ constexpr bool cb_wave_variants()
     return S == SW_STOKES_I || (S == SW_STOKES_IQUV && !TMI);
 }
 
-template <int S, int TS, int NS, bool TMI, int WV>
+template <int S, int TS, int NS, bool TMI, int WV, bool FD = false>
 int launch_mfma_wv(const uint8_t* ftpa, const uint8_t* packed, int8_t* out, const float* sc, const float* off,
                    const int* bmap, const float* ib, const sw_bf_dims& d, hipStream_t st, long ld, long w0,
                    const FusedIb& fib)
 {
     if constexpr(cb_mfma_lds_bytes<S, TS, NS, TMI, WV>() <= 163840) {
         const int ntchunks = (d.nsamples + 32 * WV - 1) / (32 * WV);
-        hipLaunchKernelGGL((cb_mfma_kernel<S, TS, NS, TMI, WV>), dim3(ntchunks * d.nchans), dim3(WV * 64), 0, st,
+        hipLaunchKernelGGL((cb_mfma_kernel<S, TS, NS, TMI, WV, FD>), dim3(ntchunks * d.nchans), dim3(WV * 64), 0, st,
                            ftpa, packed, out, sc, off, bmap, ib, d.nchans, d.nsamples, d.nbeams, ntchunks,
                            d.ib_subtract, d.nbeamsets, ld, w0, fib);
         SW_LAUNCH_CHECK();
@@ -897,7 +938,8 @@ namespace sw {
 int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_packed, const float* d_scale,
                           const float* d_offset, const int* d_beamset_map, const float* d_ib_raw, void* d_out,
                           const sw_bf_dims& d, hipStream_t st, float* d_ib_out_raw, void* d_ib_out,
-                          const float* d_ib_scale, const float* d_ib_offset, const float* d_ant_weights, int waves)
+                          const float* d_ib_scale, const float* d_ib_offset, const float* d_ant_weights, int waves,
+                          bool fastdiv)
 {
     int rc = check_dims(d);
     if(rc) return rc;
@@ -916,9 +958,19 @@ int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_pa
         SW_CHECK_ARG(d_ib_out && d_ib_scale && d_ib_offset && d_ant_weights, "null buffer");
         fib = FusedIb{d_ib_out_raw, static_cast<int8_t*>(d_ib_out), d_ib_scale, d_ib_offset, d_ant_weights};
     }
-    return launch_mfma<SW_STOKES_I, true>(static_cast<const uint8_t*>(d_ring), static_cast<const uint8_t*>(d_packed),
-                                          static_cast<int8_t*>(d_out), d_scale, d_offset, d_beamset_map, d_ib_raw, d,
-                                          st, ld, w0, fib, waves);
+    const auto* ring = static_cast<const uint8_t*>(d_ring);
+    const auto* pk = static_cast<const uint8_t*>(d_packed);
+    auto* out = static_cast<int8_t*>(d_out);
+    // every scale checked by the caller: the division by div_by, for the
+    // 64-antenna, 4-wave shapes at tscrunch 16 and 8
+    if(fastdiv && waves == kMfmaWaves && d.nantennas == 64 && d.tscrunch == 16)
+        return launch_mfma_wv<SW_STOKES_I, 16, 4, true, kMfmaWaves, true>(ring, pk, out, d_scale, d_offset,
+                                                                          d_beamset_map, d_ib_raw, d, st, ld, w0, fib);
+    if(fastdiv && waves == kMfmaWaves && d.nantennas == 64 && d.tscrunch == 8)
+        return launch_mfma_wv<SW_STOKES_I, 8, 4, true, kMfmaWaves, true>(ring, pk, out, d_scale, d_offset,
+                                                                         d_beamset_map, d_ib_raw, d, st, ld, w0, fib);
+    return launch_mfma<SW_STOKES_I, true>(ring, pk, out, d_scale, d_offset, d_beamset_map, d_ib_raw, d, st, ld, w0,
+                                          fib, waves);
 }
 
 // sw_cb_beamform_packed with `waves` waves (32 time rows each) per workgroup

@@ -47,7 +47,8 @@ int ib_beamform_tm(const void* d_ring, long ld, long w0, float* d_raw, void* d_o
 int cb_beamform_packed_tm(const void* d_ring, long ld, long w0, const void* d_packed, const float* d_scale,
                           const float* d_offset, const int* d_beamset_map, const float* d_ib_raw, void* d_out,
                           const sw_bf_dims& d, hipStream_t st, float* d_ib_out_raw, void* d_ib_out,
-                          const float* d_ib_scale, const float* d_ib_offset, const float* d_ant_weights, int waves);
+                          const float* d_ib_scale, const float* d_ib_offset, const float* d_ant_weights, int waves,
+                          bool fastdiv);
 int cb_beamform_packed_waves(const void* d_ftpa, const void* d_packed, const float* d_scale, const float* d_offset,
                              const int* d_beamset_map, const float* d_ib_raw, void* d_out, const sw_bf_dims& d,
                              hipStream_t st, int waves);
@@ -182,6 +183,7 @@ struct sw_pipeline {
     bool weights_set   = false;
     bool use_mfma      = false;
     bool scalings_dirty = true;    // recompute the scalings from this gulp's statistics
+    bool cb_fastdiv = false;       // every coherent-beam scale lies where the beamformer may divide with div_by
     int cb_waves = 4;              // MFMA beamformer workgroup shape (waves of 32 time rows)
     int tile_source = 0;           // sw_pipeline_tile's *source
     std::string tuning_file;
@@ -693,7 +695,7 @@ int enqueue_beams(sw_pipeline* p, int dm, bool ring_direct, bool fuse_ib, hipStr
     if(ring_direct)
         return sw::cb_beamform_packed_tm(ring, ld, w0, p->packed, cb_scaling, cb_offsets, p->bmap, p->ib_raw,
                                          p->cb_out, d, st, fuse_ib ? p->ib_raw : nullptr, p->ib_q, ib_scaling,
-                                         ib_offsets, p->antw, p->cb_waves);
+                                         ib_offsets, p->antw, p->cb_waves, p->cb_fastdiv);
     if(p->use_mfma)
         return sw::cb_beamform_packed_waves(p->ftpa_dd[dm], p->packed, cb_scaling, cb_offsets, p->bmap, p->ib_raw,
                                             p->cb_out, d, st, p->cb_waves);
@@ -892,6 +894,11 @@ int run_gulp(sw_pipeline* p, const void* in, hipEvent_t consumed)
                                 c.output_level, h.data(), h.data() + S * F, h.data() + 2 * S * F,
                                 h.data() + 3 * S * F);
         if(rc) return rc;
+        // the beamformer divides with a staged reciprocal when every coherent
+        // scale lies in [2^-40, 2^60] (csrc/cb.hip, div_by)
+        p->cb_fastdiv = true;
+        for(size_t i = (size_t)S * F; i < 2 * (size_t)S * F; ++i)
+            p->cb_fastdiv = p->cb_fastdiv && h[i] >= 0x1p-40f && h[i] <= 0x1p60f;
         SW_HIP_TRY(hipMemcpyAsync(p->scl, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, st));
         SW_HIP_TRY(hipStreamSynchronize(st));  // h is freed at the end of this scope
         p->scalings_dirty = false;
